@@ -1,0 +1,470 @@
+"""Python mirror of the reference's sigmo:: API over the C ABI (libbms2.so).
+
+Names and argument meaning follow library/include of AnySparse/BMS2:
+  Queue                   sycl::queue                          (BMS2.cpp:14)
+  create_device_csr_graph createDeviceCSRGraph                 (graph.hpp:251-310)
+  Signature               signature::Signature<PowerGraph, 4>  (signature.hpp:18-390)
+  PathSignature           signature::PathSignature             (signature.hpp:406-532)
+  CycleSignature          signature::CycleSignature            (signature.hpp:535-732)
+  Candidates              candidates::Candidates               (candidates.hpp:10-200)
+  filter_candidates       isomorphism::filter::filterCandidates  (isomorphism.hpp:30-104)
+  refine_candidates       isomorphism::filter::refineCandidates  (isomorphism.hpp:106-203)
+  GMCR                    isomorphism::mapping::GMCR           (gmcr.hpp:16-145)
+  join_candidates         isomorphism::join::joinCandidates    (isomorphism.hpp:291-472)
+  Pipeline                the fused stage sequence of BMS2.cpp:131-234
+Every call goes to HIP kernels; errors raise Bms2Error with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import Bms2Error, check
+from .graphs import GraphBatch
+
+SCOPE_DATA = _lib.SCOPE_DATA
+SCOPE_QUERY = _lib.SCOPE_QUERY
+
+
+def _L():
+    return _lib.lib()
+
+
+class Event:
+    """utils::BatchedEvent: elapsed device time of one call's kernels."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def elapsed_ms(self) -> float:
+        ms = C.c_float()
+        check(_L().bms2_event_elapsed_ms(self._h, C.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _L().bms2_event_destroy(self._h)
+            self._h = None
+
+
+def _ev_out(want: bool):
+    return C.c_void_p() if want else None
+
+
+def _ev(ptr):
+    return Event(ptr.value) if ptr is not None and ptr.value else None
+
+
+class Queue:
+    """A HIP stream on one device (sycl::queue)."""
+
+    def __init__(self, device: int = 0, stream: int | None = None):
+        h = C.c_void_p()
+        if stream is None:
+            check(_L().bms2_queue_create(device, C.byref(h)))
+        else:
+            check(_L().bms2_queue_wrap(device, C.c_void_p(stream), C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def wait(self) -> None:
+        check(_L().bms2_queue_wait(self.h))
+
+    @property
+    def stream(self) -> int:
+        return int(_L().bms2_queue_stream(self.h) or 0)
+
+    def close(self) -> None:
+        if self.h:
+            check(_L().bms2_queue_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """Plain device allocation (u8 bytes) owned by Python."""
+
+    def __init__(self, q: Queue, nbytes: int, zero: bool = True):
+        self.q = q
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(_L().bms2_device_malloc(q.h, max(self.nbytes, 8), C.byref(p)))
+        self.ptr = p.value
+        if zero:
+            check(_L().bms2_memset_zero(q.h, C.c_void_p(self.ptr), max(self.nbytes, 8)))
+
+    @staticmethod
+    def from_array(q: Queue, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = DeviceBuffer(q, a.nbytes, zero=False)
+        check(_L().bms2_memcpy_h2d(q.h, C.c_void_p(b.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes))
+        q.wait()
+        return b
+
+    def to_numpy(self, dtype, count: int | None = None) -> np.ndarray:
+        dt = np.dtype(dtype)
+        n = self.nbytes // dt.itemsize if count is None else count
+        out = np.empty(n, dtype=dt)
+        if n:
+            check(_L().bms2_memcpy_d2h(self.q.h, out.ctypes.data_as(C.c_void_p),
+                                       C.c_void_p(self.ptr), n * dt.itemsize))
+        return out
+
+    def zero(self) -> None:
+        check(_L().bms2_memset_zero(self.q.h, C.c_void_p(self.ptr), max(self.nbytes, 8)))
+
+    def free(self) -> None:
+        if self.ptr:
+            check(_L().bms2_device_free(self.q.h, C.c_void_p(self.ptr)))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceBatchedCSRGraph:
+    """sigmo::DeviceBatchedCSRGraph (graph.hpp:58-108) created by createDeviceCSRGraph."""
+
+    def __init__(self, q: Queue, host: GraphBatch):
+        self.q = q
+        self.c = _lib.DeviceCsr()
+        h = host.to_c()
+        check(_L().bms2_device_csr_create(q.h, C.byref(h), C.byref(self.c)))
+        self.host = host
+
+    @property
+    def num_graphs(self) -> int:
+        return int(self.c.num_graphs)
+
+    @property
+    def total_nodes(self) -> int:
+        return int(self.c.total_nodes)
+
+    @property
+    def total_edges(self) -> int:
+        return int(self.c.total_edges)
+
+    def alloc_size(self) -> int:
+        return int(_L().bms2_device_csr_alloc_size(C.byref(self.c)))
+
+    def destroy(self) -> None:
+        if self.c.graph_offsets:
+            check(_L().bms2_device_csr_destroy(self.q.h, C.byref(self.c)))
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def create_device_csr_graph(q: Queue, host: GraphBatch) -> DeviceBatchedCSRGraph:
+    return DeviceBatchedCSRGraph(q, host)
+
+
+class _Handle:
+    _destroy = ""
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            try:
+                getattr(_L(), self._destroy)(h)
+            except Exception:
+                pass
+            self.h = None
+
+
+class Signature(_Handle):
+    """Signature<PowerGraph, 4>(queue, data_nodes, query_nodes) (signature.hpp:156-167)."""
+
+    _destroy = "bms2_signature_destroy"
+
+    def __init__(self, q: Queue, data_nodes: int, query_nodes: int):
+        self.q, self.n = q, (int(data_nodes), int(query_nodes))
+        h = C.c_void_p()
+        check(_L().bms2_signature_create(q.h, data_nodes, query_nodes, C.byref(h)))
+        self.h = h
+
+    def _gen(self, g, scope, event):
+        e = _ev_out(event)
+        check(_L().bms2_signature_generate(self.h, C.byref(g.c), scope,
+                                           C.byref(e) if e is not None else None))
+        return _ev(e)
+
+    def _ref(self, g, view_size, scope, event):
+        e = _ev_out(event)
+        check(_L().bms2_signature_refine(self.h, C.byref(g.c), view_size, scope,
+                                         C.byref(e) if e is not None else None))
+        return _ev(e)
+
+    def generate_data_signatures(self, g, event=False):
+        return self._gen(g, SCOPE_DATA, event)
+
+    def generate_query_signatures(self, g, event=False):
+        return self._gen(g, SCOPE_QUERY, event)
+
+    def refine_data_signatures(self, g, view_size=1, event=False):
+        return self._ref(g, view_size, SCOPE_DATA, event)
+
+    def refine_query_signatures(self, g, view_size=1, event=False):
+        return self._ref(g, view_size, SCOPE_QUERY, event)
+
+    def host(self, scope: int) -> np.ndarray:
+        out = np.zeros(self.n[scope], np.uint64)
+        check(_L().bms2_signature_copy_to_host(self.h, scope, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+
+class PathSignature(_Handle):
+    _destroy = "bms2_path_signature_destroy"
+
+    def __init__(self, q: Queue, data_nodes: int, query_nodes: int):
+        self.q, self.n = q, (int(data_nodes), int(query_nodes))
+        h = C.c_void_p()
+        check(_L().bms2_path_signature_create(q.h, data_nodes, query_nodes, C.byref(h)))
+        self.h = h
+
+    def _gen(self, g, scope, event):
+        e = _ev_out(event)
+        check(_L().bms2_path_signature_generate(self.h, C.byref(g.c), scope,
+                                                C.byref(e) if e is not None else None))
+        return _ev(e)
+
+    def generate_data_path_signatures(self, g, event=False):
+        return self._gen(g, SCOPE_DATA, event)
+
+    def generate_query_path_signatures(self, g, event=False):
+        return self._gen(g, SCOPE_QUERY, event)
+
+    def host(self, scope: int) -> np.ndarray:
+        out = np.zeros((self.n[scope], 4), np.uint64)
+        check(_L().bms2_path_signature_copy_to_host(self.h, scope, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+
+class CycleSignature(_Handle):
+    _destroy = "bms2_cycle_signature_destroy"
+
+    def __init__(self, q: Queue, data_nodes: int, query_nodes: int):
+        self.q, self.n = q, (int(data_nodes), int(query_nodes))
+        h = C.c_void_p()
+        check(_L().bms2_cycle_signature_create(q.h, data_nodes, query_nodes, C.byref(h)))
+        self.h = h
+
+    def _gen(self, g, scope, event):
+        e = _ev_out(event)
+        check(_L().bms2_cycle_signature_generate(self.h, C.byref(g.c), scope,
+                                                 C.byref(e) if e is not None else None))
+        return _ev(e)
+
+    def generate_data_cycle_signatures(self, g, event=False):
+        return self._gen(g, SCOPE_DATA, event)
+
+    def generate_query_cycle_signatures(self, g, event=False):
+        return self._gen(g, SCOPE_QUERY, event)
+
+    def host(self, scope: int) -> np.ndarray:
+        out = np.zeros(self.n[scope], np.uint64)
+        check(_L().bms2_cycle_signature_copy_to_host(self.h, scope, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+
+class Candidates(_Handle):
+    """candidates::Candidates(queue, source_nodes, target_nodes), zero-filled."""
+
+    _destroy = "bms2_candidates_destroy"
+
+    def __init__(self, q: Queue, source_nodes: int, target_nodes: int):
+        self.q = q
+        self.rows = int(source_nodes)
+        h = C.c_void_p()
+        check(_L().bms2_candidates_create(q.h, source_nodes, target_nodes, C.byref(h)))
+        self.h = h
+        self.W = int(_L().bms2_candidates_words_per_row(h))
+
+    def host(self) -> np.ndarray:
+        out = np.zeros((self.rows, self.W), np.uint64)
+        check(_L().bms2_candidates_copy_to_host(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def row_counts(self) -> np.ndarray:
+        out = np.zeros(self.rows, np.uint32)
+        check(_L().bms2_candidates_row_counts(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+
+def filter_candidates(q: Queue, query, data, sig: Signature, path: PathSignature,
+                      cycle: CycleSignature, cand: Candidates, event=False):
+    e = _ev_out(event)
+    check(_L().bms2_filter_candidates(q.h, C.byref(query.c), C.byref(data.c), sig.h, path.h,
+                                      cycle.h, cand.h, C.byref(e) if e is not None else None))
+    return _ev(e)
+
+
+def refine_candidates(q: Queue, query, data, sig: Signature, cand: Candidates, event=False):
+    e = _ev_out(event)
+    check(_L().bms2_refine_candidates(q.h, C.byref(query.c), C.byref(data.c), sig.h, cand.h,
+                                      C.byref(e) if e is not None else None))
+    return _ev(e)
+
+
+class GMCR(_Handle):
+    _destroy = "bms2_gmcr_destroy"
+
+    def __init__(self, q: Queue):
+        self.q = q
+        h = C.c_void_p()
+        check(_L().bms2_gmcr_create(q.h, C.byref(h)))
+        self.h = h
+        self.num_data = 0
+
+    def generate_gmcr(self, query, data, cand: Candidates, event=False):
+        e = _ev_out(event)
+        check(_L().bms2_gmcr_generate(self.h, C.byref(query.c), C.byref(data.c), cand.h,
+                                      C.byref(e) if e is not None else None))
+        self.num_data = data.num_graphs
+        return _ev(e)
+
+    @property
+    def total(self) -> int:
+        return int(_L().bms2_gmcr_total(self.h))
+
+    def host(self) -> tuple[np.ndarray, np.ndarray]:
+        off = np.zeros(self.num_data + 1, np.uint32)
+        idx = np.zeros(max(self.total, 1), np.uint32)
+        check(_L().bms2_gmcr_copy_to_host(self.h, off.ctypes.data_as(C.c_void_p),
+                                          idx.ctypes.data_as(C.c_void_p)))
+        return off, idx[:self.total]
+
+
+def join_candidates(q: Queue, query, data, cand: Candidates, gmcr: GMCR, num_matches: DeviceBuffer,
+                    find_first: bool = True, pair_counts: DeviceBuffer | None = None, event=False):
+    """joinCandidates: adds the match total into the device u64 `num_matches`."""
+    e = _ev_out(event)
+    check(_L().bms2_join_candidates(q.h, C.byref(query.c), C.byref(data.c), cand.h, gmcr.h,
+                                    C.c_void_p(num_matches.ptr), int(find_first),
+                                    C.c_void_p(pair_counts.ptr) if pair_counts else None,
+                                    C.byref(e) if e is not None else None))
+    return _ev(e)
+
+
+class Pipeline(_Handle):
+    """The fused stage sequence over a resident data batch."""
+
+    _destroy = "bms2_pipeline_destroy"
+
+    def __init__(self, q: Queue, query, data, refinement_steps: int = 0, find_first: bool = False):
+        self.q = q
+        self.Q = query.num_graphs
+        self.D = data.num_graphs
+        o = _lib.PipelineOptions()
+        o.refinement_steps = refinement_steps
+        o.find_first = int(find_first)
+        h = C.c_void_p()
+        check(_L().bms2_pipeline_create(q.h, C.byref(query.c), C.byref(data.c), C.byref(o),
+                                        C.byref(h)))
+        self.h = h
+        self.VW = (self.Q + 63) // 64
+
+    def alloc_outputs(self) -> tuple[DeviceBuffer, DeviceBuffer, DeviceBuffer]:
+        counts = DeviceBuffer(self.q, 8 * self.D * self.Q, zero=False)
+        valid = DeviceBuffer(self.q, 8 * self.D * self.VW, zero=False)
+        stats = DeviceBuffer(self.q, 64, zero=True)
+        return counts, valid, stats
+
+    def run(self, counts: DeviceBuffer, valid: DeviceBuffer, stats: DeviceBuffer | None = None,
+            event=False, kernel_event=False):
+        """Enqueues one run.  Returns the whole-call Event (event=True), or the pair
+        (whole-call Event, fused-kernel Event) when kernel_event=True."""
+        e = _ev_out(event or kernel_event)
+        k = _ev_out(kernel_event)
+        check(_L().bms2_pipeline_run(self.h, C.c_void_p(counts.ptr), C.c_void_p(valid.ptr),
+                                     C.c_void_p(stats.ptr) if stats else None,
+                                     C.byref(e) if e is not None else None,
+                                     C.byref(k) if k is not None else None))
+        if kernel_event:
+            return _ev(e), _ev(k)
+        return _ev(e)
+
+    def results(self, counts: DeviceBuffer, valid: DeviceBuffer):
+        c = counts.to_numpy(np.uint64, self.D * self.Q).reshape(self.D, self.Q)
+        v = valid.to_numpy(np.uint64, self.D * self.VW).reshape(self.D, self.VW)
+        return c, v
+
+
+def run_staged(q: Queue, query: GraphBatch, data: GraphBatch, K: int, find_first: bool = False,
+               keep_domains: bool = True):
+    """The stage sequence of BMS2.cpp:131-234 through the staged entry points.
+
+    Returns dict(domains=[iteration 0..K], gmcr=(offsets, indices), counts=per GMCR entry,
+    total=match total, signatures=...)."""
+    dq = DeviceBatchedCSRGraph(q, query)
+    dd = DeviceBatchedCSRGraph(q, data)
+    sig = Signature(q, dd.total_nodes, dq.total_nodes)
+    path = PathSignature(q, dd.total_nodes, dq.total_nodes)
+    cyc = CycleSignature(q, dd.total_nodes, dq.total_nodes)
+    cand = Candidates(q, dq.total_nodes, dd.total_nodes)
+    sig.generate_data_signatures(dd)
+    path.generate_data_path_signatures(dd)
+    cyc.generate_data_cycle_signatures(dd)
+    sig.generate_query_signatures(dq)
+    path.generate_query_path_signatures(dq)
+    cyc.generate_query_cycle_signatures(dq)
+    q.wait()
+    out = dict(domains=[], signatures=dict(
+        dlabel=sig.host(SCOPE_DATA), qlabel=sig.host(SCOPE_QUERY), dpath=path.host(SCOPE_DATA),
+        qpath=path.host(SCOPE_QUERY), dcycle=cyc.host(SCOPE_DATA), qcycle=cyc.host(SCOPE_QUERY)))
+    filter_candidates(q, dq, dd, sig, path, cyc, cand)
+    if keep_domains:
+        out["domains"].append(cand.host())
+    for k in range(1, K + 1):
+        sig.refine_data_signatures(dd, k)
+        sig.refine_query_signatures(dq, k)
+        refine_candidates(q, dq, dd, sig, cand)
+        if keep_domains:
+            out["domains"].append(cand.host())
+            out["signatures"][f"dsig{k}"] = sig.host(SCOPE_DATA)
+            out["signatures"][f"qsig{k}"] = sig.host(SCOPE_QUERY)
+    q.wait()
+    out["final_domains"] = cand.host()
+    gm = GMCR(q)
+    gm.generate_gmcr(dq, dd, cand)
+    out["gmcr"] = gm.host()
+    total = DeviceBuffer(q, 8, zero=True)
+    per_pair = DeviceBuffer(q, 8 * max(gm.total, 1), zero=True)
+    join_candidates(q, dq, dd, cand, gm, total, find_first, per_pair)
+    q.wait()
+    out["total"] = int(total.to_numpy(np.uint64)[0])
+    out["counts"] = per_pair.to_numpy(np.uint64, gm.total)
+    return out
+
+
+def run_pipeline(q: Queue, query: GraphBatch, data: GraphBatch, K: int, find_first: bool = False):
+    """Fused pipeline: (counts [D][Q] u64, valid bit rows [D][ceil(Q/64)], stats[8])."""
+    dq = DeviceBatchedCSRGraph(q, query)
+    dd = DeviceBatchedCSRGraph(q, data)
+    p = Pipeline(q, dq, dd, K, find_first)
+    counts, valid, stats = p.alloc_outputs()
+    p.run(counts, valid, stats)
+    q.wait()
+    c, v = p.results(counts, valid)
+    return c, v, stats.to_numpy(np.uint64, 8)
+
+
+__all__ = [
+    "Bms2Error", "Queue", "DeviceBuffer", "DeviceBatchedCSRGraph", "create_device_csr_graph",
+    "Signature", "PathSignature", "CycleSignature", "Candidates", "filter_candidates",
+    "refine_candidates", "GMCR", "join_candidates", "Pipeline", "run_staged", "run_pipeline",
+    "SCOPE_DATA", "SCOPE_QUERY",
+]

@@ -1,0 +1,36 @@
+// Shared host-side helpers of the bms2 library: error reporting and small utilities.
+#pragma once
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/bms2.h"
+
+namespace bms2 {
+
+// Records a formatted message for bms2_last_error() and returns `code`.
+int set_error(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Host-side view of a packed batch (host or device pointers, same layout).
+struct CsrView {
+  uint32_t num_graphs;
+  uint64_t total_nodes;
+  uint64_t total_edges;
+  const uint32_t* graph_offsets;
+  const uint32_t* row_offsets;
+  const uint32_t* column_indices;
+  const uint8_t* node_labels;
+  const uint8_t* edge_labels;
+};
+
+// Checks the structural invariants every kernel relies on: monotone offsets, column ids inside
+// their own graph, sizes within BMS2_MAX_GRAPH_NODES.  Returns BMS2_OK or an error code with the
+// message set.  `max_nodes_out` receives the largest graph size.
+int validate_host_csr(const bms2_host_csr* g, uint32_t max_graph_nodes, uint32_t* max_nodes_out);
+
+}  // namespace bms2

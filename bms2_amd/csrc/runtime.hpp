@@ -1,0 +1,49 @@
+// Queues, events and HIP error plumbing shared by the C ABI implementation files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+struct bms2_queue_s {
+  int device;
+  hipStream_t stream;
+  bool owned;
+};
+
+struct bms2_event_s {
+  hipEvent_t start;
+  hipEvent_t stop;
+};
+
+namespace bms2 {
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define BMS2_HIP(call)                                                              \
+  do {                                                                              \
+    hipError_t bms2_e_ = (call);                                                    \
+    if (bms2_e_ != hipSuccess) return ::bms2::hip_fail(bms2_e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define BMS2_CHECK_LAUNCH(name)                                                     \
+  do {                                                                              \
+    hipError_t bms2_e_ = hipGetLastError();                                        \
+    if (bms2_e_ != hipSuccess) return ::bms2::hip_fail(bms2_e_, name, __FILE__, __LINE__); \
+  } while (0)
+
+// Makes the queue's device current.
+int use_device(bms2_queue q);
+
+// Brackets a stage call with an event pair when `ev` is non-null.
+struct EventScope {
+  bms2_queue q;
+  bms2_event* out;
+  bms2_event_s* ev = nullptr;
+  int begin();
+  int end();
+};
+
+int device_alloc(void** p, uint64_t bytes);
+
+}  // namespace bms2

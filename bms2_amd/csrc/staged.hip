@@ -1,0 +1,957 @@
+// Staged entry points with the reference's per-stage contracts (signature.hpp, candidates.hpp,
+// isomorphism.hpp, gmcr.hpp): each call materialises its stage's output in HBM, so callers can
+// inspect the domain bit-matrix after every refinement step.  Kernels here favour exactness for
+// any graph up to 256 nodes over speed; the fused pipeline is the fast path.
+#include <algorithm>
+#include <vector>
+
+#include "device_common.hpp"
+#include "runtime.hpp"
+
+namespace bms2 {
+
+// ------------------------------------------------------------------ small bitsets (<= 256 nodes)
+struct Bits4 {
+  uint64_t w[4];
+  __device__ void clear() { w[0] = w[1] = w[2] = w[3] = 0; }
+  __device__ void set(uint32_t i) { w[i >> 6] |= uint64_t(1) << (i & 63); }
+  __device__ bool get(uint32_t i) const { return (w[i >> 6] >> (i & 63)) & 1; }
+  __device__ bool empty() const { return !(w[0] | w[1] | w[2] | w[3]); }
+};
+
+// ------------------------------------------------------------------ Signature<PowerGraph, 4>
+
+__global__ void label_signature_kernel(const uint32_t* __restrict__ ro, const uint32_t* __restrict__ ci,
+                                       const uint8_t* __restrict__ labels, uint64_t n,
+                                       uint64_t* __restrict__ sig) {
+  // generateCSRSignatures (signature.hpp:135-146): increments on the existing value.
+  for (uint64_t u = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; u < n;
+       u += uint64_t(gridDim.x) * blockDim.x) {
+    uint64_t s = sig[u];
+    for (uint32_t k = ro[u]; k < ro[u + 1]; ++k) s = inc_label(s, labels[ci[k]]);
+    sig[u] = s;
+  }
+}
+
+__global__ void refine_signature_kernel(const uint32_t* __restrict__ go, uint32_t num_graphs,
+                                        const uint32_t* __restrict__ ro,
+                                        const uint32_t* __restrict__ ci,
+                                        const uint8_t* __restrict__ labels, uint64_t n,
+                                        uint32_t view_size, uint64_t* __restrict__ sig,
+                                        uint64_t* __restrict__ reach /* [4][n] */) {
+  // refineCSRSignatures<PowerGraph> (signature.hpp:347-385) with exact 256-bit sets.
+  for (uint64_t u = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; u < n;
+       u += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t g = graph_of(go, num_graphs, uint32_t(u));
+    const uint32_t gs = go[g], gn = go[g + 1] - gs;
+    const uint32_t ul = uint32_t(u) - gs;
+    Bits4 reachable, frontier;
+    reachable.clear();
+    frontier.clear();
+    reachable.set(ul);
+    frontier.set(ul);
+    for (uint32_t it = 0; it < view_size && !frontier.empty(); ++it) {
+      Bits4 next;
+      next.clear();
+      for (int w = 0; w < 4; ++w) {
+        uint64_t f = frontier.w[w];
+        while (f) {
+          const uint32_t x = uint32_t(w * 64 + ctz64(f));
+          f &= f - 1;
+          for (uint32_t k = ro[gs + x]; k < ro[gs + x + 1]; ++k) {
+            const uint32_t y = ci[k] - gs;
+            if (!reachable.get(y)) {
+              reachable.set(y);
+              next.set(y);
+            }
+          }
+        }
+      }
+      frontier = next;
+    }
+    reachable.w[ul >> 6] &= ~(uint64_t(1) << (ul & 63));
+    uint64_t old[4];
+    for (int w = 0; w < 4; ++w) {
+      old[w] = reach[uint64_t(w) * n + u];
+      reachable.w[w] &= ~old[w];
+    }
+    uint64_t s = 0;
+    for (uint32_t x = 0; x < gn; ++x)
+      if (reachable.get(x)) s = inc_label(s, labels[gs + x]);
+    sig[u] = s;
+    for (int w = 0; w < 4; ++w) reach[uint64_t(w) * n + u] = reachable.w[w] | old[w];
+  }
+}
+
+// ------------------------------------------------------------------ PathSignature
+
+__global__ void path_signature_kernel(const uint32_t* __restrict__ ro, const uint32_t* __restrict__ ci,
+                                      const uint8_t* __restrict__ labels, uint64_t n,
+                                      uint64_t* __restrict__ out /* [n][4] */) {
+  // generateCSRPathSignatures (signature.hpp:482-514): OR-s into the existing bits.
+  for (uint64_t u = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; u < n;
+       u += uint64_t(gridDim.x) * blockDim.x) {
+    uint64_t b[4] = {out[4 * u], out[4 * u + 1], out[4 * u + 2], out[4 * u + 3]};
+    const uint32_t lu = labels[u];
+    auto insert = [&](uint32_t h1) {
+      const uint32_t h2 = hash32(h1), h3 = hash32(h2);
+      const uint32_t i1 = h1 & 255u, i2 = h2 & 255u, i3 = h3 & 255u;
+      for (int w = 0; w < 4; ++w) {
+        uint64_t m = 0;
+        if ((i1 >> 6) == uint32_t(w)) m |= uint64_t(1) << (i1 & 63);
+        if ((i2 >> 6) == uint32_t(w)) m |= uint64_t(1) << (i2 & 63);
+        if ((i3 >> 6) == uint32_t(w)) m |= uint64_t(1) << (i3 & 63);
+        b[w] |= m;
+      }
+    };
+    for (uint32_t i = ro[u]; i < ro[u + 1]; ++i) {
+      const uint32_t v = ci[i];
+      const uint32_t lv = labels[v];
+      for (uint32_t j = ro[v]; j < ro[v + 1]; ++j) {
+        const uint32_t w = ci[j];
+        if (w == u) continue;
+        const uint32_t p2 = combine_hash(lu, combine_hash(lv, labels[w]));
+        insert(hash32(p2));
+        for (uint32_t k = ro[w]; k < ro[w + 1]; ++k) {
+          const uint32_t x = ci[k];
+          if (x == v || x == u) continue;
+          insert(hash32(combine_hash(p2, labels[x])));
+        }
+      }
+    }
+    for (int w = 0; w < 4; ++w) out[4 * u + w] = b[w];
+  }
+}
+
+// ------------------------------------------------------------------ CycleSignature
+
+__global__ void cycle_signature_kernel(const uint32_t* __restrict__ go, uint32_t num_graphs,
+                                       const uint32_t* __restrict__ ro,
+                                       const uint32_t* __restrict__ ci,
+                                       const uint8_t* __restrict__ labels, uint64_t n,
+                                       uint64_t* __restrict__ out) {
+  // generateCSRCycleSignatures (signature.hpp:602-715): simple paths of 4 and 5 edges from s that
+  // close back to s, per CSR entry (parallel edges count), u16 counters halved.  Graphs above 64
+  // nodes keep their value (:614).
+  for (uint64_t u = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; u < n;
+       u += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t g = graph_of(go, num_graphs, uint32_t(u));
+    const uint32_t gs = go[g], gn = go[g + 1] - gs;
+    if (gn > 64) continue;
+    const uint32_t s = uint32_t(u) - gs;
+    const uint8_t ls = labels[u];
+    uint32_t c5s = 0, c5d = 0, c6s = 0, c6d = 0;
+    const uint64_t m0 = uint64_t(1) << s;
+    for (uint32_t k1 = ro[u]; k1 < ro[u + 1]; ++k1) {
+      const uint32_t v1 = ci[k1] - gs;
+      if ((m0 >> v1) & 1) continue;
+      const bool s1 = labels[gs + v1] == ls;
+      const uint64_t m1 = m0 | (uint64_t(1) << v1);
+      for (uint32_t k2 = ro[gs + v1]; k2 < ro[gs + v1 + 1]; ++k2) {
+        const uint32_t v2 = ci[k2] - gs;
+        if ((m1 >> v2) & 1) continue;
+        const bool s2 = s1 && labels[gs + v2] == ls;
+        const uint64_t m2 = m1 | (uint64_t(1) << v2);
+        for (uint32_t k3 = ro[gs + v2]; k3 < ro[gs + v2 + 1]; ++k3) {
+          const uint32_t v3 = ci[k3] - gs;
+          if ((m2 >> v3) & 1) continue;
+          const bool s3 = s2 && labels[gs + v3] == ls;
+          const uint64_t m3 = m2 | (uint64_t(1) << v3);
+          for (uint32_t k4 = ro[gs + v3]; k4 < ro[gs + v3 + 1]; ++k4) {
+            const uint32_t v4 = ci[k4] - gs;
+            if ((m3 >> v4) & 1) continue;
+            const bool s4 = s3 && labels[gs + v4] == ls;
+            const uint64_t m4 = m3 | (uint64_t(1) << v4);
+            for (uint32_t k5 = ro[gs + v4]; k5 < ro[gs + v4 + 1]; ++k5) {
+              const uint32_t v5 = ci[k5] - gs;
+              if (v5 == s) {
+                if (s4) ++c5s; else ++c5d;
+                continue;
+              }
+              if ((m4 >> v5) & 1) continue;
+              const bool s5 = s4 && labels[gs + v5] == ls;
+              for (uint32_t k6 = ro[gs + v5]; k6 < ro[gs + v5 + 1]; ++k6)
+                if (ci[k6] - gs == s) {
+                  if (s5) ++c6s; else ++c6d;
+                }
+            }
+          }
+        }
+      }
+    }
+    out[u] = uint64_t((c5s & 0xffffu) / 2) | (uint64_t((c5d & 0xffffu) / 2) << 16) |
+             (uint64_t((c6s & 0xffffu) / 2) << 32) | (uint64_t((c6d & 0xffffu) / 2) << 48);
+  }
+}
+
+// ------------------------------------------------------------------ filter / refine
+
+// FilterCandidatesKernel_v2 (isomorphism.hpp:59-97): one wave per 64 data nodes (= one u64 word
+// of every row), query nodes streamed through scalar loads, the row word from one ballot.
+__global__ __launch_bounds__(256) void filter_kernel(
+    const uint32_t* __restrict__ qro, const uint8_t* __restrict__ qlab, uint64_t Qn,
+    const uint64_t* __restrict__ qpath, const uint64_t* __restrict__ qcyc,
+    const uint32_t* __restrict__ dro, const uint8_t* __restrict__ dlab, uint64_t Dn,
+    const uint64_t* __restrict__ dpath, const uint64_t* __restrict__ dcyc,
+    uint64_t* __restrict__ cand, uint64_t W) {
+  const uint64_t word = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / 64;
+  if (word >= W) return;
+  const uint64_t d = word * 64 + lane_id();
+  const bool active = d < Dn;
+  uint32_t dl = 0x100, ddeg = 0;
+  uint64_t dp[4] = {0, 0, 0, 0}, dc = 0;
+  if (active) {
+    dl = dlab[d];
+    ddeg = dro[d + 1] - dro[d];
+    for (int w = 0; w < 4; ++w) dp[w] = dpath[4 * d + w];
+    dc = dcyc[d];
+  }
+  for (uint64_t qn = 0; qn < Qn; ++qn) {
+    const uint32_t ql = qlab[qn];
+    const uint32_t qdeg = qro[qn + 1] - qro[qn];
+    bool ok = active && (dl == ql || ql == kWildcard) && qdeg <= ddeg;
+    if (ballot(ok)) {
+      for (int w = 0; w < 4; ++w) {
+        const uint64_t qp = qpath[4 * qn + w];
+        ok = ok && (dp[w] & qp) == qp;
+      }
+      const uint64_t qc = qcyc[qn];
+      for (int f = 0; f < 4; ++f) ok = ok && ((dc >> (16 * f)) & 0xffff) >= ((qc >> (16 * f)) & 0xffff);
+      const uint64_t m = ballot(ok);
+      if (m && lane_id() == 0) cand[qn * W + word] |= m;
+    }
+  }
+}
+
+__device__ inline bool dominates(uint64_t d, uint64_t q) {
+  for (int l = 0; l < 16; ++l)
+    if (((q >> (4 * l)) & 15) > ((d >> (4 * l)) & 15)) return false;
+  return true;
+}
+
+// RefineCandidatesKernel_v3 (isomorphism.hpp:142-196).
+__global__ void refine_candidates_kernel(const uint64_t* __restrict__ qsig,
+                                         const uint64_t* __restrict__ dsig, uint64_t Dn,
+                                         uint64_t* __restrict__ cand, uint64_t W, uint64_t total) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < total;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t block = cand[i];
+    if (!block) continue;
+    const uint64_t qn = i / W, w = i % W;
+    const uint64_t qs = qsig[qn];
+    uint64_t nb = block, bits = block;
+    while (bits) {
+      const int b = ctz64(bits);
+      bits &= bits - 1;
+      const uint64_t d = w * 64 + b;
+      if (d >= Dn || !dominates(dsig[d], qs)) nb &= ~(uint64_t(1) << b);
+    }
+    if (nb != block) cand[i] = nb;
+  }
+}
+
+__global__ void row_count_kernel(const uint64_t* __restrict__ cand, uint64_t rows, uint64_t W,
+                                 uint32_t* __restrict__ out) {
+  const uint64_t row = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / 64;
+  if (row >= rows) return;
+  uint32_t c = 0;
+  for (uint64_t w = lane_id(); w < W; w += 64) c += popc64(cand[row * W + w]);
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if (lane_id() == 0) out[row] = c;
+}
+
+// ------------------------------------------------------------------ GMCR
+
+// Bits [s, e) of a candidate row as words of 64 local ids (exact masks: candidates.hpp:94,140
+// shift by 32 when e % 32 == 0).
+__device__ inline uint64_t row_bits(const uint64_t* row, uint32_t s, uint32_t e, uint32_t k) {
+  const uint32_t lo = s + 64 * k;
+  if (lo >= e) return 0;
+  const uint32_t w = lo >> 6, off = lo & 63;
+  uint64_t x = row[w] >> off;
+  if (off && (lo - off + 64) < e) x |= row[w + 1] << (64 - off);
+  return x & low_mask(e - lo);
+}
+
+__device__ inline bool pair_valid(const uint64_t* cand, uint64_t W, const uint32_t* qgo, uint32_t q,
+                                  uint32_t s, uint32_t e) {
+  const uint32_t qb = qgo[q], nq = qgo[q + 1] - qb;
+  if (nq <= 1) return false;  // gmcr.hpp:64
+  for (uint32_t i = 0; i < nq; ++i) {
+    const uint64_t* row = cand + uint64_t(qb + i) * W;
+    bool any = false;
+    for (uint32_t k = 0; s + 64 * k < e && !any; ++k) any = row_bits(row, s, e, k) != 0;
+    if (!any) return false;
+  }
+  return true;
+}
+
+// GMCR k1 (gmcr.hpp:54-88): one wave per data graph, lanes over queries.
+__global__ __launch_bounds__(256) void gmcr_count_kernel(const uint64_t* __restrict__ cand, uint64_t W,
+                                                         const uint32_t* __restrict__ qgo, uint32_t Q,
+                                                         const uint32_t* __restrict__ dgo, uint32_t D,
+                                                         uint64_t* __restrict__ vbits, uint64_t VW,
+                                                         uint32_t* __restrict__ cnt) {
+  const uint64_t g = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / 64;
+  if (g >= D) return;
+  const uint32_t s = dgo[g], e = dgo[g + 1];
+  uint32_t total = 0;
+  for (uint32_t q0 = 0; q0 < Q; q0 += 64) {
+    const uint32_t q = q0 + lane_id();
+    const bool v = q < Q && pair_valid(cand, W, qgo, q, s, e);
+    const uint64_t m = ballot(v);
+    if (lane_id() == 0) vbits[g * VW + q0 / 64] = m;
+    total += popc64(m);
+  }
+  if (lane_id() == 0) cnt[g] = total;
+}
+
+// Exclusive scan of u32 counts into u32 offsets [n + 1] (replaces the serial single_task of
+// gmcr.hpp:92-94): one 1024-thread block, chunked.
+__global__ __launch_bounds__(1024) void scan_kernel(const uint32_t* __restrict__ cnt, uint64_t n,
+                                                    uint32_t* __restrict__ offsets) {
+  __shared__ uint32_t part[1024];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const uint32_t per = 8;
+  for (uint64_t base = 0; base < n; base += 1024 * per) {
+    uint32_t local[8];
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < per; ++k) {
+      const uint64_t i = base + uint64_t(threadIdx.x) * per + k;
+      local[k] = i < n ? cnt[i] : 0;
+      sum += local[k];
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+      const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+      __syncthreads();
+      part[threadIdx.x] += v;
+      __syncthreads();
+    }
+    uint32_t run = carry + part[threadIdx.x] - sum;
+    for (uint32_t k = 0; k < per; ++k) {
+      const uint64_t i = base + uint64_t(threadIdx.x) * per + k;
+      if (i < n) offsets[i] = run;
+      run += local[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += part[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) offsets[n] = carry;
+}
+
+// GMCR k3 (gmcr.hpp:111-126) made deterministic: ascending query ids per data graph.
+__global__ __launch_bounds__(256) void gmcr_fill_kernel(const uint64_t* __restrict__ vbits, uint64_t VW,
+                                                        uint32_t Q, uint32_t D,
+                                                        const uint32_t* __restrict__ offsets,
+                                                        uint32_t* __restrict__ indices) {
+  const uint64_t g = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / 64;
+  if (g >= D) return;
+  uint32_t pos = offsets[g];
+  for (uint32_t q0 = 0; q0 < Q; q0 += 64) {
+    const uint64_t m = vbits[g * VW + q0 / 64];
+    const uint32_t q = q0 + lane_id();
+    if ((m >> lane_id()) & 1) indices[pos + popc64(m & low_mask(lane_id()))] = q;
+    pos += popc64(m);
+  }
+}
+
+// ------------------------------------------------------------------ join
+
+// JoinCandidates3Optimized (isomorphism.hpp:309-468): one lane per GMCR pair, standard
+// backtracking over bitsets of the data graph's local ids (up to 4 words = 256 nodes).
+template <int NW>
+__global__ __launch_bounds__(256) void join_kernel(
+    const uint64_t* __restrict__ cand, uint64_t W, const uint32_t* __restrict__ qgo,
+    const uint32_t* __restrict__ qro, const uint32_t* __restrict__ qci,
+    const uint32_t* __restrict__ dgo, uint32_t D, const uint32_t* __restrict__ dro,
+    const uint32_t* __restrict__ dci, const uint32_t* __restrict__ goff,
+    const uint32_t* __restrict__ gidx, uint64_t T, int find_first, uint64_t* __restrict__ total,
+    uint64_t* __restrict__ per_pair) {
+  const uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  uint64_t c = 0;
+  if (t < T) {
+    // data graph of entry t (utils.hpp:110-121)
+    uint32_t lo = 0, hi = D + 1;
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (t >= goff[mid]) lo = mid + 1; else hi = mid;
+    }
+    const uint32_t g = lo - 1;
+    const uint32_t q = gidx[t];
+    const uint32_t s = dgo[g], e = dgo[g + 1], n = e - s;
+    const uint32_t qb = qgo[q], nq = qgo[q + 1] - qb;
+    uint64_t dom[32][NW];
+    uint32_t qadj[32];
+    for (uint32_t i = 0; i < nq; ++i) {
+      const uint64_t* row = cand + uint64_t(qb + i) * W;
+      for (int k = 0; k < NW; ++k) dom[i][k] = row_bits(row, s, e, k);
+      uint32_t a = 0;
+      for (uint32_t k = qro[qb + i]; k < qro[qb + i + 1]; ++k) a |= 1u << (qci[k] - qb);
+      qadj[i] = a;
+    }
+    auto nrow = [&](uint32_t x, uint64_t* out) {
+      for (int k = 0; k < NW; ++k) out[k] = 0;
+      for (uint32_t k = dro[s + x]; k < dro[s + x + 1]; ++k) {
+        const uint32_t y = dci[k] - s;
+        out[y >> 6] |= uint64_t(1) << (y & 63);
+      }
+    };
+    (void)n;
+    uint64_t R[32][NW];
+    uint32_t f[32];
+    uint64_t used[NW];
+    for (int k = 0; k < NW; ++k) {
+      used[k] = 0;
+      R[0][k] = dom[0][k];
+    }
+    int depth = 0;
+    bool stop = false;
+    while (!stop) {
+      int k0 = -1;
+      for (int k = 0; k < NW; ++k)
+        if (R[depth][k]) {
+          k0 = k;
+          break;
+        }
+      if (k0 < 0) {
+        if (depth == 0) break;
+        --depth;
+        used[f[depth] >> 6] &= ~(uint64_t(1) << (f[depth] & 63));
+        continue;
+      }
+      const uint32_t v = uint32_t(k0 * 64 + ctz64(R[depth][k0]));
+      R[depth][k0] &= R[depth][k0] - 1;
+      f[depth] = v;
+      const int nd = depth + 1;
+      uint64_t Rn[NW];
+      for (int k = 0; k < NW; ++k) Rn[k] = dom[nd][k] & ~used[k];
+      Rn[v >> 6] &= ~(uint64_t(1) << (v & 63));
+      for (int i = 0; i < nd; ++i) {
+        uint64_t a[NW];
+        nrow(f[i], a);
+        const bool adj = (qadj[nd] >> i) & 1;
+        for (int k = 0; k < NW; ++k) Rn[k] &= adj ? a[k] : ~a[k];
+      }
+      if (uint32_t(nd) == nq - 1) {
+        for (int k = 0; k < NW; ++k) c += uint64_t(popc64(Rn[k]));
+        if (find_first && c) stop = true;
+      } else {
+        used[v >> 6] |= uint64_t(1) << (v & 63);
+        depth = nd;
+        for (int k = 0; k < NW; ++k) R[nd][k] = Rn[k];
+      }
+    }
+    if (find_first) c = c ? 1 : 0;
+    if (per_pair) per_pair[t] = c;
+  }
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if (lane_id() == 0 && c) atomicAdd(reinterpret_cast<unsigned long long*>(total), (unsigned long long)c);
+}
+
+inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 1u << 20) {
+  uint64_t g = (n + block - 1) / block;
+  return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(g, cap)));
+}
+
+}  // namespace bms2
+
+using namespace bms2;
+
+// ------------------------------------------------------------------ handles
+
+struct bms2_signature_s {
+  bms2_queue q;
+  uint64_t n[2];
+  uint64_t* sig[2] = {nullptr, nullptr};
+  uint64_t* reach[2] = {nullptr, nullptr};
+};
+
+struct bms2_path_signature_s {
+  bms2_queue q;
+  uint64_t n[2];
+  uint64_t* bits[2] = {nullptr, nullptr};
+};
+
+struct bms2_cycle_signature_s {
+  bms2_queue q;
+  uint64_t n[2];
+  uint64_t* sig[2] = {nullptr, nullptr};
+};
+
+struct bms2_candidates_s {
+  bms2_queue q;
+  uint64_t rows, targets, W;
+  uint64_t* bits = nullptr;
+};
+
+struct bms2_gmcr_s {
+  bms2_queue q;
+  uint32_t num_data = 0;
+  uint64_t total = 0;
+  uint32_t* offsets = nullptr;
+  uint32_t* indices = nullptr;
+};
+
+namespace {
+
+int check_scope(int scope) {
+  return (scope == BMS2_SCOPE_DATA || scope == BMS2_SCOPE_QUERY)
+             ? BMS2_OK
+             : set_error(BMS2_ERR_INVALID, "scope must be BMS2_SCOPE_DATA or BMS2_SCOPE_QUERY");
+}
+
+int check_graph(const bms2_device_csr* g, uint64_t nodes) {
+  if (!g) return set_error(BMS2_ERR_INVALID, "graph is null");
+  if (g->total_nodes != nodes)
+    return set_error(BMS2_ERR_INVALID, "graph has %llu nodes, the object was sized for %llu",
+                     (unsigned long long)g->total_nodes, (unsigned long long)nodes);
+  return BMS2_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- Signature
+int bms2_signature_create(bms2_queue q, uint64_t data_nodes, uint64_t query_nodes,
+                          bms2_signature* out) {
+  if (int rc = use_device(q)) return rc;
+  if (!out) return set_error(BMS2_ERR_INVALID, "null output");
+  auto* s = new bms2_signature_s{};
+  s->q = q;
+  s->n[0] = data_nodes;
+  s->n[1] = query_nodes;
+  for (int sc = 0; sc < 2; ++sc) {
+    int rc;
+    if ((rc = device_alloc((void**)&s->sig[sc], 8 * s->n[sc])) ||
+        (rc = device_alloc((void**)&s->reach[sc], 32 * s->n[sc]))) {
+      bms2_signature_destroy(s);
+      return rc;
+    }
+    BMS2_HIP(hipMemsetAsync(s->sig[sc], 0, 8 * s->n[sc], q->stream));
+    BMS2_HIP(hipMemsetAsync(s->reach[sc], 0, 32 * s->n[sc], q->stream));
+  }
+  *out = s;
+  return BMS2_OK;
+}
+
+int bms2_signature_destroy(bms2_signature s) {
+  if (!s) return BMS2_OK;
+  (void)hipSetDevice(s->q->device);
+  (void)hipStreamSynchronize(s->q->stream);
+  for (int sc = 0; sc < 2; ++sc) {
+    (void)hipFree(s->sig[sc]);
+    (void)hipFree(s->reach[sc]);
+  }
+  delete s;
+  return BMS2_OK;
+}
+
+int bms2_signature_generate(bms2_signature s, const bms2_device_csr* g, int scope, bms2_event* ev) {
+  if (!s) return set_error(BMS2_ERR_INVALID, "signature is null");
+  if (int rc = check_scope(scope)) return rc;
+  if (int rc = use_device(s->q)) return rc;
+  if (int rc = check_graph(g, s->n[scope])) return rc;
+  EventScope es{s->q, ev};
+  if (int rc = es.begin()) return rc;
+  if (g->total_nodes)
+    hipLaunchKernelGGL(label_signature_kernel, dim3(grid_for(g->total_nodes, 256)), dim3(256), 0,
+                       s->q->stream, g->row_offsets, g->column_indices, g->node_labels,
+                       g->total_nodes, s->sig[scope]);
+  BMS2_CHECK_LAUNCH("label_signature_kernel");
+  return es.end();
+}
+
+int bms2_signature_refine(bms2_signature s, const bms2_device_csr* g, uint32_t view_size, int scope,
+                          bms2_event* ev) {
+  if (!s) return set_error(BMS2_ERR_INVALID, "signature is null");
+  if (int rc = check_scope(scope)) return rc;
+  if (int rc = use_device(s->q)) return rc;
+  if (int rc = check_graph(g, s->n[scope])) return rc;
+  if (g->max_graph_nodes > BMS2_MAX_GRAPH_NODES)
+    return set_error(BMS2_ERR_UNSUPPORTED, "graph of %u nodes", g->max_graph_nodes);
+  EventScope es{s->q, ev};
+  if (int rc = es.begin()) return rc;
+  if (g->total_nodes)
+    hipLaunchKernelGGL(refine_signature_kernel, dim3(grid_for(g->total_nodes, 256)), dim3(256), 0,
+                       s->q->stream, g->graph_offsets, g->num_graphs, g->row_offsets,
+                       g->column_indices, g->node_labels, g->total_nodes, view_size,
+                       s->sig[scope], s->reach[scope]);
+  BMS2_CHECK_LAUNCH("refine_signature_kernel");
+  return es.end();
+}
+
+int bms2_signature_copy_to_host(bms2_signature s, int scope, uint64_t* out) {
+  if (!s || !out) return set_error(BMS2_ERR_INVALID, "null argument");
+  if (int rc = check_scope(scope)) return rc;
+  if (int rc = use_device(s->q)) return rc;
+  if (s->n[scope]) {
+    BMS2_HIP(hipMemcpyAsync(out, s->sig[scope], 8 * s->n[scope], hipMemcpyDeviceToHost, s->q->stream));
+    BMS2_HIP(hipStreamSynchronize(s->q->stream));
+  }
+  return BMS2_OK;
+}
+
+const uint64_t* bms2_signature_device_ptr(bms2_signature s, int scope) {
+  if (!s || check_scope(scope)) return nullptr;
+  return s->sig[scope];
+}
+
+// ---- PathSignature
+int bms2_path_signature_create(bms2_queue q, uint64_t data_nodes, uint64_t query_nodes,
+                               bms2_path_signature* out) {
+  if (int rc = use_device(q)) return rc;
+  if (!out) return set_error(BMS2_ERR_INVALID, "null output");
+  auto* s = new bms2_path_signature_s{};
+  s->q = q;
+  s->n[0] = data_nodes;
+  s->n[1] = query_nodes;
+  for (int sc = 0; sc < 2; ++sc) {
+    if (int rc = device_alloc((void**)&s->bits[sc], 32 * s->n[sc])) {
+      bms2_path_signature_destroy(s);
+      return rc;
+    }
+    BMS2_HIP(hipMemsetAsync(s->bits[sc], 0, 32 * s->n[sc], q->stream));
+  }
+  *out = s;
+  return BMS2_OK;
+}
+
+int bms2_path_signature_destroy(bms2_path_signature s) {
+  if (!s) return BMS2_OK;
+  (void)hipSetDevice(s->q->device);
+  (void)hipStreamSynchronize(s->q->stream);
+  for (int sc = 0; sc < 2; ++sc) (void)hipFree(s->bits[sc]);
+  delete s;
+  return BMS2_OK;
+}
+
+int bms2_path_signature_generate(bms2_path_signature s, const bms2_device_csr* g, int scope,
+                                 bms2_event* ev) {
+  if (!s) return set_error(BMS2_ERR_INVALID, "path signature is null");
+  if (int rc = check_scope(scope)) return rc;
+  if (int rc = use_device(s->q)) return rc;
+  if (int rc = check_graph(g, s->n[scope])) return rc;
+  EventScope es{s->q, ev};
+  if (int rc = es.begin()) return rc;
+  if (g->total_nodes)
+    hipLaunchKernelGGL(path_signature_kernel, dim3(grid_for(g->total_nodes, 256)), dim3(256), 0,
+                       s->q->stream, g->row_offsets, g->column_indices, g->node_labels,
+                       g->total_nodes, s->bits[scope]);
+  BMS2_CHECK_LAUNCH("path_signature_kernel");
+  return es.end();
+}
+
+int bms2_path_signature_copy_to_host(bms2_path_signature s, int scope, uint64_t* out) {
+  if (!s || !out) return set_error(BMS2_ERR_INVALID, "null argument");
+  if (int rc = check_scope(scope)) return rc;
+  if (int rc = use_device(s->q)) return rc;
+  if (s->n[scope]) {
+    BMS2_HIP(hipMemcpyAsync(out, s->bits[scope], 32 * s->n[scope], hipMemcpyDeviceToHost, s->q->stream));
+    BMS2_HIP(hipStreamSynchronize(s->q->stream));
+  }
+  return BMS2_OK;
+}
+
+// ---- CycleSignature
+int bms2_cycle_signature_create(bms2_queue q, uint64_t data_nodes, uint64_t query_nodes,
+                                bms2_cycle_signature* out) {
+  if (int rc = use_device(q)) return rc;
+  if (!out) return set_error(BMS2_ERR_INVALID, "null output");
+  auto* s = new bms2_cycle_signature_s{};
+  s->q = q;
+  s->n[0] = data_nodes;
+  s->n[1] = query_nodes;
+  for (int sc = 0; sc < 2; ++sc) {
+    if (int rc = device_alloc((void**)&s->sig[sc], 8 * s->n[sc])) {
+      bms2_cycle_signature_destroy(s);
+      return rc;
+    }
+    BMS2_HIP(hipMemsetAsync(s->sig[sc], 0, 8 * s->n[sc], q->stream));
+  }
+  *out = s;
+  return BMS2_OK;
+}
+
+int bms2_cycle_signature_destroy(bms2_cycle_signature s) {
+  if (!s) return BMS2_OK;
+  (void)hipSetDevice(s->q->device);
+  (void)hipStreamSynchronize(s->q->stream);
+  for (int sc = 0; sc < 2; ++sc) (void)hipFree(s->sig[sc]);
+  delete s;
+  return BMS2_OK;
+}
+
+int bms2_cycle_signature_generate(bms2_cycle_signature s, const bms2_device_csr* g, int scope,
+                                  bms2_event* ev) {
+  if (!s) return set_error(BMS2_ERR_INVALID, "cycle signature is null");
+  if (int rc = check_scope(scope)) return rc;
+  if (int rc = use_device(s->q)) return rc;
+  if (int rc = check_graph(g, s->n[scope])) return rc;
+  if (g->max_degree > 27)
+    return set_error(BMS2_ERR_UNSUPPORTED,
+                     "node degree %u above 27: the reference's 128-entry ring-count stack could "
+                     "overflow (signature.hpp:638,697)",
+                     g->max_degree);
+  EventScope es{s->q, ev};
+  if (int rc = es.begin()) return rc;
+  if (g->total_nodes)
+    hipLaunchKernelGGL(cycle_signature_kernel, dim3(grid_for(g->total_nodes, 256)), dim3(256), 0,
+                       s->q->stream, g->graph_offsets, g->num_graphs, g->row_offsets,
+                       g->column_indices, g->node_labels, g->total_nodes, s->sig[scope]);
+  BMS2_CHECK_LAUNCH("cycle_signature_kernel");
+  return es.end();
+}
+
+int bms2_cycle_signature_copy_to_host(bms2_cycle_signature s, int scope, uint64_t* out) {
+  if (!s || !out) return set_error(BMS2_ERR_INVALID, "null argument");
+  if (int rc = check_scope(scope)) return rc;
+  if (int rc = use_device(s->q)) return rc;
+  if (s->n[scope]) {
+    BMS2_HIP(hipMemcpyAsync(out, s->sig[scope], 8 * s->n[scope], hipMemcpyDeviceToHost, s->q->stream));
+    BMS2_HIP(hipStreamSynchronize(s->q->stream));
+  }
+  return BMS2_OK;
+}
+
+// ---- Candidates
+int bms2_candidates_create(bms2_queue q, uint64_t source_nodes, uint64_t target_nodes,
+                           bms2_candidates* out) {
+  if (int rc = use_device(q)) return rc;
+  if (!out) return set_error(BMS2_ERR_INVALID, "null output");
+  auto* c = new bms2_candidates_s{};
+  c->q = q;
+  c->rows = source_nodes;
+  c->targets = target_nodes;
+  c->W = ceil_div(target_nodes, 64);
+  if (int rc = device_alloc((void**)&c->bits, 8 * c->rows * c->W)) {
+    delete c;
+    return rc;
+  }
+  BMS2_HIP(hipMemsetAsync(c->bits, 0, 8 * c->rows * c->W, q->stream));
+  *out = c;
+  return BMS2_OK;
+}
+
+int bms2_candidates_destroy(bms2_candidates c) {
+  if (!c) return BMS2_OK;
+  (void)hipSetDevice(c->q->device);
+  (void)hipStreamSynchronize(c->q->stream);
+  (void)hipFree(c->bits);
+  delete c;
+  return BMS2_OK;
+}
+
+uint64_t bms2_candidates_words_per_row(bms2_candidates c) { return c ? c->W : 0; }
+
+int bms2_candidates_copy_to_host(bms2_candidates c, uint64_t* out) {
+  if (!c || !out) return set_error(BMS2_ERR_INVALID, "null argument");
+  if (int rc = use_device(c->q)) return rc;
+  if (c->rows * c->W) {
+    BMS2_HIP(hipMemcpyAsync(out, c->bits, 8 * c->rows * c->W, hipMemcpyDeviceToHost, c->q->stream));
+    BMS2_HIP(hipStreamSynchronize(c->q->stream));
+  }
+  return BMS2_OK;
+}
+
+int bms2_candidates_row_counts(bms2_candidates c, uint32_t* out) {
+  if (!c || !out) return set_error(BMS2_ERR_INVALID, "null argument");
+  if (int rc = use_device(c->q)) return rc;
+  if (!c->rows) return BMS2_OK;
+  uint32_t* d = nullptr;
+  if (int rc = device_alloc((void**)&d, 4 * c->rows)) return rc;
+  hipLaunchKernelGGL(row_count_kernel, dim3(grid_for(c->rows * 64, 256, 1u << 30)), dim3(256), 0,
+                     c->q->stream, c->bits, c->rows, c->W, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, 4 * c->rows, hipMemcpyDeviceToHost, c->q->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->q->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "row counts", __FILE__, __LINE__);
+  return BMS2_OK;
+}
+
+// ---- filter / refine
+int bms2_filter_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
+                           bms2_signature sig, bms2_path_signature path,
+                           bms2_cycle_signature cycle, bms2_candidates cand, bms2_event* ev) {
+  if (int rc = use_device(q)) return rc;
+  if (!query || !data || !sig || !path || !cycle || !cand)
+    return set_error(BMS2_ERR_INVALID, "null argument");
+  if (cand->rows != query->total_nodes || cand->targets != data->total_nodes)
+    return set_error(BMS2_ERR_INVALID, "candidates sized %llux%llu for a %llux%llu problem",
+                     (unsigned long long)cand->rows, (unsigned long long)cand->targets,
+                     (unsigned long long)query->total_nodes, (unsigned long long)data->total_nodes);
+  if (path->n[0] != data->total_nodes || path->n[1] != query->total_nodes ||
+      cycle->n[0] != data->total_nodes || cycle->n[1] != query->total_nodes)
+    return set_error(BMS2_ERR_INVALID, "signature objects sized for another problem");
+  (void)sig;  // the label histogram is not part of the initial filter (isomorphism.hpp:71)
+  EventScope es{q, ev};
+  if (int rc = es.begin()) return rc;
+  if (cand->W && query->total_nodes)
+    hipLaunchKernelGGL(filter_kernel, dim3(grid_for(cand->W * 64, 256, 1u << 30)), dim3(256), 0,
+                       q->stream, query->row_offsets, query->node_labels, query->total_nodes,
+                       path->bits[1], cycle->sig[1], data->row_offsets, data->node_labels,
+                       data->total_nodes, path->bits[0], cycle->sig[0], cand->bits, cand->W);
+  BMS2_CHECK_LAUNCH("filter_kernel");
+  return es.end();
+}
+
+int bms2_refine_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
+                           bms2_signature sig, bms2_candidates cand, bms2_event* ev) {
+  if (int rc = use_device(q)) return rc;
+  if (!query || !data || !sig || !cand) return set_error(BMS2_ERR_INVALID, "null argument");
+  if (cand->rows != query->total_nodes || cand->targets != data->total_nodes ||
+      sig->n[0] != data->total_nodes || sig->n[1] != query->total_nodes)
+    return set_error(BMS2_ERR_INVALID, "objects sized for another problem");
+  EventScope es{q, ev};
+  if (int rc = es.begin()) return rc;
+  const uint64_t total = cand->rows * cand->W;
+  if (total)
+    hipLaunchKernelGGL(refine_candidates_kernel, dim3(grid_for(total, 256)), dim3(256), 0, q->stream,
+                       sig->sig[1], sig->sig[0], data->total_nodes, cand->bits, cand->W, total);
+  BMS2_CHECK_LAUNCH("refine_candidates_kernel");
+  return es.end();
+}
+
+// ---- GMCR
+int bms2_gmcr_create(bms2_queue q, bms2_gmcr* out) {
+  if (int rc = use_device(q)) return rc;
+  if (!out) return set_error(BMS2_ERR_INVALID, "null output");
+  auto* g = new bms2_gmcr_s{};
+  g->q = q;
+  *out = g;
+  return BMS2_OK;
+}
+
+int bms2_gmcr_destroy(bms2_gmcr g) {
+  if (!g) return BMS2_OK;
+  (void)hipSetDevice(g->q->device);
+  (void)hipStreamSynchronize(g->q->stream);
+  (void)hipFree(g->offsets);
+  (void)hipFree(g->indices);
+  delete g;
+  return BMS2_OK;
+}
+
+int bms2_gmcr_generate(bms2_gmcr gm, const bms2_device_csr* query, const bms2_device_csr* data,
+                       bms2_candidates cand, bms2_event* ev) {
+  if (!gm) return set_error(BMS2_ERR_INVALID, "gmcr is null");
+  if (int rc = use_device(gm->q)) return rc;
+  if (!query || !data || !cand) return set_error(BMS2_ERR_INVALID, "null argument");
+  if (cand->rows != query->total_nodes || cand->targets != data->total_nodes)
+    return set_error(BMS2_ERR_INVALID, "candidates sized for another problem");
+  hipStream_t s = gm->q->stream;
+  const uint32_t D = data->num_graphs, Q = query->num_graphs;
+  const uint64_t VW = ceil_div(Q, 64);
+  (void)hipFree(gm->offsets);
+  (void)hipFree(gm->indices);
+  gm->offsets = gm->indices = nullptr;
+  gm->total = 0;
+  gm->num_data = D;
+  uint64_t* vbits = nullptr;
+  uint32_t* cnt = nullptr;
+  int rc;
+  if ((rc = device_alloc((void**)&gm->offsets, 4ull * (D + 1))) ||
+      (rc = device_alloc((void**)&vbits, 8 * std::max<uint64_t>(1, uint64_t(D) * VW))) ||
+      (rc = device_alloc((void**)&cnt, 4ull * std::max<uint32_t>(D, 1)))) {
+    (void)hipFree(vbits);
+    (void)hipFree(cnt);
+    return rc;
+  }
+  EventScope es{gm->q, ev};
+  if ((rc = es.begin())) return rc;
+  hipError_t e = hipSuccess;
+  if (D) {
+    if (Q)
+      hipLaunchKernelGGL(gmcr_count_kernel, dim3(grid_for(uint64_t(D) * 64, 256, 1u << 30)), dim3(256),
+                         0, s, cand->bits, cand->W, query->graph_offsets, Q, data->graph_offsets, D,
+                         vbits, VW, cnt);
+    else
+      e = hipMemsetAsync(cnt, 0, 4ull * D, s);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, cnt, uint64_t(D), gm->offsets);
+  } else {
+    e = hipMemsetAsync(gm->offsets, 0, 4, s);
+  }
+  if (e == hipSuccess) e = hipGetLastError();
+  uint32_t total = 0;
+  // The total sizes the index array (gmcr.hpp:99 reads it back the same way).
+  if (e == hipSuccess) e = hipMemcpyAsync(&total, gm->offsets + D, 4, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess) {
+    gm->total = total;
+    if ((rc = device_alloc((void**)&gm->indices, 4ull * std::max<uint32_t>(total, 1)))) {
+      (void)hipFree(vbits);
+      (void)hipFree(cnt);
+      return rc;
+    }
+    if (D && Q && total)
+      hipLaunchKernelGGL(gmcr_fill_kernel, dim3(grid_for(uint64_t(D) * 64, 256, 1u << 30)), dim3(256), 0,
+                         s, vbits, VW, Q, D, gm->offsets, gm->indices);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(vbits);
+  (void)hipFree(cnt);
+  if (e != hipSuccess) return hip_fail(e, "generateGMCR", __FILE__, __LINE__);
+  return es.end();
+}
+
+uint64_t bms2_gmcr_total(bms2_gmcr g) { return g ? g->total : 0; }
+
+int bms2_gmcr_device_ptrs(bms2_gmcr g, const uint32_t** offsets, const uint32_t** indices) {
+  if (!g) return set_error(BMS2_ERR_INVALID, "gmcr is null");
+  if (offsets) *offsets = g->offsets;
+  if (indices) *indices = g->indices;
+  return BMS2_OK;
+}
+
+int bms2_gmcr_copy_to_host(bms2_gmcr g, uint32_t* offsets, uint32_t* indices) {
+  if (!g) return set_error(BMS2_ERR_INVALID, "gmcr is null");
+  if (!g->offsets) return set_error(BMS2_ERR_INVALID, "gmcr not generated");
+  if (int rc = use_device(g->q)) return rc;
+  hipStream_t s = g->q->stream;
+  if (offsets) BMS2_HIP(hipMemcpyAsync(offsets, g->offsets, 4ull * (g->num_data + 1), hipMemcpyDeviceToHost, s));
+  if (indices && g->total)
+    BMS2_HIP(hipMemcpyAsync(indices, g->indices, 4ull * g->total, hipMemcpyDeviceToHost, s));
+  BMS2_HIP(hipStreamSynchronize(s));
+  return BMS2_OK;
+}
+
+// ---- join
+int bms2_join_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
+                         bms2_candidates cand, bms2_gmcr gmcr, uint64_t* d_num_matches,
+                         int find_first, uint64_t* d_pair_counts, bms2_event* ev) {
+  if (int rc = use_device(q)) return rc;
+  if (!query || !data || !cand || !gmcr || !d_num_matches)
+    return set_error(BMS2_ERR_INVALID, "null argument");
+  if (!gmcr->offsets) return set_error(BMS2_ERR_INVALID, "gmcr not generated");
+  if (gmcr->num_data != data->num_graphs) return set_error(BMS2_ERR_INVALID, "gmcr of another batch");
+  if (query->max_graph_nodes > BMS2_MAX_QUERY_NODES)
+    return set_error(BMS2_ERR_UNSUPPORTED, "query with %u nodes (limit %u, isomorphism.hpp:248)",
+                     query->max_graph_nodes, BMS2_MAX_QUERY_NODES);
+  EventScope es{q, ev};
+  if (int rc = es.begin()) return rc;
+  const uint64_t T = gmcr->total;
+  if (T) {
+    auto args = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(grid_for(T, 256, 1u << 30)), dim3(256), 0, q->stream, cand->bits,
+                         cand->W, query->graph_offsets, query->row_offsets, query->column_indices,
+                         data->graph_offsets, data->num_graphs, data->row_offsets,
+                         data->column_indices, gmcr->offsets, gmcr->indices, T, find_first,
+                         d_num_matches, d_pair_counts);
+    };
+    if (data->max_graph_nodes <= 64)
+      args(join_kernel<1>);
+    else
+      args(join_kernel<4>);
+  }
+  BMS2_CHECK_LAUNCH("join_kernel");
+  return es.end();
+}
+
+}  // extern "C"

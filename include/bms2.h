@@ -1,0 +1,261 @@
+/*
+ * bms2.h -- C ABI of the MI355X (gfx950) batched subgraph-isomorphism library.
+ *
+ * Every entry point returns an int status (BMS2_OK == 0); bms2_last_error() gives the message of
+ * the last failure on the calling thread.  Device work is enqueued on a bms2_queue (a HIP stream);
+ * calls are ordered on that queue and return before the work completes unless stated otherwise.
+ * Plain pointers and sizes only: no C++ or torch types cross this boundary.
+ *
+ * Each function names the reference interface it replaces (paths relative to the AnySparse/BMS2
+ * tree).  The C++ facade in include/sigmo.hpp wraps these calls behind the reference's own class
+ * and function names.
+ */
+#ifndef BMS2_H
+#define BMS2_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+#define BMS2_OK 0
+#define BMS2_ERR_INVALID 1     /* bad argument, shape or handle */
+#define BMS2_ERR_UNSUPPORTED 2 /* input outside the supported limits below */
+#define BMS2_ERR_HIP 3         /* HIP runtime failure */
+#define BMS2_ERR_IO 4          /* file could not be opened / written */
+#define BMS2_ERR_FORMAT 5      /* malformed .dat line (graph.hpp:199-201) */
+#define BMS2_ERR_NOMEM 6       /* host or device allocation failed */
+#define BMS2_ERR_CAPACITY 7    /* caller buffer too small */
+
+/* ---------------------------------------------------------------- constants
+ * types.hpp:18 WILDCARD_NODE; isomorphism.hpp:248 MAX_QUERY_NODES (=30 there, 32 here);
+ * data graphs up to 256 nodes (the reference is exact only up to 64, utils.hpp:95-97). */
+#define BMS2_WILDCARD_NODE 255u
+#define BMS2_MAX_QUERY_NODES 32u
+#define BMS2_MAX_GRAPH_NODES 256u
+#define BMS2_SCOPE_DATA 0
+#define BMS2_SCOPE_QUERY 1
+
+const char* bms2_last_error(void);
+/* Library version as 10000*major + 100*minor + patch. */
+int bms2_version(void);
+
+/* ---------------------------------------------------------------- host containers
+ * A packed batch of CSR graphs in host memory, the layout createDeviceCSRGraph packs
+ * (graph.hpp:251-290): graph_offsets is the node prefix over graphs, column_indices hold global
+ * node ids, each undirected edge appears under both endpoints in file edge order
+ * (graph.hpp:229-240).  Arrays filled by the library are released with bms2_host_csr_free. */
+typedef struct bms2_host_csr {
+  uint32_t num_graphs;
+  uint32_t reserved;
+  uint64_t total_nodes;
+  uint64_t total_edges; /* directed CSR entries */
+  uint32_t* graph_offsets;  /* [num_graphs + 1] */
+  uint32_t* row_offsets;    /* [total_nodes + 1] */
+  uint32_t* column_indices; /* [total_edges] */
+  uint8_t* node_labels;     /* [total_nodes] */
+  uint8_t* edge_labels;     /* [total_edges] */
+} bms2_host_csr;
+
+void bms2_host_csr_free(bms2_host_csr* g);
+
+/* Reads a .dat file, one graph per line: "n#N l#L (id label)xN e#E (u v [label])xE".
+ * Replaces io::loadCSRGraphsFromFile (io.hpp:43-52) with IntermediateGraph (graph.hpp:150-202)
+ * and toCSRGraph (graph.hpp:212-243).  max_graphs == 0 reads every line.  A malformed line
+ * fails with BMS2_ERR_FORMAT naming the line, unless skip_malformed is set, in which case it is
+ * dropped as BMS2_batch.cpp:121-124 does. */
+int bms2_load_dat(const char* path, uint64_t max_graphs, int skip_malformed, bms2_host_csr* out);
+/* Writes the batch in the same format (the writer of generate_data.py:76-92). */
+int bms2_write_dat(const char* path, const bms2_host_csr* g);
+
+/* Deterministic synthetic molecule workloads (no reference counterpart: the reference reads
+ * RDKit-converted SMARTS files, generate_data.py).  Graph i of the data batch depends only on
+ * (seed, i), so any prefix of a large batch equals the smaller batch. */
+typedef struct bms2_workload_spec {
+  uint64_t seed;
+  uint32_t num_data_graphs;
+  uint32_t num_query_graphs;
+  uint32_t data_size_dist;       /* 0: uniform [min,max]; 1: lognormal(mu, sigma) clipped */
+  uint32_t data_min_nodes;
+  uint32_t data_max_nodes;
+  uint32_t polycyclic_permille;  /* share of data graphs built as fused ring systems */
+  double data_lognormal_mu;
+  double data_lognormal_sigma;
+  uint32_t poly_min_rings;
+  uint32_t poly_max_rings;
+  uint32_t query_min_nodes;
+  uint32_t query_max_nodes;
+  uint32_t extracted_permille;   /* share of queries cut from data graphs (rest independent) */
+  uint32_t query_shapes;         /* 0: molecule-like; 1: adds ring and branched shapes */
+  uint32_t max_ring_closures;    /* ring-closing edges per molecule: uniform 0..max */
+  uint32_t num_threads;          /* 0: hardware concurrency */
+} bms2_workload_spec;
+
+int bms2_generate_workload(const bms2_workload_spec* spec, bms2_host_csr* data_out,
+                           bms2_host_csr* query_out);
+
+/* ---------------------------------------------------------------- queues and events */
+typedef struct bms2_queue_s* bms2_queue;
+typedef struct bms2_event_s* bms2_event;
+
+/* A queue owns a new HIP stream on `device` (sycl::queue in BMS2.cpp:14). */
+int bms2_queue_create(int device, bms2_queue* out);
+/* A queue over an existing hipStream_t (not owned). */
+int bms2_queue_wrap(int device, void* hip_stream, bms2_queue* out);
+int bms2_queue_destroy(bms2_queue q);
+int bms2_queue_wait(bms2_queue q); /* queue.wait_and_throw() */
+void* bms2_queue_stream(bms2_queue q);
+
+/* Every stage call takes an optional bms2_event* out: when non-null it receives a new event
+ * pair bracketing the call's kernels (utils::BatchedEvent, utils.hpp:123-157). */
+int bms2_event_elapsed_ms(bms2_event e, float* ms); /* waits for the event */
+int bms2_event_destroy(bms2_event e);
+
+/* Device memory helpers for callers without another allocator. */
+int bms2_device_malloc(bms2_queue q, uint64_t bytes, void** out);
+int bms2_device_free(bms2_queue q, void* ptr);
+int bms2_memcpy_h2d(bms2_queue q, void* dst, const void* src, uint64_t bytes);
+int bms2_memcpy_d2h(bms2_queue q, void* dst, const void* src, uint64_t bytes); /* synchronous */
+int bms2_memset_zero(bms2_queue q, void* dst, uint64_t bytes);
+
+/* ---------------------------------------------------------------- device graphs
+ * Same fields as sigmo::DeviceBatchedCSRGraph (graph.hpp:58-66), device pointers. */
+typedef struct bms2_device_csr {
+  uint32_t* graph_offsets;
+  uint32_t* row_offsets;
+  uint32_t* column_indices;
+  uint8_t* node_labels;
+  uint8_t* edge_labels;
+  uint32_t num_graphs;
+  uint32_t max_graph_nodes; /* largest graph, filled at creation */
+  uint64_t total_nodes;
+  uint64_t total_edges;
+  uint32_t max_degree; /* largest CSR row, filled at creation */
+  uint32_t reserved;
+} bms2_device_csr;
+
+/* createDeviceCSRGraph (graph.hpp:251-310): uploads a packed host batch. */
+int bms2_device_csr_create(bms2_queue q, const bms2_host_csr* host, bms2_device_csr* out);
+/* destroyDeviceCSRGraph (graph.hpp:312-318). */
+int bms2_device_csr_destroy(bms2_queue q, bms2_device_csr* g);
+/* getDeviceCSRGraphAllocSize (graph.hpp:320-324). */
+uint64_t bms2_device_csr_alloc_size(const bms2_device_csr* g);
+
+/* ---------------------------------------------------------------- Signature<PowerGraph, 4>
+ * 16 x 4-bit saturating label histograms (signature.hpp:22-53).  Host copies are one u64 per
+ * node, the reference's SignatureDevice layout. */
+typedef struct bms2_signature_s* bms2_signature;
+/* Signature(queue, data_nodes, query_nodes) (signature.hpp:156-167); reachability state is
+ * zero-initialised. */
+int bms2_signature_create(bms2_queue q, uint64_t data_nodes, uint64_t query_nodes,
+                          bms2_signature* out);
+int bms2_signature_destroy(bms2_signature s);
+/* generateDataSignatures / generateQuerySignatures (signature.hpp:56-84, 124-151). */
+int bms2_signature_generate(bms2_signature s, const bms2_device_csr* g, int scope, bms2_event* ev);
+/* refineDataSignatures / refineQuerySignatures (signature.hpp:61-73, 336-389): histogram of the
+ * nodes within view_size hops, minus the node and every node an earlier call already counted. */
+int bms2_signature_refine(bms2_signature s, const bms2_device_csr* g, uint32_t view_size, int scope,
+                          bms2_event* ev);
+int bms2_signature_copy_to_host(bms2_signature s, int scope, uint64_t* out);
+const uint64_t* bms2_signature_device_ptr(bms2_signature s, int scope);
+
+/* ---------------------------------------------------------------- PathSignature
+ * 256-bit Bloom filter of label paths of 2 and 3 edges (signature.hpp:406-532).  Host copies are
+ * four u64 per node ([node][4], the reference's PathSignatureDevice layout). */
+typedef struct bms2_path_signature_s* bms2_path_signature;
+int bms2_path_signature_create(bms2_queue q, uint64_t data_nodes, uint64_t query_nodes,
+                               bms2_path_signature* out); /* signature.hpp:440-447 */
+int bms2_path_signature_destroy(bms2_path_signature s);
+int bms2_path_signature_generate(bms2_path_signature s, const bms2_device_csr* g, int scope,
+                                 bms2_event* ev); /* signature.hpp:455-519 */
+int bms2_path_signature_copy_to_host(bms2_path_signature s, int scope, uint64_t* out);
+
+/* ---------------------------------------------------------------- CycleSignature
+ * Per node counts of 5- and 6-rings, same-label vs mixed (signature.hpp:535-732).  Host copies
+ * are one u64 per node: u16 fields {5 same, 5 diff, 6 same, 6 diff} from the low end. */
+typedef struct bms2_cycle_signature_s* bms2_cycle_signature;
+int bms2_cycle_signature_create(bms2_queue q, uint64_t data_nodes, uint64_t query_nodes,
+                                bms2_cycle_signature* out); /* signature.hpp:557-566 */
+int bms2_cycle_signature_destroy(bms2_cycle_signature s);
+int bms2_cycle_signature_generate(bms2_cycle_signature s, const bms2_device_csr* g, int scope,
+                                  bms2_event* ev); /* signature.hpp:574-719 */
+int bms2_cycle_signature_copy_to_host(bms2_cycle_signature s, int scope, uint64_t* out);
+
+/* ---------------------------------------------------------------- Candidates
+ * Dense domain bit-matrix: one row per global query node, bit d%64 of u64 word d/64 for global
+ * data node d (candidates.hpp:12-159, whose u32 words hold the same bits). */
+typedef struct bms2_candidates_s* bms2_candidates;
+int bms2_candidates_create(bms2_queue q, uint64_t source_nodes, uint64_t target_nodes,
+                           bms2_candidates* out); /* candidates.hpp:161-175, zero-filled */
+int bms2_candidates_destroy(bms2_candidates c);
+uint64_t bms2_candidates_words_per_row(bms2_candidates c);
+int bms2_candidates_copy_to_host(bms2_candidates c, uint64_t* out); /* candidates.hpp:181-186 */
+/* Per-row popcounts (getCandidatesCount(source_node), candidates.hpp:73-80) into out[rows]. */
+int bms2_candidates_row_counts(bms2_candidates c, uint32_t* out);
+
+/* ---------------------------------------------------------------- filter / refine
+ * filterCandidates (isomorphism.hpp:30-104): sets bit (q, d) when the labels agree (or the query
+ * label is the wildcard), deg(q) <= deg(d), path(d) contains path(q) and every cycle count of d
+ * is >= that of q.  Bits are OR-ed into the matrix. */
+int bms2_filter_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
+                           bms2_signature sig, bms2_path_signature path,
+                           bms2_cycle_signature cycle, bms2_candidates cand, bms2_event* ev);
+/* refineCandidates (isomorphism.hpp:106-203): clears (q, d) unless every label count of the
+ * current query signature is <= that of the data signature. */
+int bms2_refine_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
+                           bms2_signature sig, bms2_candidates cand, bms2_event* ev);
+
+/* ---------------------------------------------------------------- GMCR
+ * mapping::GMCR (gmcr.hpp:16-145): CSR from data graph to the query graphs with more than one
+ * node whose every node keeps a candidate in that data graph; ascending query id per segment. */
+typedef struct bms2_gmcr_s* bms2_gmcr;
+int bms2_gmcr_create(bms2_queue q, bms2_gmcr* out); /* gmcr.hpp:26 */
+int bms2_gmcr_destroy(bms2_gmcr g);
+int bms2_gmcr_generate(bms2_gmcr g, const bms2_device_csr* query, const bms2_device_csr* data,
+                       bms2_candidates cand, bms2_event* ev); /* gmcr.hpp:35-142 */
+uint64_t bms2_gmcr_total(bms2_gmcr g);
+int bms2_gmcr_device_ptrs(bms2_gmcr g, const uint32_t** data_graph_offsets,
+                          const uint32_t** query_graph_indices);
+int bms2_gmcr_copy_to_host(bms2_gmcr g, uint32_t* data_graph_offsets /*[D+1]*/,
+                           uint32_t* query_graph_indices /*[total]*/);
+
+/* ---------------------------------------------------------------- join
+ * joinCandidates (isomorphism.hpp:291-472): for every GMCR pair, the number of injective maps f
+ * with f(i) in the domain of query node i inside the data graph and query adjacency equal to data
+ * adjacency on every mapped pair (induced).  find_first counts 1 per pair with any embedding.
+ * d_num_matches is a device u64 the call atomically adds the total to (the caller zeroes it, as
+ * BMS2.cpp:205-206 does).  d_pair_counts (nullable) receives one u64 per GMCR entry. */
+int bms2_join_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
+                         bms2_candidates cand, bms2_gmcr gmcr, uint64_t* d_num_matches,
+                         int find_first, uint64_t* d_pair_counts, bms2_event* ev);
+
+/* ---------------------------------------------------------------- fused pipeline
+ * The stage sequence of BMS2.cpp:131-234 (signatures -> filter -> refine x K -> GMCR -> join) as
+ * fused launches that keep domains on chip.  Results equal the staged calls above. */
+typedef struct bms2_pipeline_options {
+  uint32_t refinement_steps; /* K (BMS2 -i) */
+  uint32_t find_first;       /* 1: first-match, 0: count-all (BMS2 --find-all) */
+  uint32_t block_waves;      /* waves per workgroup of the fused filter/join kernel; 0 = tuned */
+  uint32_t join_waves;       /* 0 = tuned */
+  uint32_t reserved[4];
+} bms2_pipeline_options;
+
+typedef struct bms2_pipeline_s* bms2_pipeline;
+int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
+                         const bms2_pipeline_options* opt, bms2_pipeline* out);
+int bms2_pipeline_destroy(bms2_pipeline p);
+/* Runs the whole sequence.  d_counts: u64[num_data_graphs * num_query_graphs], data-graph major
+ * (pair (g, q) at g*Q + q), 0 for pairs outside the GMCR.  d_valid: u64 bit rows
+ * [num_data_graphs][ceil(Q/64)], bit q set when (q, g) is a GMCR pair.  d_stats (nullable)
+ * u64[8]: {GMCR pairs, matched pairs, total embeddings, DFS nodes expanded, 0...}.  ev brackets
+ * the whole call; kernel_ev (nullable) only the fused screen + join kernel. */
+int bms2_pipeline_run(bms2_pipeline p, uint64_t* d_counts, uint64_t* d_valid, uint64_t* d_stats,
+                      bms2_event* ev, bms2_event* kernel_ev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BMS2_H */

@@ -1,0 +1,91 @@
+"""HIP path vs the CPU oracle, through the C ABI (bit-exact: everything here is integer work)."""
+import numpy as np
+import pytest
+
+import oracle
+from bms2_amd import api, graphs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def queue():
+    q = api.Queue(0)
+    yield q
+    q.close()
+
+
+def dense_from_gmcr(offsets, indices, counts, D, Q):
+    dense = np.zeros((D, Q), np.uint64)
+    valid = np.zeros((D, Q), bool)
+    for g in range(D):
+        s, e = int(offsets[g]), int(offsets[g + 1])
+        dense[g, indices[s:e]] = counts[s:e]
+        valid[g, indices[s:e]] = True
+    return dense, valid
+
+
+def unpack_valid(v, Q):
+    bits = np.unpackbits(v.view(np.uint8), axis=1, bitorder="little")
+    return bits[:, :Q].astype(bool)
+
+
+def test_signatures_match_oracle(queue, molecules_2k):
+    data, queries = molecules_2k
+    out = api.run_staged(queue, queries, data, K=6, keep_domains=True)
+    ref = oracle.staged(queries, data, 6, keep_domains=True)
+    for key in ("dlabel", "qlabel", "dpath", "qpath", "dcycle", "qcycle"):
+        assert np.array_equal(out["signatures"][key], ref.signatures[key]), key
+    for k in range(1, 7):
+        assert np.array_equal(out["signatures"][f"dsig{k}"], ref.signatures[f"dsig{k}"]), k
+        assert np.array_equal(out["signatures"][f"qsig{k}"], ref.signatures[f"qsig{k}"]), k
+
+
+@pytest.mark.parametrize("K", [0, 2, 6])
+@pytest.mark.parametrize("find_first", [False, True])
+def test_staged_matches_oracle_every_iteration(queue, molecules_2k, K, find_first):
+    data, queries = molecules_2k
+    out = api.run_staged(queue, queries, data, K=K, find_first=find_first)
+    ref = oracle.staged(queries, data, K, find_first=find_first)
+    assert len(out["domains"]) == K + 1
+    for it, (a, b) in enumerate(zip(out["domains"], ref.domains)):
+        assert np.array_equal(a, b), f"domains differ after iteration {it}"
+    off, idx = out["gmcr"]
+    assert np.array_equal(off, ref.gmcr_offsets)
+    assert np.array_equal(idx, ref.gmcr_indices)
+    assert np.array_equal(out["counts"], ref.counts)
+    assert out["total"] == ref.total
+
+
+@pytest.mark.parametrize("K", [0, 2, 6])
+@pytest.mark.parametrize("find_first", [False, True])
+def test_fused_pipeline_matches_oracle(queue, molecules_2k, K, find_first):
+    data, queries = molecules_2k
+    counts, valid, stats = api.run_pipeline(queue, queries, data, K, find_first)
+    rc, rv, rs = oracle.pipeline(queries, data, K, find_first)
+    assert np.array_equal(valid, rv)
+    assert np.array_equal(counts, rc)
+    assert int(stats[0]) == int(rs[0])  # GMCR pairs
+    assert int(stats[1]) == int(rs[1])  # matched pairs
+    assert int(stats[2]) == int(rs[2])  # embeddings
+    if not find_first:
+        assert int(stats[3]) == int(rs[3])  # DFS nodes (identity order)
+
+
+def test_fused_equals_staged(queue, molecules_2k):
+    data, queries = molecules_2k
+    out = api.run_staged(queue, queries, data, K=6, keep_domains=False)
+    counts, valid, _ = api.run_pipeline(queue, queries, data, 6, False)
+    off, idx = out["gmcr"]
+    dense, dvalid = dense_from_gmcr(off, idx, out["counts"], data.num_graphs, queries.num_graphs)
+    assert np.array_equal(unpack_valid(valid, queries.num_graphs), dvalid)
+    assert np.array_equal(counts, dense)
+
+
+def test_fused_1m_shard(queue, molecules_1m_shard):
+    data, queries = molecules_1m_shard
+    counts, valid, stats = api.run_pipeline(queue, queries, data, 6, False)
+    rc, rv, rs = oracle.pipeline(queries, data, 6, False)
+    assert np.array_equal(valid, rv)
+    assert np.array_equal(counts, rc)
+    assert np.array_equal(stats[:4], rs)
