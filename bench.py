@@ -135,8 +135,8 @@ def main() -> int:
         cpu = {"value": round(S * Q / cpu_s, 1), "unit": "pairs/s", "cores": threads,
                "kind": "port", "seconds": round(cpu_s, 2),
                "sample": f"first {S} of {D} data graphs x all {Q} queries, same K and mode"}
-        gc, gv = pipe.results(counts, valid)
-        parity = bool(np.array_equal(gc[:S], rc) and np.array_equal(gv[:S], rv))
+        gc, gv = pipe.results(counts, valid, rows=S)
+        parity = bool(np.array_equal(gc, rc) and np.array_equal(gv, rv))
         print(f"[bench] CPU {cpu['value']:.4g} pairs/s in {cpu_s:.1f}s; GPU/oracle counts on the "
               f"sample equal: {parity}", file=sys.stderr, flush=True)
 

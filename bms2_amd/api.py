@@ -397,9 +397,11 @@ class Pipeline(_Handle):
             return _ev(e), _ev(k)
         return _ev(e)
 
-    def results(self, counts: DeviceBuffer, valid: DeviceBuffer):
-        c = counts.to_numpy(np.uint64, self.D * self.Q).reshape(self.D, self.Q)
-        v = valid.to_numpy(np.uint64, self.D * self.VW).reshape(self.D, self.VW)
+    def results(self, counts: DeviceBuffer, valid: DeviceBuffer, rows: int | None = None):
+        """Host copies of the first `rows` data graphs' counts and GMCR bit rows."""
+        rows = self.D if rows is None else min(int(rows), self.D)
+        c = counts.to_numpy(np.uint64, rows * self.Q).reshape(rows, self.Q)
+        v = valid.to_numpy(np.uint64, rows * self.VW).reshape(rows, self.VW)
         return c, v
 
 

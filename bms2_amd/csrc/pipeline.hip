@@ -1,16 +1,18 @@
-// Fused batched pipeline: signatures -> filter -> K refinement steps -> GMCR -> join in one
-// launch per resident data batch (the stage sequence of BMS2.cpp:131-234).
+// Batched pipeline: signatures -> filter -> K refinement steps -> GMCR -> join over a resident
+// data batch (the stage sequence of BMS2.cpp:131-234), as two kernels per chunk of data packs.
 //
-// One wave64 owns one pack of whole data graphs (<= 64 nodes, <= 8 graphs) for its lifetime:
-//   1. node signature tuples (label, degree, path Bloom, ring counts, shell histograms) are
-//      computed on chip, one node per lane, and kept in VGPRs;
-//   2. every query is screened against the pack: its nodes' signature records come in through
-//      scalar loads, each node's domain is one u64 ballot over the lanes, and a graph of the pack
-//      drops out as soon as one query node has no candidate in it (GMCR validity, gmcr.hpp:54-88);
-//   3. surviving (query, graph) pairs go to an LDS queue; the wave counts their embeddings with one
-//      DFS per lane (isomorphism.hpp:291-472 semantics, standard backtracking), lanes refilling from
-//      the queue as their pair finishes, so divergent DFS sizes do not idle the wave.
-// Results: u64 count per (data graph, query), data-graph major, and GMCR membership bits.
+// screen_kernel: one wave64 per pack of whole data graphs (<= 64 nodes, <= 8 graphs).
+//   1. Node signature tuples (label, degree, path Bloom, ring counts, shell histograms) are built
+//      on chip, one node per lane, and stay in VGPRs; the pack's adjacency rows go to HBM for the
+//      join.
+//   2. Every query is screened against the pack.  Its node records arrive through scalar loads in
+//      a most-selective-first order; a node's domain is one u64 ballot over the lanes, and a graph
+//      drops out as soon as one query node has no candidate in it (GMCR validity,
+//      gmcr.hpp:54-88).  Pairs outside the GMCR get count 0 immediately.
+//   3. GMCR pairs (query, graph, domains in matching order) are staged in LDS and appended to an
+//      HBM record buffer in batches.
+// join_kernel: persistent; one DFS per lane over the records (isomorphism.hpp:291-472 semantics
+//   with standard backtracking), lanes refilling from a global cursor as their pair finishes.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -20,54 +22,49 @@
 #include "runtime.hpp"
 #include "tuples.hpp"
 
+#define BMS2_CONST __attribute__((address_space(4)))
+
 namespace bms2 {
 
-constexpr int kPackGraphs = 8;                 // graphs per pack
-constexpr int kQueueSlots = kWave + kPackGraphs;  // LDS pair records
+constexpr int kPackGraphs = 8;   // graphs per pack
+constexpr int kStageRecs = 32;   // LDS-staged pair records per wave
+constexpr int kRecDwords = 48;   // screening record of one query node
 
-// Query node signature record, read with scalar loads (48 dwords).
-struct alignas(16) QNode {
-  uint32_t label;
-  uint32_t deg;
-  uint32_t level_mask;  // bit k: shell k+1 histogram non-zero
-  uint32_t pad0;
-  uint32_t path[8];
-  uint32_t cyc[2];
-  uint32_t pad1[2];
-  // Per level: even / odd nibbles of the low and high histogram words, each nibble in its own
-  // byte, plus 15 per byte: byte = q + 15, so (byte - d) has bit 4 set exactly when q > d.
-  uint32_t hist[kMaxLevels][4];
-};
-static_assert(sizeof(QNode) == 192, "QNode layout");
-
-// Per-query metadata (32 dwords).
-struct alignas(16) QMeta {
-  uint32_t node_begin;
-  uint32_t nq;
-  uint32_t valid;       // nq > 1 (gmcr.hpp:64)
-  uint32_t pad;
-  uint8_t forder[32];   // screening order of local query nodes (most selective first)
-  uint8_t jorder[32];   // matching order: query node matched at each DFS depth
-  uint32_t pad2[4];
-};
-static_assert(sizeof(QMeta) == 96, "QMeta layout");
-
-struct FusedParams {
+// Screening record layout (u32 words): [0] label | deg << 8 | local id << 16 | level mask << 24,
+// [1..8] path Bloom, [9..10] ring counts, [12 + 4k .. 15 + 4k] shell level k.  Shell words hold
+// the even / odd nibbles of the low and high histogram words, one nibble per byte, plus 15 per
+// byte: byte = q + 15, so (byte - d) has bit 4 set exactly when q > d.
+struct ScreenParams {
   const uint32_t* graph_offsets;
   const uint32_t* row_offsets;
   const uint32_t* column_indices;
   const uint8_t* labels;
-  const uint32_t* pack_gbegin;  // [num_packs + 1]
-  uint32_t num_packs;
+  const uint32_t* pack_gbegin;
+  uint32_t pack_begin;
   uint32_t Q;
-  const QMeta* qmeta;
-  const QNode* qnodes;
-  const uint32_t* qadj;  // [Q][32] in matching order: bit i of word d = depth i adjacent to d
-  uint64_t* counts;      // [D][Q]
-  uint64_t* valid;       // [D][VW]
-  uint64_t* stats;       // [4] or null
+  const uint32_t* qhead;    // [Q][4]: first record, nq, valid (nq > 1, gmcr.hpp:64), 0
+  const uint32_t* qpre;     // [Q][16]: required presence bits [0..3], path union [4..11], valid [12]
+  const uint32_t* qrec;     // [Qn][kRecDwords] records in screening order per query
+  const uint8_t* jorder;    // [Q][32] local query node matched at each DFS depth
+  uint64_t* adj_out;        // [Dn] adjacency rows over pack lanes
+  uint64_t* counts;         // [D][Q]
+  uint64_t* valid;          // [D][VW]
   uint32_t VW;
-  uint32_t nq_cap;       // max query nodes, sizes the LDS records and stacks
+  uint32_t rec_words;       // u64 words per pair record: 2 + nq_cap
+  uint64_t* recs;
+  uint64_t rec_cap;
+  uint64_t* ctl;            // [0] records reserved, [1] overflow, [2] join cursor, [3..6] stats
+};
+
+struct JoinParams {
+  const uint64_t* recs;
+  uint64_t rec_cap;
+  uint32_t rec_words;
+  uint32_t Q;
+  const uint64_t* adj;      // [Dn]
+  const uint32_t* qadj;     // [Q][32]: bit i of word d = depths i and d adjacent
+  uint64_t* counts;
+  uint64_t* ctl;
   uint32_t find_first;
 };
 
@@ -77,46 +74,12 @@ struct QueryPrepParams {
   const uint32_t* column_indices;
   const uint8_t* labels;
   const uint32_t* pack_gbegin;
-  uint32_t num_packs;
-  QNode* qnodes;
+  const uint32_t* rec_pos;  // [Qn] record slot of each query node
+  uint32_t* qrec;
 };
 
-// LDS layout of the fused kernel (per wave; one wave per workgroup).
-struct FusedLds {
-  uint64_t* adj;    // [64]
-  uint64_t* m;      // [32] domain word per local query node of the current query
-  uint8_t* lbl;     // [64]
-  uint32_t* rec;    // [kQueueSlots][rec_words]  (aliases the Bloom scratch before the query loop)
-  uint64_t* rstk;   // [nq_cap][64] remaining candidates per depth
-  uint8_t* fstk;    // [nq_cap][64] matched data node per depth
-  uint32_t rec_words;
-};
-
-__host__ __device__ inline uint32_t fused_rec_words(uint32_t nq_cap) { return 2 + 3 * nq_cap; }
-
-__host__ __device__ inline size_t fused_lds_bytes(uint32_t nq_cap) {
-  size_t b = 64 * 8 + 32 * 8 + 64;
-  size_t rec = size_t(kQueueSlots) * fused_rec_words(nq_cap) * 4;
-  b += std::max<size_t>(rec, 64 * 8 * 4);
-  b += size_t(nq_cap) * 64 * 8 + size_t(nq_cap) * 64;
-  return (b + 15) & ~size_t(15);
-}
-
-__device__ inline FusedLds carve(uint8_t* base, uint32_t nq_cap) {
-  FusedLds l;
-  l.adj = reinterpret_cast<uint64_t*>(base);
-  base += 64 * 8;
-  l.m = reinterpret_cast<uint64_t*>(base);
-  base += 32 * 8;
-  l.lbl = base;
-  base += 64;
-  l.rec = reinterpret_cast<uint32_t*>(base);
-  l.rec_words = fused_rec_words(nq_cap);
-  base += std::max<size_t>(size_t(kQueueSlots) * l.rec_words * 4, 64 * 8 * 4);
-  l.rstk = reinterpret_cast<uint64_t*>(base);
-  base += size_t(nq_cap) * 64 * 8;
-  l.fstk = base;
-  return l;
+__device__ inline uint32_t cload(const uint32_t* p, uint32_t i) {
+  return ((const BMS2_CONST uint32_t*)(p))[i];
 }
 
 // ------------------------------------------------------------------ query signature records
@@ -134,52 +97,80 @@ __global__ __launch_bounds__(64) void query_prep_kernel(QueryPrepParams p) {
                     t);
   const uint32_t lane = lane_id();
   if (lane >= nn) return;
-  QNode r;
-  r.label = t.label;
-  r.deg = t.deg;
-  r.level_mask = 0;
-  r.pad0 = 0;
+  const uint32_t node = n0 + lane;
+  // local id inside its query graph
+  uint32_t g = gb;
+  while (p.graph_offsets[g + 1] <= node) ++g;
+  const uint32_t local = node - p.graph_offsets[g];
+  uint32_t lm = 0;
 #pragma unroll
-  for (int w = 0; w < 8; ++w) r.path[w] = t.path[w];
-  r.cyc[0] = uint32_t(t.cycle);
-  r.cyc[1] = uint32_t(t.cycle >> 32);
-  r.pad1[0] = r.pad1[1] = 0;
+  for (int k = 0; k < kMaxLevels; ++k)
+    if (t.hist[k]) lm |= 1u << k;
+  uint32_t* r = p.qrec + size_t(p.rec_pos[node]) * kRecDwords;
+  r[0] = (t.label & 255u) | (min(t.deg, 255u) << 8) | (local << 16) | (lm << 24);
+#pragma unroll
+  for (int w = 0; w < 8; ++w) r[1 + w] = t.path[w];
+  r[9] = uint32_t(t.cycle);
+  r[10] = uint32_t(t.cycle >> 32);
+  r[11] = 0;
 #pragma unroll
   for (int k = 0; k < kMaxLevels; ++k) {
-    const uint64_t h = t.hist[k];
-    if (h) r.level_mask |= 1u << k;
-    const uint32_t lo = uint32_t(h), hi = uint32_t(h >> 32);
-    r.hist[k][0] = (lo & 0x0f0f0f0fu) + 0x0f0f0f0fu;
-    r.hist[k][1] = ((lo >> 4) & 0x0f0f0f0fu) + 0x0f0f0f0fu;
-    r.hist[k][2] = (hi & 0x0f0f0f0fu) + 0x0f0f0f0fu;
-    r.hist[k][3] = ((hi >> 4) & 0x0f0f0f0fu) + 0x0f0f0f0fu;
+    const uint32_t lo = uint32_t(t.hist[k]), hi = uint32_t(t.hist[k] >> 32);
+    r[12 + 4 * k + 0] = (lo & 0x0f0f0f0fu) + 0x0f0f0f0fu;
+    r[12 + 4 * k + 1] = ((lo >> 4) & 0x0f0f0f0fu) + 0x0f0f0f0fu;
+    r[12 + 4 * k + 2] = (hi & 0x0f0f0f0fu) + 0x0f0f0f0fu;
+    r[12 + 4 * k + 3] = ((hi >> 4) & 0x0f0f0f0fu) + 0x0f0f0f0fu;
   }
-  p.qnodes[n0 + lane] = r;
+  for (int w = 12 + 4 * kMaxLevels; w < kRecDwords; ++w) r[w] = 0;
 }
 
-// ------------------------------------------------------------------ fused screen + join kernel
+// Per-query prefilter record: [0..3] presence bits every node needs (label < 16 at its degree
+// threshold, capped at 7), [4..11] union of the nodes' path Bloom filters, [12] nq > 1.
+__global__ void query_pre_kernel(const uint32_t* __restrict__ qhead, const uint32_t* __restrict__ qrec,
+                                 uint32_t Q, uint32_t* __restrict__ qpre) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= Q) return;
+  const uint32_t rb = qhead[4 * q], nq = qhead[4 * q + 1];
+  uint32_t out[16] = {};
+  for (uint32_t i = 0; i < nq; ++i) {
+    const uint32_t* r = qrec + size_t(rb + i) * kRecDwords;
+    const uint32_t l = r[0] & 255u, d = (r[0] >> 8) & 255u;
+    if (l < 16) {
+      const uint32_t bit = l * 8 + min(d, 7u);
+      out[bit >> 5] |= 1u << (bit & 31);
+    }
+    for (int w = 0; w < 8; ++w) out[4 + w] |= r[1 + w];
+  }
+  out[12] = nq > 1 ? 1u : 0u;
+  for (int w = 0; w < 16; ++w) qpre[size_t(q) * 16 + w] = out[w];
+}
+
+// ------------------------------------------------------------------ screening
 
 template <int K>
-__global__ __launch_bounds__(64) void fused_kernel(FusedParams p) {
-  extern __shared__ __align__(16) uint8_t smem[];
-  const FusedLds L = carve(smem, p.nq_cap);
+__global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
+  __shared__ uint64_t adj[64];
+  __shared__ uint8_t lbl[64];
+  __shared__ uint64_t dom[32];
+  extern __shared__ __align__(16) uint64_t stage[];  // kStageRecs records; Bloom scratch first
   const uint32_t lane = lane_id();
-  const uint32_t pack = blockIdx.x;
+  const uint32_t pack = p.pack_begin + blockIdx.x;
   const uint32_t gb = p.pack_gbegin[pack], ge = p.pack_gbegin[pack + 1];
   const uint32_t ng = ge - gb;
   const uint32_t n0 = p.graph_offsets[gb], nn = p.graph_offsets[ge] - n0;
   const uint64_t node_mask = low_mask(nn);
 
-  // ---- 1. data node tuples (VGPR resident)
   Tuple t;
   compute_tuples<K>(n0, nn, p.row_offsets, p.column_indices, p.labels,
-                    PackScratch{L.adj, L.lbl, L.rec}, t);
+                    PackScratch{adj, lbl, reinterpret_cast<uint32_t*>(stage)}, t);
+  if (lane < nn) p.adj_out[n0 + lane] = adj[lane];
+
   const uint32_t d_label = lane < nn ? t.label : 0x100u;  // never equal to a u8 query label
   const uint32_t d_deg = t.deg;
   uint32_t d_pnot[8];
 #pragma unroll
   for (int w = 0; w < 8; ++w) d_pnot[w] = ~t.path[w];
-  // Ring counts with the top bit of each u16 set: (d | 0x8000) - q has bit 15 set iff d >= q.
+  // ring counts with the top bit of each u16 set: (d | 0x8000) - q has bit 15 set iff d >= q
   const uint32_t d_cyc0 = uint32_t(t.cycle) | 0x80008000u;
   const uint32_t d_cyc1 = uint32_t(t.cycle >> 32) | 0x80008000u;
   uint32_t d_hist[K > 0 ? K : 1][4];
@@ -192,232 +183,273 @@ __global__ __launch_bounds__(64) void fused_kernel(FusedParams p) {
     d_hist[k][3] = (hi >> 4) & 0x0f0f0f0fu;
   }
 
-  // graph lane masks of the pack (uniform)
-  uint64_t gm[kPackGraphs];
-#pragma unroll
-  for (int j = 0; j < kPackGraphs; ++j) {
-    gm[j] = 0;
-    if (uint32_t(j) < ng) {
-      const uint32_t s = p.graph_offsets[gb + j] - n0, e = p.graph_offsets[gb + j + 1] - n0;
-      gm[j] = low_mask(e) & ~low_mask(s);
+  // Graph segments: node lanes of a graph are contiguous.  mytop = last lane of the node's graph
+  // (64 for lanes past the pack); graph-slot lanes j < ng hold the graph id and its top lane.
+  uint32_t mytop = 64, mygraph = 0, slot_top = 64, slot_gid = 0;
+  for (uint32_t j = 0; j < ng; ++j) {
+    const uint32_t s = p.graph_offsets[gb + j] - n0, e = p.graph_offsets[gb + j + 1] - n0;
+    if (lane >= s && lane < e) {
+      mytop = e - 1;
+      mygraph = gb + j;
+    }
+    if (lane == j) {
+      slot_gid = gb + j;
+      slot_top = e > s ? e - 1 : 64;
     }
   }
+  const uint64_t T = ballot(lane < nn && lane == mytop);  // top lane of every non-empty graph
+  const uint64_t S = ((T << 1) | 1) & node_mask;           // first lane of every non-empty graph
 
-  // ---- join state (per lane) and queue state (uniform)
-  int slot = -1;        // LDS record processed by this lane, -1 when idle
-  int depth = 0;
-  uint32_t jq = 0, jg = 0, jnq = 0;
-  uint64_t used = 0, cnt = 0, nodes = 0;
-  uint64_t st_matched = 0, st_emb = 0, st_nodes = 0;
-  uint64_t free_lo = ~uint64_t(0), free_hi = low_mask(kQueueSlots - 64);
-  uint64_t pend_lo = 0, pend_hi = 0;
-  uint64_t st_pairs = 0;
-  uint64_t vbits = 0;  // lane j < ng: GMCR bits of graph j for the current block of 64 queries
-  const uint32_t RW = L.rec_words;
-  const uint32_t ncap = p.nq_cap;
+  const uint32_t RW = p.rec_words;
+  uint32_t nstage = 0;
+  uint64_t vbits = 0, pairs = 0;
 
-  auto rec_ptr = [&](int s) { return L.rec + size_t(s) * RW; };
-
-  // One DFS step for every busy lane; returns nothing, updates lane state.
-  auto join_phase = [&](bool drain) {
-    while (true) {
-      // hand pending records to idle lanes
-      uint64_t idle = ballot(slot < 0);
-      bool fresh = false;
-      while (idle && (pend_lo | pend_hi)) {
-        const int l = ctz64(idle);
-        idle &= idle - 1;
-        int s;
-        if (pend_lo) {
-          s = ctz64(pend_lo);
-          pend_lo &= pend_lo - 1;
-        } else {
-          s = 64 + ctz64(pend_hi);
-          pend_hi &= pend_hi - 1;
-        }
-        if (int(lane) == l) {
-          slot = s;
-          fresh = true;
-        }
-      }
-      if (fresh) {
-        const uint32_t* r = rec_ptr(slot);
-        jq = r[0];
-        jg = r[1] & 0xffu;
-        jnq = r[1] >> 8;
-        depth = 0;
-        used = 0;
-        cnt = 0;
-        nodes = 0;
-        L.rstk[lane] = *reinterpret_cast<const uint64_t*>(r + 2);  // domain of depth 0
-      }
-      const uint64_t busy = ballot(slot >= 0);
-      if (!busy) break;
-      if (!drain && !(pend_lo | pend_hi) && busy != ~uint64_t(0)) break;
-      bool finished = false;
-      if (slot >= 0) {
-        const uint32_t* r = rec_ptr(slot);
-        const uint64_t* dom = reinterpret_cast<const uint64_t*>(r + 2);
-        const uint32_t* qa = r + 2 + 2 * ncap;
-        uint64_t R = L.rstk[depth * 64 + lane];
-        if (R == 0) {
-          if (depth == 0) {
-            finished = true;
-          } else {
-            --depth;
-            used &= ~(uint64_t(1) << L.fstk[depth * 64 + lane]);
-          }
-        } else {
-          const int v = ctz64(R);
-          L.rstk[depth * 64 + lane] = R & (R - 1);
-          L.fstk[depth * 64 + lane] = uint8_t(v);
-          ++nodes;
-          const int nd = depth + 1;
-          // candidates of depth nd: domain, unused, and query adjacency == data adjacency to
-          // every matched node (induced, isomorphism.hpp:268-288)
-          const uint64_t vbit = uint64_t(1) << v;
-          uint64_t Rn = dom[nd] & ~(used | vbit);
-          const uint32_t am = qa[nd];
-          for (int i = 0; i < nd && Rn; ++i) {
-            const uint64_t a = L.adj[L.fstk[i * 64 + lane]];
-            Rn &= ((am >> i) & 1) ? a : ~a;
-          }
-          if (uint32_t(nd) == jnq - 1) {
-            cnt += uint64_t(popc64(Rn));
-            if (p.find_first && cnt) finished = true;
-          } else {
-            used |= vbit;
-            depth = nd;
-            L.rstk[nd * 64 + lane] = Rn;
-          }
-        }
-      }
-      if (finished) {
-        const uint64_t c = p.find_first ? (cnt ? 1 : 0) : cnt;
-        p.counts[uint64_t(gb + jg) * p.Q + jq] = c;
-        st_matched += c ? 1 : 0;
-        st_emb += c;
-        st_nodes += nodes;
-      }
-      uint64_t fin = ballot(finished);
-      while (fin) {
-        const int l = ctz64(fin);
-        fin &= fin - 1;
-        const int s = __builtin_amdgcn_readlane(slot, l);
-        if (s < 64)
-          free_lo |= uint64_t(1) << s;
-        else
-          free_hi |= uint64_t(1) << (s - 64);
-      }
-      if (finished) slot = -1;
+  auto flush = [&]() {
+    if (!nstage) return;
+    __syncthreads();
+    uint64_t base = 0;
+    if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl), nstage);
+    base = uniform64(readlane64(base, 0));
+    if (base + nstage > p.rec_cap) {
+      if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(p.ctl + 1), 1ull);
+    } else {
+      uint64_t* dst = p.recs + base * RW;
+      for (uint32_t w = lane; w < nstage * RW; w += 64) dst[w] = stage[w];
     }
+    __syncthreads();
+    nstage = 0;
   };
 
-  // ---- 2. screen every query against the pack
-  for (uint32_t q = 0; q < p.Q; ++q) {
-    const QMeta* meta = p.qmeta + q;
-    const uint32_t nq = meta->nq;
-    uint32_t alive = meta->valid ? (ng >= 32 ? ~0u : ((1u << ng) - 1)) : 0u;
-    uint64_t lanes_alive = node_mask;
-    if (alive) {
-      const QNode* qbase = p.qnodes + meta->node_begin;
-      for (uint32_t ti = 0; ti < nq; ++ti) {
-        const uint32_t i = meta->forder[ti];
-        const QNode* qn = qbase + i;
-        const uint32_t ql = qn->label;
-        uint64_t m = ballot((d_label == ql || ql == kWildcard) && d_deg >= qn->deg) & lanes_alive;
+  // Pack summary for the query prefilter: presence bit (label l < 16, degree threshold t < 8)
+  // set when some node has label l and degree >= t; union of the path Bloom filters.
+  uint32_t pres[4] = {0, 0, 0, 0}, punion[8];
+  if (lane < nn && t.label < 16) {
+    const uint32_t top = min(t.deg, 7u), base = t.label * 8;
+    const uint64_t bitsl = (uint64_t(2) << top) - 1;  // thresholds 0..top
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const int lo = 32 * w;
+      if (base + 8 > uint32_t(lo) && base < uint32_t(lo + 32))
+        pres[w] = uint32_t(bitsl << (base - lo));
+    }
+  }
+#pragma unroll
+  for (int w = 0; w < 8; ++w) punion[w] = lane < nn ? t.path[w] : 0u;
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int w = 0; w < 4; ++w) pres[w] |= __shfl_xor(pres[w], off);
+#pragma unroll
+    for (int w = 0; w < 8; ++w) punion[w] |= __shfl_xor(punion[w], off);
+  }
+
+  for (uint32_t qb = 0; qb < p.Q; qb += 64) {
+    // 64 queries at once, one per lane: every node's (label, degree) present in the pack and
+    // every node's path bits inside the pack's union (both necessary for a candidate).
+    const uint32_t qq = qb + lane;
+    bool pass = false;
+    if (qq < p.Q) {
+      const uint4* pr = reinterpret_cast<const uint4*>(p.qpre + size_t(qq) * 16);
+      const uint4 a = pr[0], b = pr[1], c = pr[2], e = pr[3];
+      pass = e.x != 0 && !((a.x & ~pres[0]) | (a.y & ~pres[1]) | (a.z & ~pres[2]) | (a.w & ~pres[3]) |
+                           (b.x & ~punion[0]) | (b.y & ~punion[1]) | (b.z & ~punion[2]) |
+                           (b.w & ~punion[3]) | (c.x & ~punion[4]) | (c.y & ~punion[5]) |
+                           (c.z & ~punion[6]) | (c.w & ~punion[7]));
+    }
+    uint64_t cand = ballot(pass);
+    while (cand) {
+      const uint32_t qu = qb + uint32_t(ctz64(cand));
+      cand &= cand - 1;
+      const uint32_t rb = cload(p.qhead, 4 * qu), nq = cload(p.qhead, 4 * qu + 1);
+      uint64_t tops = T;
+      uint64_t lanes = node_mask;
+      for (uint32_t ti = 0; ti < nq && tops; ++ti) {
+        const uint32_t* r = p.qrec + size_t(rb + ti) * kRecDwords;
+        // one batch of scalar loads for label, degree, path and ring words
+        uint32_t rr[12];
+#pragma unroll
+        for (int w = 0; w < 12; ++w) rr[w] = cload(r, w);
+        const uint32_t w0 = rr[0];
+        const uint32_t ql = w0 & 255u, qd = (w0 >> 8) & 255u, local = (w0 >> 16) & 255u;
+        const uint32_t lm = w0 >> 24;
+        uint64_t m = ballot((d_label == ql || ql == kWildcard) && d_deg >= qd) & lanes;
         if (m) {
           uint32_t acc = 0;
 #pragma unroll
-          for (int w = 0; w < 8; ++w) acc |= qn->path[w] & d_pnot[w];
+          for (int w = 0; w < 8; ++w) acc |= rr[1 + w] & d_pnot[w];
           m &= ballot(acc == 0);
         }
         if (m) {
-          const uint32_t c0 = d_cyc0 - qn->cyc[0], c1 = d_cyc1 - qn->cyc[1];
+          const uint32_t c0 = d_cyc0 - rr[9], c1 = d_cyc1 - rr[10];
           m &= ballot(((c0 & c1) & 0x80008000u) == 0x80008000u);
         }
-        if (K > 0 && m) {
-          const uint32_t lm = qn->level_mask;
+        if (K > 0 && m && lm) {
           uint32_t fail = 0;
 #pragma unroll
           for (int k = 0; k < K; ++k) {
             if ((lm >> k) & 1) {
-              fail |= (qn->hist[k][0] - d_hist[k][0]) | (qn->hist[k][1] - d_hist[k][1]) |
-                      (qn->hist[k][2] - d_hist[k][2]) | (qn->hist[k][3] - d_hist[k][3]);
+              fail |= (cload(r, 12 + 4 * k) - d_hist[k][0]) | (cload(r, 13 + 4 * k) - d_hist[k][1]) |
+                      (cload(r, 14 + 4 * k) - d_hist[k][2]) | (cload(r, 15 + 4 * k) - d_hist[k][3]);
             }
           }
           m &= ballot((fail & 0x10101010u) == 0);
         }
-        if (lane == 0) L.m[i] = m;
-#pragma unroll
-        for (int j = 0; j < kPackGraphs; ++j) {
-          if (((alive >> j) & 1) && !(m & gm[j])) {
-            alive &= ~(1u << j);
-            lanes_alive &= ~gm[j];
-          }
+        if (lane == 0) dom[local] = m;
+        // Graphs left with a candidate: per segment, (m | top) - first has its top bit set
+        // unless every non-top bit of m is clear, in which case m's own top bit decides.
+        const uint64_t u = (m | T) - S;
+        const uint64_t alive = ((u | m) & T) & tops;
+        if (alive != tops) {
+          tops = alive;
+          lanes = ballot(mytop < 64 && ((tops >> mytop) & 1)) & node_mask;
         }
-        if (!alive) break;
       }
+      if (!tops) continue;
+      __syncthreads();  // dom[] visible to all lanes
+      if (lane < ng && slot_top < 64 && ((tops >> slot_top) & 1)) vbits |= uint64_t(1) << (qu & 63);
+      // GMCR pairs -> staged records {q | g << 32, n0 | nq << 32, domains in matching order}
+      uint64_t at = tops;
+      while (at) {
+        const int tb = ctz64(at);
+        at &= at - 1;
+        const uint64_t seg = ballot(mytop == uint32_t(tb));
+        const uint32_t g = __builtin_amdgcn_readlane(mygraph, tb);
+        uint64_t* rec = stage + size_t(nstage) * RW;
+        if (lane == 0) {
+          rec[0] = uint64_t(qu) | (uint64_t(g) << 32);
+          rec[1] = uint64_t(n0) | (uint64_t(nq) << 32);
+        }
+        if (lane < nq) rec[2 + lane] = dom[p.jorder[size_t(qu) * 32 + lane]] & seg;
+        ++nstage;
+        ++pairs;
+      }
+      if (nstage > uint32_t(kStageRecs - kPackGraphs)) flush();
     }
-    __syncthreads();  // L.m visible to every lane
-    // pairs outside the GMCR: count 0
-    if (lane < ng && !((alive >> lane) & 1)) p.counts[uint64_t(gb + lane) * p.Q + q] = 0;
-    if (lane < ng) vbits |= uint64_t((alive >> lane) & 1) << (q & 63);
-    if ((q & 63) == 63 || q + 1 == p.Q) {
-      if (lane < ng) p.valid[uint64_t(gb + lane) * p.VW + (q >> 6)] = vbits;
-      vbits = 0;
+    // GMCR bit rows of the block, and count 0 for every pair of the block outside the GMCR
+    // (one coalesced 512-byte store per graph)
+    if (lane < ng) p.valid[uint64_t(slot_gid) * p.VW + (qb >> 6)] = vbits;
+    for (uint32_t j = 0; j < ng; ++j) {
+      const uint64_t vb = readlane64(vbits, j);
+      const uint32_t gj = __builtin_amdgcn_readlane(slot_gid, j);
+      if (qq < p.Q && !((vb >> lane) & 1)) p.counts[uint64_t(gj) * p.Q + qq] = 0;
     }
-    // GMCR pairs -> LDS records (domains and adjacency in matching order)
-    while (alive) {
-      const int j = __builtin_ctz(alive);
-      alive &= alive - 1;
-      int s;
-      if (free_lo) {
-        s = ctz64(free_lo);
-        free_lo &= free_lo - 1;
-        pend_lo |= uint64_t(1) << s;
-      } else {
-        s = 64 + ctz64(free_hi);
-        free_hi &= free_hi - 1;
-        pend_hi |= uint64_t(1) << (s - 64);
-      }
-      ++st_pairs;
-      uint32_t* r = rec_ptr(s);
-      uint64_t gmj = 0;
-#pragma unroll
-      for (int jj = 0; jj < kPackGraphs; ++jj)
-        if (jj == j) gmj = gm[jj];
-      if (lane == 0) {
-        r[0] = q;
-        r[1] = uint32_t(j) | (nq << 8);
-      }
-      if (lane < nq) {
-        const uint32_t node = meta->jorder[lane];
-        reinterpret_cast<uint64_t*>(r + 2)[lane] = L.m[node] & gmj;
-        r[2 + 2 * ncap + lane] = p.qadj[size_t(q) * 32 + lane];
-      }
-    }
-    __syncthreads();
-    const uint64_t idle = ballot(slot < 0);
-    const uint32_t npend = uint32_t(popc64(pend_lo) + popc64(pend_hi));
-    if (npend && npend >= uint32_t(popc64(idle))) join_phase(false);
+    vbits = 0;
   }
-  join_phase(true);
+  flush();
+  if (lane == 0 && pairs) atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + 3), (unsigned long long)pairs);
+}
 
-  // ---- stats
-  if (p.stats) {
-    for (int off = 32; off > 0; off >>= 1) {
-      st_matched += __shfl_xor(st_matched, off);
-      st_emb += __shfl_xor(st_emb, off);
-      st_nodes += __shfl_xor(st_nodes, off);
+// ------------------------------------------------------------------ join
+
+template <int NQ>
+__global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
+  __shared__ uint64_t R[NQ][64];   // remaining candidates per depth
+  __shared__ uint64_t NF[NQ][64];  // neighbour row of the node matched per depth
+  __shared__ uint8_t F[NQ][64];    // node matched per depth
+  const uint32_t lane = lane_id();
+  const uint64_t total = min(p.ctl[0], p.rec_cap);
+  const uint32_t RW = p.rec_words;
+  bool busy = false, exhausted = false;
+  uint64_t rec = 0, used = 0, cnt = 0, nodes = 0;
+  uint32_t q = 0, g = 0, n0 = 0, nq = 0;
+  int depth = 0;
+  uint64_t st_matched = 0, st_emb = 0, st_nodes = 0;
+  const uint64_t* recs = p.recs;
+  // records claimed by this wave from the global cursor, handed to idle lanes without atomics
+  uint64_t local_next = 0, local_end = 0;
+  constexpr uint64_t kClaim = 256;
+  while (true) {
+    const uint64_t idle = ballot(!busy);
+    if (idle) {
+      if (local_next >= local_end && !exhausted) {
+        uint64_t base = 0;
+        if (lane == 0)
+          base = atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + 2), (unsigned long long)kClaim);
+        base = uniform64(readlane64(base, 0));
+        local_next = base;
+        local_end = base >= total ? base : min(base + kClaim, total);
+        if (base + kClaim >= total) exhausted = true;
+      }
+      const uint64_t take = min(local_end - local_next, uint64_t(popc64(idle)));
+      const uint64_t rank = uint64_t(popc64(idle & low_mask(lane)));
+      if (!busy && rank < take) {
+        busy = true;
+        rec = local_next + rank;
+        const uint64_t h0 = recs[rec * RW], h1 = recs[rec * RW + 1];
+        q = uint32_t(h0);
+        g = uint32_t(h0 >> 32);
+        n0 = uint32_t(h1);
+        nq = uint32_t(h1 >> 32);
+        depth = 0;
+        used = 0;
+        cnt = 0;
+        nodes = 0;
+        R[0][lane] = recs[rec * RW + 2];
+      }
+      local_next += take;
     }
-    if (lane == 0) {
-      atomicAdd(reinterpret_cast<unsigned long long*>(p.stats + 0), (unsigned long long)st_pairs);
-      atomicAdd(reinterpret_cast<unsigned long long*>(p.stats + 1), (unsigned long long)st_matched);
-      atomicAdd(reinterpret_cast<unsigned long long*>(p.stats + 2), (unsigned long long)st_emb);
-      atomicAdd(reinterpret_cast<unsigned long long*>(p.stats + 3), (unsigned long long)st_nodes);
+    if (!ballot(busy)) {
+      if (exhausted && local_next >= local_end) break;
+      continue;
+    }
+    bool finished = false;
+    if (busy) {
+      const uint64_t Rd = R[depth][lane];
+      if (Rd == 0) {
+        if (depth == 0) {
+          finished = true;
+        } else {
+          --depth;
+          used &= ~(uint64_t(1) << F[depth][lane]);
+        }
+      } else {
+        const int v = ctz64(Rd);
+        R[depth][lane] = Rd & (Rd - 1);
+        F[depth][lane] = uint8_t(v);
+        const uint64_t nv = p.adj[n0 + v];
+        NF[depth][lane] = nv;
+        ++nodes;
+        const int nd = depth + 1;
+        const uint64_t vbit = uint64_t(1) << v;
+        uint64_t Rn = recs[rec * RW + 2 + nd] & ~(used | vbit);
+        const uint32_t am = p.qadj[size_t(q) * 32 + nd];
+        Rn &= ((am >> depth) & 1) ? nv : ~nv;
+        for (int i = 0; i < depth; ++i) {
+          const uint64_t a = NF[i][lane];
+          Rn &= ((am >> i) & 1) ? a : ~a;
+        }
+        if (uint32_t(nd) == nq - 1) {
+          cnt += uint64_t(popc64(Rn));
+          if (p.find_first && cnt) finished = true;
+        } else {
+          used |= vbit;
+          depth = nd;
+          R[nd][lane] = Rn;
+        }
+      }
+    }
+    if (finished) {
+      const uint64_t c = p.find_first ? (cnt ? 1 : 0) : cnt;
+      p.counts[uint64_t(g) * p.Q + q] = c;
+      st_matched += c ? 1 : 0;
+      st_emb += c;
+      st_nodes += nodes;
+      busy = false;
     }
   }
+  for (int off = 32; off > 0; off >>= 1) {
+    st_matched += __shfl_xor(st_matched, off);
+    st_emb += __shfl_xor(st_emb, off);
+    st_nodes += __shfl_xor(st_nodes, off);
+  }
+  if (lane == 0 && st_matched + st_emb + st_nodes) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + 4), (unsigned long long)st_matched);
+    atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + 5), (unsigned long long)st_emb);
+    atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + 6), (unsigned long long)st_nodes);
+  }
+}
+
+__global__ void zero_kernel(uint64_t* p, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
 // ------------------------------------------------------------------ host side
@@ -466,12 +498,21 @@ struct bms2_pipeline_s {
   bms2_queue q;
   bms2_device_csr data, query;
   bms2_pipeline_options opt;
-  uint32_t num_packs = 0, num_qpacks = 0, nq_cap = 0;
+  uint32_t num_packs = 0, num_qpacks = 0, nq_cap = 0, rec_words = 0;
+  uint64_t rec_cap = 0;
+  double pairs_per_pack = 0.0;  // measured GMCR density, sizes the chunks
+  int join_grid = 0;
   uint32_t* d_packs = nullptr;
   uint32_t* d_qpacks = nullptr;
-  QMeta* d_qmeta = nullptr;
-  QNode* d_qnodes = nullptr;
+  uint32_t* d_qhead = nullptr;
+  uint32_t* d_qpre = nullptr;
+  uint32_t* d_qrec = nullptr;
+  uint32_t* d_qpos = nullptr;
+  uint8_t* d_jorder = nullptr;
   uint32_t* d_qadj = nullptr;
+  uint64_t* d_adj = nullptr;
+  uint64_t* d_recs = nullptr;
+  uint64_t* d_ctl = nullptr;
 };
 
 namespace {
@@ -480,10 +521,23 @@ void pipeline_free(bms2_pipeline p) {
   if (!p) return;
   (void)hipFree(p->d_packs);
   (void)hipFree(p->d_qpacks);
-  (void)hipFree(p->d_qmeta);
-  (void)hipFree(p->d_qnodes);
+  (void)hipFree(p->d_qhead);
+  (void)hipFree(p->d_qpre);
+  (void)hipFree(p->d_qrec);
+  (void)hipFree(p->d_qpos);
+  (void)hipFree(p->d_jorder);
   (void)hipFree(p->d_qadj);
+  (void)hipFree(p->d_adj);
+  (void)hipFree(p->d_recs);
+  (void)hipFree(p->d_ctl);
   delete p;
+}
+
+// Pair-record budget: a quarter of free device memory, at most 16 GiB.
+uint64_t record_budget_bytes() {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 1ull << 30;
+  return std::min<uint64_t>(free_b / 4, 16ull << 30);
 }
 
 }  // namespace
@@ -514,26 +568,31 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
   p->query = *query;
   p->opt = *opt;
   p->nq_cap = query->max_graph_nodes <= 16 ? 16 : 32;
+  p->rec_words = 2 + std::max<uint32_t>(query->max_graph_nodes, 2);
   hipStream_t s = q->stream;
   int rc = BMS2_OK;
   auto fail = [&](int code) {
     pipeline_free(p);
     return code;
   };
-  // data packs (plan built once per resident batch)
+  // data packs: the plan of one resident batch
   std::vector<uint32_t> go(data->num_graphs + 1);
   if (hipMemcpy(go.data(), data->graph_offsets, 4ull * go.size(), hipMemcpyDeviceToHost) !=
       hipSuccess)
     return fail(set_error(BMS2_ERR_HIP, "copy of data graph offsets failed"));
   std::vector<uint32_t> packs = build_packs(go);
   p->num_packs = uint32_t(packs.size() - 1);
-  if ((rc = device_alloc((void**)&p->d_packs, 4ull * packs.size()))) return fail(rc);
+  if ((rc = device_alloc((void**)&p->d_packs, 4ull * packs.size())) ||
+      (rc = device_alloc((void**)&p->d_adj, 8ull * std::max<uint64_t>(data->total_nodes, 1))) ||
+      (rc = device_alloc((void**)&p->d_ctl, 8 * 8)))
+    return fail(rc);
   BMS2_HIP(hipMemcpyAsync(p->d_packs, packs.data(), 4ull * packs.size(), hipMemcpyHostToDevice, s));
 
-  // queries: host copy of the small query batch for orders and adjacency
+  // queries: host copy of the (small) query batch for orders and adjacency
   const uint32_t Q = query->num_graphs;
-  std::vector<uint32_t> qgo(Q + 1), qro(query->total_nodes + 1), qci(query->total_edges);
-  std::vector<uint8_t> qlab(query->total_nodes);
+  const uint64_t Qn = query->total_nodes;
+  std::vector<uint32_t> qgo(Q + 1), qro(Qn + 1), qci(query->total_edges);
+  std::vector<uint8_t> qlab(Qn);
   if (hipMemcpy(qgo.data(), query->graph_offsets, 4ull * qgo.size(), hipMemcpyDeviceToHost) ||
       hipMemcpy(qro.data(), query->row_offsets, 4ull * qro.size(), hipMemcpyDeviceToHost) ||
       (qci.size() && hipMemcpy(qci.data(), query->column_indices, 4ull * qci.size(),
@@ -542,16 +601,15 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
     return fail(set_error(BMS2_ERR_HIP, "copy of the query batch failed"));
   std::vector<uint32_t> label_freq(256, 0);
   for (uint8_t l : qlab) label_freq[l]++;
-  std::vector<QMeta> meta(Q);
+  std::vector<uint32_t> qhead(size_t(Q) * 4, 0), qpos(std::max<uint64_t>(Qn, 1), 0);
+  std::vector<uint8_t> jorder(size_t(Q) * 32, 0);
   std::vector<uint32_t> qadj(size_t(Q) * 32, 0);
   for (uint32_t g = 0; g < Q; ++g) {
-    QMeta& m = meta[g];
-    std::memset(&m, 0, sizeof(m));
     const uint32_t b = qgo[g], n = qgo[g + 1] - qgo[g];
-    m.node_begin = b;
-    m.nq = n;
-    m.valid = n > 1 ? 1 : 0;
-    // screening order: rarest label first, then highest degree (only affects speed)
+    qhead[4 * g + 0] = b;
+    qhead[4 * g + 1] = n;
+    qhead[4 * g + 2] = n > 1 ? 1 : 0;
+    // screening order: rarest label first, then highest degree (affects speed only)
     std::vector<uint32_t> ord(n);
     std::iota(ord.begin(), ord.end(), 0u);
     std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t c) {
@@ -560,15 +618,14 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
       if (fa != fc) return fa < fc;
       return qro[b + a + 1] - qro[b + a] > qro[b + c + 1] - qro[b + c];
     });
-    for (uint32_t i = 0; i < n; ++i) {
-      m.forder[i] = uint8_t(ord[i]);
-      m.jorder[i] = uint8_t(i);  // identity matching order (isomorphism.hpp:371-447)
-    }
+    for (uint32_t t = 0; t < n; ++t) qpos[b + ord[t]] = b + t;
+    // matching order: identity, the reference's (isomorphism.hpp:371-447)
+    for (uint32_t d = 0; d < n; ++d) jorder[size_t(g) * 32 + d] = uint8_t(d);
     for (uint32_t d = 0; d < n; ++d) {
-      const uint32_t u = b + m.jorder[d];
+      const uint32_t u = b + jorder[size_t(g) * 32 + d];
       uint32_t w = 0;
       for (uint32_t i = 0; i < d; ++i) {
-        const uint32_t v = b + m.jorder[i];
+        const uint32_t v = b + jorder[size_t(g) * 32 + i];
         for (uint32_t k = qro[u]; k < qro[u + 1]; ++k)
           if (qci[k] == v) {
             w |= 1u << i;
@@ -581,16 +638,30 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
   std::vector<uint32_t> qpacks = build_packs(qgo);
   p->num_qpacks = uint32_t(qpacks.size() - 1);
   if ((rc = device_alloc((void**)&p->d_qpacks, 4ull * qpacks.size())) ||
-      (rc = device_alloc((void**)&p->d_qmeta, sizeof(QMeta) * std::max<size_t>(Q, 1))) ||
-      (rc = device_alloc((void**)&p->d_qnodes,
-                         sizeof(QNode) * std::max<uint64_t>(query->total_nodes, 1))) ||
-      (rc = device_alloc((void**)&p->d_qadj, 4ull * std::max<size_t>(qadj.size(), 1))))
+      (rc = device_alloc((void**)&p->d_qhead, 4ull * std::max<size_t>(qhead.size(), 4))) ||
+      (rc = device_alloc((void**)&p->d_qpre, 64ull * std::max<uint32_t>(Q, 1))) ||
+      (rc = device_alloc((void**)&p->d_qrec, 4ull * kRecDwords * std::max<uint64_t>(Qn, 1))) ||
+      (rc = device_alloc((void**)&p->d_qpos, 4ull * qpos.size())) ||
+      (rc = device_alloc((void**)&p->d_jorder, std::max<size_t>(jorder.size(), 32))) ||
+      (rc = device_alloc((void**)&p->d_qadj, 4ull * std::max<size_t>(qadj.size(), 32))))
     return fail(rc);
   BMS2_HIP(hipMemcpyAsync(p->d_qpacks, qpacks.data(), 4ull * qpacks.size(), hipMemcpyHostToDevice, s));
+  BMS2_HIP(hipMemcpyAsync(p->d_qpos, qpos.data(), 4ull * qpos.size(), hipMemcpyHostToDevice, s));
   if (Q) {
-    BMS2_HIP(hipMemcpyAsync(p->d_qmeta, meta.data(), sizeof(QMeta) * Q, hipMemcpyHostToDevice, s));
+    BMS2_HIP(hipMemcpyAsync(p->d_qhead, qhead.data(), 4ull * qhead.size(), hipMemcpyHostToDevice, s));
+    BMS2_HIP(hipMemcpyAsync(p->d_jorder, jorder.data(), jorder.size(), hipMemcpyHostToDevice, s));
     BMS2_HIP(hipMemcpyAsync(p->d_qadj, qadj.data(), 4ull * qadj.size(), hipMemcpyHostToDevice, s));
   }
+  // pair records: worst case or the budget, whichever is smaller
+  const uint64_t worst = uint64_t(data->num_graphs) * Q;
+  const uint64_t rec_bytes = 8ull * p->rec_words;
+  p->rec_cap = std::max<uint64_t>(1, std::min<uint64_t>(worst, record_budget_bytes() / rec_bytes));
+  if ((rc = device_alloc((void**)&p->d_recs, p->rec_cap * rec_bytes))) return fail(rc);
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  p->join_grid = cus * (p->nq_cap <= 16 ? 12 : 6);
   BMS2_HIP(hipStreamSynchronize(s));
   *out = p;
   return BMS2_OK;
@@ -612,43 +683,94 @@ extern "C" int bms2_pipeline_run(bms2_pipeline p, uint64_t* d_counts, uint64_t* 
   hipStream_t s = p->q->stream;
   EventScope es{p->q, ev};
   if (int rc = es.begin()) return rc;
+  EventScope ks{p->q, kernel_ev};
   const uint32_t Q = p->query.num_graphs;
-  if (d_stats) BMS2_HIP(hipMemsetAsync(d_stats, 0, 8 * 8, s));
+  uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (p->num_packs == 0 || Q == 0) {
-    BMS2_HIP(hipMemsetAsync(d_valid, 0, 8ull * p->data.num_graphs * ((Q + 63) / 64), s));
-    return es.end();
-  }
-  QueryPrepParams qp{p->query.graph_offsets, p->query.row_offsets, p->query.column_indices,
-                     p->query.node_labels, p->d_qpacks, p->num_qpacks, p->d_qnodes};
-  FusedParams fp{};
-  fp.graph_offsets = p->data.graph_offsets;
-  fp.row_offsets = p->data.row_offsets;
-  fp.column_indices = p->data.column_indices;
-  fp.labels = p->data.node_labels;
-  fp.pack_gbegin = p->d_packs;
-  fp.num_packs = p->num_packs;
-  fp.Q = Q;
-  fp.qmeta = p->d_qmeta;
-  fp.qnodes = p->d_qnodes;
-  fp.qadj = p->d_qadj;
-  fp.counts = d_counts;
-  fp.valid = d_valid;
-  fp.stats = d_stats;
-  fp.VW = (Q + 63) / 64;
-  fp.nq_cap = p->nq_cap;
-  fp.find_first = p->opt.find_first ? 1 : 0;
-  const size_t lds = fused_lds_bytes(p->nq_cap);
-  int rc = dispatch_levels(p->opt.refinement_steps, [&](auto kc) -> int {
-    constexpr int K = decltype(kc)::value;
-    if (p->num_qpacks)
+    if (int rc = ks.begin()) return rc;
+    if (int rc = ks.end()) return rc;
+  } else {
+    ScreenParams sp{};
+    sp.graph_offsets = p->data.graph_offsets;
+    sp.row_offsets = p->data.row_offsets;
+    sp.column_indices = p->data.column_indices;
+    sp.labels = p->data.node_labels;
+    sp.pack_gbegin = p->d_packs;
+    sp.Q = Q;
+    sp.qhead = p->d_qhead;
+    sp.qpre = p->d_qpre;
+    sp.qrec = p->d_qrec;
+    sp.jorder = p->d_jorder;
+    sp.adj_out = p->d_adj;
+    sp.counts = d_counts;
+    sp.valid = d_valid;
+    sp.VW = (Q + 63) / 64;
+    sp.rec_words = p->rec_words;
+    sp.recs = p->d_recs;
+    sp.rec_cap = p->rec_cap;
+    sp.ctl = p->d_ctl;
+    JoinParams jp{};
+    jp.recs = p->d_recs;
+    jp.rec_cap = p->rec_cap;
+    jp.rec_words = p->rec_words;
+    jp.Q = Q;
+    jp.adj = p->d_adj;
+    jp.qadj = p->d_qadj;
+    jp.counts = d_counts;
+    jp.ctl = p->d_ctl;
+    jp.find_first = p->opt.find_first ? 1 : 0;
+    QueryPrepParams qp{p->query.graph_offsets, p->query.row_offsets, p->query.column_indices,
+                       p->query.node_labels, p->d_qpacks, p->d_qpos, p->d_qrec};
+    const size_t stage_bytes =
+        std::max<size_t>(size_t(kStageRecs) * p->rec_words * 8, 64 * 8 * sizeof(uint32_t));
+    int rc = dispatch_levels(p->opt.refinement_steps, [&](auto kc) -> int {
+      constexpr int K = decltype(kc)::value;
       hipLaunchKernelGGL(query_prep_kernel<K>, dim3(p->num_qpacks), dim3(64), 0, s, qp);
-    BMS2_CHECK_LAUNCH("query_prep_kernel");
-    EventScope ks{p->q, kernel_ev};
-    if (int krc = ks.begin()) return krc;
-    hipLaunchKernelGGL(fused_kernel<K>, dim3(p->num_packs), dim3(64), lds, s, fp);
-    BMS2_CHECK_LAUNCH("fused_kernel");
-    return ks.end();
-  });
-  if (rc) return rc;
+      BMS2_CHECK_LAUNCH("query_prep_kernel");
+      hipLaunchKernelGGL(query_pre_kernel, dim3((Q + 255) / 256), dim3(256), 0, s, p->d_qhead,
+                         p->d_qrec, Q, p->d_qpre);
+      BMS2_CHECK_LAUNCH("query_pre_kernel");
+      if (int krc = ks.begin()) return krc;
+      uint32_t p0 = 0;
+      while (p0 < p->num_packs) {
+        // chunk of packs sized so the expected GMCR pairs fit the record buffer
+        uint64_t chunk = p->num_packs - p0;
+        if (p->pairs_per_pack > 0)
+          chunk = std::min<uint64_t>(chunk, std::max<uint64_t>(1, uint64_t(0.9 * p->rec_cap / p->pairs_per_pack)));
+        const uint32_t p1 = p0 + uint32_t(chunk);
+        hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_ctl, 8u);
+        sp.pack_begin = p0;
+        hipLaunchKernelGGL(screen_kernel<K>, dim3(p1 - p0), dim3(64), stage_bytes, s, sp);
+        BMS2_CHECK_LAUNCH("screen_kernel");
+        if (p->nq_cap <= 16)
+          hipLaunchKernelGGL(join_kernel<16>, dim3(p->join_grid), dim3(64), 0, s, jp);
+        else
+          hipLaunchKernelGGL(join_kernel<32>, dim3(p->join_grid), dim3(64), 0, s, jp);
+        BMS2_CHECK_LAUNCH("join_kernel");
+        uint64_t ctl[8];
+        BMS2_HIP(hipMemcpyAsync(ctl, p->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, s));
+        BMS2_HIP(hipStreamSynchronize(s));
+        const double density = double(ctl[0]) / double(p1 - p0);
+        if (ctl[1]) {
+          // record buffer overflowed: measure, shrink the chunk, redo it (results are rewritten)
+          if (chunk == 1)
+            return set_error(BMS2_ERR_CAPACITY, "pair record buffer (%llu records) too small",
+                             (unsigned long long)p->rec_cap);
+          p->pairs_per_pack = std::max(p->pairs_per_pack * 2.0, density);
+          continue;
+        }
+        p->pairs_per_pack = std::max(p->pairs_per_pack, density);
+        stats[0] += ctl[3];
+        stats[1] += ctl[4];
+        stats[2] += ctl[5];
+        stats[3] += ctl[6];
+        p0 = p1;
+      }
+      return ks.end();
+    });
+    if (rc) return rc;
+  }
+  if (d_stats) BMS2_HIP(hipMemcpyAsync(d_stats, stats, sizeof(stats), hipMemcpyHostToDevice, s));
+  if (d_stats) BMS2_HIP(hipStreamSynchronize(s));
   return es.end();
 }
