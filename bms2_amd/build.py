@@ -65,6 +65,15 @@ def build_library(jobs: int | None = None) -> Path:
     return LIB
 
 
+def build_driver() -> Path:
+    """bms2_main: the reference's BMS2 driver written against include/sigmo.hpp."""
+    exe = PKG / "bms2_main"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", str(ROOT / "include"),
+          str(PKG / "examples" / "bms2_main.cpp"), "-o", str(exe), "-L", str(PKG), "-lbms2",
+          "-Wl,-rpath,$ORIGIN"])
+    return exe
+
+
 def build_oracle() -> Path:
     _run(["make", "-s", "-C", str(ORACLE), "-B", "liboracle.so"])
     return ORACLE / "liboracle.so"
@@ -72,6 +81,7 @@ def build_oracle() -> Path:
 
 def build_all() -> None:
     build_library()
+    build_driver()
     build_oracle()
 
 

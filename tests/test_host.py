@@ -1,0 +1,147 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports every declared entry
+point, the workload generator and the .dat reader/writer behave, errors are reported."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from bms2_amd import _lib, graphs
+from bms2_amd.graphs import GraphBatch
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    text = (ROOT / "include" / "bms2.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bms2_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    names = declared_functions()
+    assert len(names) > 40
+    handle = C.CDLL(str(_lib.LIB_PATH))
+    missing = [n for n in names if not hasattr(handle, n)]
+    assert not missing, missing
+    assert set(names) == set(_lib.PROTOTYPES), set(names) ^ set(_lib.PROTOTYPES)
+
+
+def test_binding_loads_all_prototypes():
+    lib = _lib.lib()
+    assert lib.bms2_version() >= 100
+
+
+def test_generator_is_deterministic_and_prefix_stable():
+    d1, q1 = graphs.generate("molecules-2k", seed=5)
+    d2, q2 = graphs.generate("molecules-2k", seed=5)
+    assert d1.digest() == d2.digest() and q1.digest() == q2.digest()
+    d3, _ = graphs.generate("molecules-2k", seed=6)
+    assert d3.digest() != d1.digest()
+    # data graph i depends only on (seed, i)
+    big, _ = graphs.generate("molecules-2k", seed=5, num_data_graphs=2500)
+    assert big.subset(0, 2000).digest() == d1.digest()
+
+
+def test_generator_shapes(molecules_2k):
+    data, queries = molecules_2k
+    sizes = data.graph_sizes()
+    assert sizes.min() >= 10 and sizes.max() <= 40
+    assert data.degrees().max() <= 4
+    qs = queries.graph_sizes()
+    assert qs.min() >= 1 and qs.max() <= 10
+    allowed = {0, 1, 3, 7, 8, 9, 10}  # N Cl Br O C S F
+    assert set(np.unique(data.node_labels)) <= allowed
+    frac_c = float(np.mean(data.node_labels == 8))
+    assert 0.68 < frac_c < 0.76
+    assert set(np.unique(data.edge_labels)) <= {1, 2, 4}
+    # every graph connected (BFS over the CSR)
+    for b in (data, queries):
+        for g in range(0, b.num_graphs, 97):
+            labels, edges = b.graph(g)
+            n = len(labels)
+            adj = [[] for _ in range(n)]
+            for u, v in edges:
+                adj[u].append(v)
+                adj[v].append(u)
+            seen, stack = {0}, [0]
+            while stack:
+                for w in adj[stack.pop()]:
+                    if w not in seen:
+                        seen.add(w)
+                        stack.append(w)
+            assert len(seen) == n
+    # rings: some graphs carry 5- or 6-ring closures (more edges than a tree)
+    extra = [len(data.graph(g)[1]) - (data.graph_sizes()[g] - 1) for g in range(200)]
+    assert max(extra) >= 1 and min(extra) >= 0
+
+
+def test_uneven_workload_shapes():
+    data, queries = graphs.generate("uneven-1m", seed=0, num_data_graphs=3000)
+    sizes = data.graph_sizes()
+    assert sizes.min() >= 5 and sizes.max() <= 250
+    assert 20 < sizes.mean() < 40
+    assert (sizes > 64).mean() > 0.005
+    assert queries.graph_sizes().max() <= 32
+
+
+def neighbor_sets(b: GraphBatch):
+    out = []
+    for u in range(b.total_nodes):
+        out.append(sorted(b.column_indices[b.row_offsets[u]:b.row_offsets[u + 1]].tolist()))
+    return out
+
+
+def test_dat_round_trip(tmp_path, molecules_2k):
+    data, _ = molecules_2k
+    sub = data.subset(0, 50)
+    p = tmp_path / "g.dat"
+    graphs.write_dat(str(p), sub)
+    back = graphs.load_dat(str(p))
+    assert np.array_equal(back.graph_offsets, sub.graph_offsets)
+    assert np.array_equal(back.node_labels, sub.node_labels)
+    assert neighbor_sets(back) == neighbor_sets(sub)
+
+
+def test_dat_formats_and_errors(tmp_path):
+    p = tmp_path / "a.dat"
+    # labelled and unlabelled edges (graph.hpp:184-198)
+    p.write_text("n#3 l#11 0 8 1 7 2 8 e#2 0 1 2 1 2 4\n"
+                 "n#2 l#11 0 8 1 8 e#1 0 1\n")
+    g = graphs.load_dat(str(p))
+    assert g.num_graphs == 2
+    assert g.node_labels.tolist() == [8, 7, 8, 8, 8]
+    assert g.edge_labels.tolist() == [2, 2, 4, 4, 0, 0]
+    assert g.column_indices.tolist() == [1, 0, 2, 1, 4, 3]
+    bad = tmp_path / "bad.dat"
+    bad.write_text("n#2 l#11 0 8 1 8 e#1 0 1\n"
+                   "n#2 l#11 0 8 1 8 e#2 0 1\n"        # 2 tokens for 2 edges: neither 4 nor 6
+                   "n#2 l#11 0 8 1 8 e#1 0 5\n")       # endpoint out of range
+    with pytest.raises(_lib.Bms2Error) as e:
+        graphs.load_dat(str(bad))
+    assert e.value.code == 5 and ":2:" in str(e.value)
+    g2 = graphs.load_dat(str(bad), skip_malformed=True)  # BMS2_batch.cpp:121-124
+    assert g2.num_graphs == 1
+    with pytest.raises(_lib.Bms2Error) as e:
+        graphs.load_dat(str(tmp_path / "missing.dat"))
+    assert e.value.code == 4
+
+
+def test_generator_rejects_bad_spec():
+    with pytest.raises(_lib.Bms2Error) as e:
+        graphs.generate({"num_data_graphs": 10, "num_query_graphs": 4, "data_min_nodes": 5,
+                         "data_max_nodes": 3, "query_min_nodes": 3, "query_max_nodes": 4})
+    assert e.value.code == 1
+    with pytest.raises(_lib.Bms2Error):
+        graphs.generate({"num_data_graphs": 10, "num_query_graphs": 4, "data_min_nodes": 5,
+                         "data_max_nodes": 10, "query_min_nodes": 3, "query_max_nodes": 40})
+
+
+def test_from_graphs_matches_reference_packing():
+    # each edge appended under both endpoints in list order (graph.hpp:229-240)
+    g = GraphBatch.from_graphs([([8, 8, 8], [(0, 1, 1), (0, 2, 2)]), ([7, 8], [(1, 0)])])
+    assert g.graph_offsets.tolist() == [0, 3, 5]
+    assert g.row_offsets.tolist() == [0, 2, 3, 4, 5, 6]
+    assert g.column_indices.tolist() == [1, 2, 0, 0, 4, 3]
+    assert g.edge_labels.tolist() == [1, 2, 1, 2, 0, 0]
