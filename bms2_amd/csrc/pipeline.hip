@@ -340,6 +340,9 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
 
 // ------------------------------------------------------------------ join
 
+// Depth-first join, one (query, graph) pair per lane (isomorphism.hpp:291-472 semantics with
+// standard backtracking); LDS stacks per depth: remaining candidates R, matched node F and its
+// neighbour row NF.
 template <int NQ>
 __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
   __shared__ uint64_t R[NQ][64];   // remaining candidates per depth
