@@ -84,6 +84,10 @@ class PipelineOptions(C.Structure):
     ]
 
 
+class Adam(C.Structure):
+    _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float)]
+
+
 vp = C.c_void_p
 vpp = C.POINTER(C.c_void_p)
 i32 = C.c_int
@@ -146,6 +150,17 @@ PROTOTYPES: dict[str, tuple] = {
     "bms2_pipeline_create": (i32, [vp, DCP, DCP, C.POINTER(PipelineOptions), vpp]),
     "bms2_pipeline_destroy": (i32, [vp]),
     "bms2_pipeline_run": (i32, [vp, vp, vp, vp, vpp, vpp]),
+    "bms2_pipeline_set_order": (i32, [vp, vp]),
+    "bms2_mlp_num_params": (u64, []),
+    "bms2_mlp_create": (i32, [vp, u64, vpp]),
+    "bms2_mlp_destroy": (i32, [vp]),
+    "bms2_mlp_get_params": (i32, [vp, vp, u64]),
+    "bms2_mlp_set_params": (i32, [vp, vp, u64]),
+    "bms2_mlp_forward": (i32, [vp, vp, u32, vp, i32]),
+    "bms2_mlp_loss_grad": (i32, [vp, vp, vp, u32, C.c_float, C.POINTER(C.c_float), vp]),
+    "bms2_mlp_train_step": (i32, [vp, vp, vp, u32, C.c_float, C.POINTER(Adam),
+                                  C.POINTER(C.c_float)]),
+    "bms2_pipeline_trace": (i32, [vp, u32, vp, vp, vp, vp, vp, vp, C.POINTER(u32)]),
 }
 
 

@@ -368,6 +368,8 @@ class Pipeline(_Handle):
         self.q = q
         self.Q = query.num_graphs
         self.D = data.num_graphs
+        # the pipeline reads the batches' device memory on every run: keep them alive with it
+        self.query_graphs, self.data_graphs = query, data
         o = _lib.PipelineOptions()
         o.refinement_steps = refinement_steps
         o.find_first = int(find_first)
@@ -403,6 +405,43 @@ class Pipeline(_Handle):
         c = counts.to_numpy(np.uint64, rows * self.Q).reshape(rows, self.Q)
         v = valid.to_numpy(np.uint64, rows * self.VW).reshape(rows, self.VW)
         return c, v
+
+    def set_order(self, order: np.ndarray | None) -> None:
+        """Join matching order: order[q, d] = local query node matched at depth d (u8 [Q, 32]);
+        None restores identity."""
+        if order is None:
+            check(_L().bms2_pipeline_set_order(self.h, None))
+            return
+        o = np.ascontiguousarray(order, np.uint8)
+        if o.shape != (self.Q, 32):
+            raise ValueError(f"order must have shape ({self.Q}, 32), got {o.shape}")
+        check(_L().bms2_pipeline_set_order(self.h, o.ctypes.data_as(C.c_void_p)))
+
+    def trace(self, max_graphs: int, query_nodes: int, pair_nodes: bool = False) -> dict:
+        """Join traces over the leading data graphs (at least max_graphs of them): summed
+        domain size per query node, GMCR pairs and DFS nodes per query, optionally DFS nodes per
+        (graph, query) pair, plus the counts of the traced graphs."""
+        G = min(int(max_graphs) + 64, self.D)  # whole packs: at most 8 graphs past max_graphs
+        counts = DeviceBuffer(self.q, 8 * max(G * self.Q, 1), zero=False)
+        valid = DeviceBuffer(self.q, 8 * max(G * self.VW, 1), zero=False)
+        domsum = DeviceBuffer(self.q, 8 * max(query_nodes, 1), zero=True)
+        qpairs = DeviceBuffer(self.q, 8 * max(self.Q, 1), zero=True)
+        qnodes = DeviceBuffer(self.q, 8 * max(self.Q, 1), zero=True)
+        pn = DeviceBuffer(self.q, 4 * max(G * self.Q, 1), zero=True) if pair_nodes else None
+        got = C.c_uint32(0)
+        check(_L().bms2_pipeline_trace(self.h, int(max_graphs), C.c_void_p(counts.ptr),
+                                       C.c_void_p(valid.ptr), C.c_void_p(domsum.ptr),
+                                       C.c_void_p(qpairs.ptr), C.c_void_p(qnodes.ptr),
+                                       C.c_void_p(pn.ptr) if pn else None, C.byref(got)))
+        g = int(got.value)
+        out = dict(graphs=g,
+                   domsum=domsum.to_numpy(np.uint64, query_nodes),
+                   qpairs=qpairs.to_numpy(np.uint64, self.Q),
+                   qnodes=qnodes.to_numpy(np.uint64, self.Q),
+                   counts=counts.to_numpy(np.uint64, g * self.Q).reshape(g, self.Q))
+        if pn is not None:
+            out["pair_nodes"] = pn.to_numpy(np.uint32, g * self.Q).reshape(g, self.Q)
+        return out
 
 
 def run_staged(q: Queue, query: GraphBatch, data: GraphBatch, K: int, find_first: bool = False,

@@ -54,6 +54,8 @@ struct ScreenParams {
   uint64_t* recs;
   uint64_t rec_cap;
   uint64_t* ctl;            // [0] records reserved, [1] overflow, [2] join cursor, [3..6] stats
+  uint64_t* domsum;         // trace (nullable): [Qn] summed domain sizes over GMCR pairs
+  uint64_t* qpairs;         // trace (nullable): [Q] GMCR pairs
 };
 
 struct JoinParams {
@@ -66,6 +68,8 @@ struct JoinParams {
   uint64_t* counts;
   uint64_t* ctl;
   uint32_t find_first;
+  uint64_t* qnodes;         // trace (nullable): [Q] DFS nodes
+  uint32_t* pair_nodes;     // trace (nullable): [D][Q] DFS nodes per pair, saturating
 };
 
 struct QueryPrepParams {
@@ -319,6 +323,12 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
           rec[1] = uint64_t(n0) | (uint64_t(nq) << 32);
         }
         if (lane < nq) rec[2 + lane] = dom[p.jorder[size_t(qu) * 32 + lane]] & seg;
+        if (p.domsum) {
+          if (lane < nq)
+            atomicAdd(reinterpret_cast<unsigned long long*>(p.domsum + rb + lane),
+                      (unsigned long long)popc64(dom[lane] & seg));
+          if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(p.qpairs + qu), 1ull);
+        }
         ++nstage;
         ++pairs;
       }
@@ -349,6 +359,9 @@ __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
   __shared__ uint64_t NF[NQ][64];  // neighbour row of the node matched per depth
   __shared__ uint8_t F[NQ][64];    // node matched per depth
   const uint32_t lane = lane_id();
+  // after an overflow the screen skipped whole flushes, so records below ctl[0] may be stale;
+  // the host redoes the chunk in smaller pieces
+  if (p.ctl[1]) return;
   const uint64_t total = min(p.ctl[0], p.rec_cap);
   const uint32_t RW = p.rec_words;
   bool busy = false, exhausted = false;
@@ -433,6 +446,8 @@ __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
     if (finished) {
       const uint64_t c = p.find_first ? (cnt ? 1 : 0) : cnt;
       p.counts[uint64_t(g) * p.Q + q] = c;
+      if (p.qnodes) atomicAdd(reinterpret_cast<unsigned long long*>(p.qnodes + q), (unsigned long long)nodes);
+      if (p.pair_nodes) p.pair_nodes[uint64_t(g) * p.Q + q] = uint32_t(min(nodes, uint64_t(0xffffffffu)));
       st_matched += c ? 1 : 0;
       st_emb += c;
       st_nodes += nodes;
@@ -516,6 +531,8 @@ struct bms2_pipeline_s {
   uint64_t* d_adj = nullptr;
   uint64_t* d_recs = nullptr;
   uint64_t* d_ctl = nullptr;
+  // host copy of the query batch (matching orders are derived from it)
+  std::vector<uint32_t> qgo, qro, qci;
 };
 
 namespace {
@@ -541,6 +558,50 @@ uint64_t record_budget_bytes() {
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 1ull << 30;
   return std::min<uint64_t>(free_b / 4, 16ull << 30);
+}
+
+// Matching order per query (identity when order is null, the reference's, isomorphism.hpp:
+// 371-447) and, per depth d, the bit mask of earlier depths adjacent to it in the query.
+int upload_order(bms2_pipeline_s* p, const uint8_t* order) {
+  const uint32_t Q = p->query.num_graphs;
+  if (!Q) return BMS2_OK;
+  std::vector<uint8_t> jorder(size_t(Q) * 32, 0);
+  std::vector<uint32_t> qadj(size_t(Q) * 32, 0);
+  for (uint32_t g = 0; g < Q; ++g) {
+    const uint32_t b = p->qgo[g], n = p->qgo[g + 1] - b;
+    uint8_t* o = jorder.data() + size_t(g) * 32;
+    if (order) {
+      uint32_t seen = 0;
+      for (uint32_t d = 0; d < n; ++d) {
+        const uint32_t u = order[size_t(g) * 32 + d];
+        if (u >= n || ((seen >> u) & 1))
+          return set_error(BMS2_ERR_INVALID, "order of query %u is not a permutation of 0..%u", g,
+                           n - 1);
+        seen |= 1u << u;
+        o[d] = uint8_t(u);
+      }
+    } else {
+      for (uint32_t d = 0; d < n; ++d) o[d] = uint8_t(d);
+    }
+    for (uint32_t d = 0; d < n; ++d) {
+      const uint32_t u = b + o[d];
+      uint32_t w = 0;
+      for (uint32_t i = 0; i < d; ++i) {
+        const uint32_t v = b + o[i];
+        for (uint32_t k = p->qro[u]; k < p->qro[u + 1]; ++k)
+          if (p->qci[k] == v) {
+            w |= 1u << i;
+            break;
+          }
+      }
+      qadj[size_t(g) * 32 + d] = w;
+    }
+  }
+  hipStream_t s = p->q->stream;
+  BMS2_HIP(hipMemcpyAsync(p->d_jorder, jorder.data(), jorder.size(), hipMemcpyHostToDevice, s));
+  BMS2_HIP(hipMemcpyAsync(p->d_qadj, qadj.data(), 4ull * qadj.size(), hipMemcpyHostToDevice, s));
+  BMS2_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
+  return BMS2_OK;
 }
 
 }  // namespace
@@ -605,8 +666,6 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
   std::vector<uint32_t> label_freq(256, 0);
   for (uint8_t l : qlab) label_freq[l]++;
   std::vector<uint32_t> qhead(size_t(Q) * 4, 0), qpos(std::max<uint64_t>(Qn, 1), 0);
-  std::vector<uint8_t> jorder(size_t(Q) * 32, 0);
-  std::vector<uint32_t> qadj(size_t(Q) * 32, 0);
   for (uint32_t g = 0; g < Q; ++g) {
     const uint32_t b = qgo[g], n = qgo[g + 1] - qgo[g];
     qhead[4 * g + 0] = b;
@@ -622,21 +681,6 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
       return qro[b + a + 1] - qro[b + a] > qro[b + c + 1] - qro[b + c];
     });
     for (uint32_t t = 0; t < n; ++t) qpos[b + ord[t]] = b + t;
-    // matching order: identity, the reference's (isomorphism.hpp:371-447)
-    for (uint32_t d = 0; d < n; ++d) jorder[size_t(g) * 32 + d] = uint8_t(d);
-    for (uint32_t d = 0; d < n; ++d) {
-      const uint32_t u = b + jorder[size_t(g) * 32 + d];
-      uint32_t w = 0;
-      for (uint32_t i = 0; i < d; ++i) {
-        const uint32_t v = b + jorder[size_t(g) * 32 + i];
-        for (uint32_t k = qro[u]; k < qro[u + 1]; ++k)
-          if (qci[k] == v) {
-            w |= 1u << i;
-            break;
-          }
-      }
-      qadj[size_t(g) * 32 + d] = w;
-    }
   }
   std::vector<uint32_t> qpacks = build_packs(qgo);
   p->num_qpacks = uint32_t(qpacks.size() - 1);
@@ -645,16 +689,18 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
       (rc = device_alloc((void**)&p->d_qpre, 64ull * std::max<uint32_t>(Q, 1))) ||
       (rc = device_alloc((void**)&p->d_qrec, 4ull * kRecDwords * std::max<uint64_t>(Qn, 1))) ||
       (rc = device_alloc((void**)&p->d_qpos, 4ull * qpos.size())) ||
-      (rc = device_alloc((void**)&p->d_jorder, std::max<size_t>(jorder.size(), 32))) ||
-      (rc = device_alloc((void**)&p->d_qadj, 4ull * std::max<size_t>(qadj.size(), 32))))
+      (rc = device_alloc((void**)&p->d_jorder, std::max<size_t>(size_t(Q) * 32, 32))) ||
+      (rc = device_alloc((void**)&p->d_qadj, 4ull * std::max<size_t>(size_t(Q) * 32, 32))))
     return fail(rc);
   BMS2_HIP(hipMemcpyAsync(p->d_qpacks, qpacks.data(), 4ull * qpacks.size(), hipMemcpyHostToDevice, s));
   BMS2_HIP(hipMemcpyAsync(p->d_qpos, qpos.data(), 4ull * qpos.size(), hipMemcpyHostToDevice, s));
   if (Q) {
     BMS2_HIP(hipMemcpyAsync(p->d_qhead, qhead.data(), 4ull * qhead.size(), hipMemcpyHostToDevice, s));
-    BMS2_HIP(hipMemcpyAsync(p->d_jorder, jorder.data(), jorder.size(), hipMemcpyHostToDevice, s));
-    BMS2_HIP(hipMemcpyAsync(p->d_qadj, qadj.data(), 4ull * qadj.size(), hipMemcpyHostToDevice, s));
   }
+  p->qgo = std::move(qgo);
+  p->qro = std::move(qro);
+  p->qci = std::move(qci);
+  if ((rc = upload_order(p, nullptr))) return fail(rc);
   // pair records: worst case or the budget, whichever is smaller
   const uint64_t worst = uint64_t(data->num_graphs) * Q;
   const uint64_t rec_bytes = 8ull * p->rec_words;
@@ -678,6 +724,114 @@ extern "C" int bms2_pipeline_destroy(bms2_pipeline p) {
   return BMS2_OK;
 }
 
+namespace {
+
+struct Trace {
+  uint64_t* domsum = nullptr;
+  uint64_t* qpairs = nullptr;
+  uint64_t* qnodes = nullptr;
+  uint32_t* pair_nodes = nullptr;
+};
+
+// Screens and joins packs [0, pack_end) in chunks that fit the record buffer; stats (host)
+// accumulates {GMCR pairs, matched pairs, embeddings, DFS nodes}.
+int run_packs(bms2_pipeline p, uint32_t pack_end, uint64_t* d_counts, uint64_t* d_valid,
+              const Trace& tr, EventScope& ks, uint64_t stats[8]) {
+  hipStream_t s = p->q->stream;
+  const uint32_t Q = p->query.num_graphs;
+  if (pack_end == 0 || Q == 0) {
+    if (int rc = ks.begin()) return rc;
+    return ks.end();
+  }
+  ScreenParams sp{};
+  sp.graph_offsets = p->data.graph_offsets;
+  sp.row_offsets = p->data.row_offsets;
+  sp.column_indices = p->data.column_indices;
+  sp.labels = p->data.node_labels;
+  sp.pack_gbegin = p->d_packs;
+  sp.Q = Q;
+  sp.qhead = p->d_qhead;
+  sp.qpre = p->d_qpre;
+  sp.qrec = p->d_qrec;
+  sp.jorder = p->d_jorder;
+  sp.adj_out = p->d_adj;
+  sp.counts = d_counts;
+  sp.valid = d_valid;
+  sp.VW = (Q + 63) / 64;
+  sp.rec_words = p->rec_words;
+  sp.recs = p->d_recs;
+  sp.rec_cap = p->rec_cap;
+  sp.ctl = p->d_ctl;
+  sp.domsum = tr.domsum;
+  sp.qpairs = tr.qpairs;
+  JoinParams jp{};
+  jp.recs = p->d_recs;
+  jp.rec_cap = p->rec_cap;
+  jp.rec_words = p->rec_words;
+  jp.Q = Q;
+  jp.adj = p->d_adj;
+  jp.qadj = p->d_qadj;
+  jp.counts = d_counts;
+  jp.ctl = p->d_ctl;
+  jp.find_first = p->opt.find_first ? 1 : 0;
+  jp.qnodes = tr.qnodes;
+  jp.pair_nodes = tr.pair_nodes;
+  QueryPrepParams qp{p->query.graph_offsets, p->query.row_offsets, p->query.column_indices,
+                     p->query.node_labels, p->d_qpacks, p->d_qpos, p->d_qrec};
+  const size_t stage_bytes =
+      std::max<size_t>(size_t(kStageRecs) * p->rec_words * 8, 64 * 8 * sizeof(uint32_t));
+  return dispatch_levels(p->opt.refinement_steps, [&](auto kc) -> int {
+    constexpr int K = decltype(kc)::value;
+    hipLaunchKernelGGL(query_prep_kernel<K>, dim3(p->num_qpacks), dim3(64), 0, s, qp);
+    BMS2_CHECK_LAUNCH("query_prep_kernel");
+    hipLaunchKernelGGL(query_pre_kernel, dim3((Q + 255) / 256), dim3(256), 0, s, p->d_qhead,
+                       p->d_qrec, Q, p->d_qpre);
+    BMS2_CHECK_LAUNCH("query_pre_kernel");
+    if (int krc = ks.begin()) return krc;
+    uint32_t p0 = 0;
+    while (p0 < pack_end) {
+      // chunk of packs sized so the expected GMCR pairs fit the record buffer
+      uint64_t chunk = pack_end - p0;
+      if (p->pairs_per_pack > 0)
+        chunk = std::min<uint64_t>(chunk, std::max<uint64_t>(1, uint64_t(0.9 * p->rec_cap / p->pairs_per_pack)));
+      const uint32_t p1 = p0 + uint32_t(chunk);
+      hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_ctl, 8u);
+      sp.pack_begin = p0;
+      hipLaunchKernelGGL(screen_kernel<K>, dim3(p1 - p0), dim3(64), stage_bytes, s, sp);
+      BMS2_CHECK_LAUNCH("screen_kernel");
+      if (p->nq_cap <= 16)
+        hipLaunchKernelGGL(join_kernel<16>, dim3(p->join_grid), dim3(64), 0, s, jp);
+      else
+        hipLaunchKernelGGL(join_kernel<32>, dim3(p->join_grid), dim3(64), 0, s, jp);
+      BMS2_CHECK_LAUNCH("join_kernel");
+      uint64_t ctl[8];
+      BMS2_HIP(hipMemcpyAsync(ctl, p->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, s));
+      BMS2_HIP(hipStreamSynchronize(s));
+      const double density = double(ctl[0]) / double(p1 - p0);
+      if (ctl[1]) {
+        // record buffer overflowed: measure, shrink the chunk, redo it (results are rewritten)
+        if (chunk == 1)
+          return set_error(BMS2_ERR_CAPACITY, "pair record buffer (%llu records) too small",
+                           (unsigned long long)p->rec_cap);
+        if (tr.domsum || tr.qnodes)
+          return set_error(BMS2_ERR_CAPACITY,
+                           "pair record buffer overflowed during a trace; trace fewer graphs");
+        p->pairs_per_pack = std::max(p->pairs_per_pack * 2.0, density);
+        continue;
+      }
+      p->pairs_per_pack = std::max(p->pairs_per_pack, density);
+      stats[0] += ctl[3];
+      stats[1] += ctl[4];
+      stats[2] += ctl[5];
+      stats[3] += ctl[6];
+      p0 = p1;
+    }
+    return ks.end();
+  });
+}
+
+}  // namespace
+
 extern "C" int bms2_pipeline_run(bms2_pipeline p, uint64_t* d_counts, uint64_t* d_valid,
                                  uint64_t* d_stats, bms2_event* ev, bms2_event* kernel_ev) {
   if (!p) return set_error(BMS2_ERR_INVALID, "pipeline is null");
@@ -687,93 +841,38 @@ extern "C" int bms2_pipeline_run(bms2_pipeline p, uint64_t* d_counts, uint64_t* 
   EventScope es{p->q, ev};
   if (int rc = es.begin()) return rc;
   EventScope ks{p->q, kernel_ev};
-  const uint32_t Q = p->query.num_graphs;
   uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (p->num_packs == 0 || Q == 0) {
-    if (int rc = ks.begin()) return rc;
-    if (int rc = ks.end()) return rc;
-  } else {
-    ScreenParams sp{};
-    sp.graph_offsets = p->data.graph_offsets;
-    sp.row_offsets = p->data.row_offsets;
-    sp.column_indices = p->data.column_indices;
-    sp.labels = p->data.node_labels;
-    sp.pack_gbegin = p->d_packs;
-    sp.Q = Q;
-    sp.qhead = p->d_qhead;
-    sp.qpre = p->d_qpre;
-    sp.qrec = p->d_qrec;
-    sp.jorder = p->d_jorder;
-    sp.adj_out = p->d_adj;
-    sp.counts = d_counts;
-    sp.valid = d_valid;
-    sp.VW = (Q + 63) / 64;
-    sp.rec_words = p->rec_words;
-    sp.recs = p->d_recs;
-    sp.rec_cap = p->rec_cap;
-    sp.ctl = p->d_ctl;
-    JoinParams jp{};
-    jp.recs = p->d_recs;
-    jp.rec_cap = p->rec_cap;
-    jp.rec_words = p->rec_words;
-    jp.Q = Q;
-    jp.adj = p->d_adj;
-    jp.qadj = p->d_qadj;
-    jp.counts = d_counts;
-    jp.ctl = p->d_ctl;
-    jp.find_first = p->opt.find_first ? 1 : 0;
-    QueryPrepParams qp{p->query.graph_offsets, p->query.row_offsets, p->query.column_indices,
-                       p->query.node_labels, p->d_qpacks, p->d_qpos, p->d_qrec};
-    const size_t stage_bytes =
-        std::max<size_t>(size_t(kStageRecs) * p->rec_words * 8, 64 * 8 * sizeof(uint32_t));
-    int rc = dispatch_levels(p->opt.refinement_steps, [&](auto kc) -> int {
-      constexpr int K = decltype(kc)::value;
-      hipLaunchKernelGGL(query_prep_kernel<K>, dim3(p->num_qpacks), dim3(64), 0, s, qp);
-      BMS2_CHECK_LAUNCH("query_prep_kernel");
-      hipLaunchKernelGGL(query_pre_kernel, dim3((Q + 255) / 256), dim3(256), 0, s, p->d_qhead,
-                         p->d_qrec, Q, p->d_qpre);
-      BMS2_CHECK_LAUNCH("query_pre_kernel");
-      if (int krc = ks.begin()) return krc;
-      uint32_t p0 = 0;
-      while (p0 < p->num_packs) {
-        // chunk of packs sized so the expected GMCR pairs fit the record buffer
-        uint64_t chunk = p->num_packs - p0;
-        if (p->pairs_per_pack > 0)
-          chunk = std::min<uint64_t>(chunk, std::max<uint64_t>(1, uint64_t(0.9 * p->rec_cap / p->pairs_per_pack)));
-        const uint32_t p1 = p0 + uint32_t(chunk);
-        hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_ctl, 8u);
-        sp.pack_begin = p0;
-        hipLaunchKernelGGL(screen_kernel<K>, dim3(p1 - p0), dim3(64), stage_bytes, s, sp);
-        BMS2_CHECK_LAUNCH("screen_kernel");
-        if (p->nq_cap <= 16)
-          hipLaunchKernelGGL(join_kernel<16>, dim3(p->join_grid), dim3(64), 0, s, jp);
-        else
-          hipLaunchKernelGGL(join_kernel<32>, dim3(p->join_grid), dim3(64), 0, s, jp);
-        BMS2_CHECK_LAUNCH("join_kernel");
-        uint64_t ctl[8];
-        BMS2_HIP(hipMemcpyAsync(ctl, p->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, s));
-        BMS2_HIP(hipStreamSynchronize(s));
-        const double density = double(ctl[0]) / double(p1 - p0);
-        if (ctl[1]) {
-          // record buffer overflowed: measure, shrink the chunk, redo it (results are rewritten)
-          if (chunk == 1)
-            return set_error(BMS2_ERR_CAPACITY, "pair record buffer (%llu records) too small",
-                             (unsigned long long)p->rec_cap);
-          p->pairs_per_pack = std::max(p->pairs_per_pack * 2.0, density);
-          continue;
-        }
-        p->pairs_per_pack = std::max(p->pairs_per_pack, density);
-        stats[0] += ctl[3];
-        stats[1] += ctl[4];
-        stats[2] += ctl[5];
-        stats[3] += ctl[6];
-        p0 = p1;
-      }
-      return ks.end();
-    });
-    if (rc) return rc;
-  }
+  if (int rc = run_packs(p, p->num_packs, d_counts, d_valid, Trace{}, ks, stats)) return rc;
   if (d_stats) BMS2_HIP(hipMemcpyAsync(d_stats, stats, sizeof(stats), hipMemcpyHostToDevice, s));
   if (d_stats) BMS2_HIP(hipStreamSynchronize(s));
   return es.end();
+}
+
+extern "C" int bms2_pipeline_set_order(bms2_pipeline p, const uint8_t* order) {
+  if (!p) return set_error(BMS2_ERR_INVALID, "pipeline is null");
+  if (int rc = use_device(p->q)) return rc;
+  BMS2_HIP(hipStreamSynchronize(p->q->stream));  // a running join still reads the old order
+  return upload_order(p, order);
+}
+
+extern "C" int bms2_pipeline_trace(bms2_pipeline p, uint32_t max_graphs, uint64_t* d_counts,
+                                   uint64_t* d_valid, uint64_t* d_domsum, uint64_t* d_qpairs,
+                                   uint64_t* d_qnodes, uint32_t* d_pair_nodes,
+                                   uint32_t* graphs_out) {
+  if (!p) return set_error(BMS2_ERR_INVALID, "pipeline is null");
+  if (int rc = use_device(p->q)) return rc;
+  if (!d_counts || !d_valid || !d_domsum || !d_qpairs || !d_qnodes)
+    return set_error(BMS2_ERR_INVALID, "null trace buffer");
+  // leading packs covering max_graphs graphs
+  std::vector<uint32_t> packs(p->num_packs + 1);
+  BMS2_HIP(hipMemcpy(packs.data(), p->d_packs, 4ull * packs.size(), hipMemcpyDeviceToHost));
+  uint32_t pe = 0;
+  while (pe < p->num_packs && packs[pe] < max_graphs) ++pe;
+  if (graphs_out) *graphs_out = packs[pe];
+  Trace tr{d_domsum, d_qpairs, d_qnodes, d_pair_nodes};
+  EventScope ks{p->q, nullptr};
+  uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (int rc = run_packs(p, pe, d_counts, d_valid, tr, ks, stats)) return rc;
+  BMS2_HIP(hipStreamSynchronize(p->q->stream));
+  return BMS2_OK;
 }

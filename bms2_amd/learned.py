@@ -1,0 +1,490 @@
+"""Learned matching order for the fused pipeline's join.
+
+The reference's join matches query nodes in identity order (isomorphism.hpp:371-447; its
+alternative defineMatchingOrder, isomorphism.hpp:476-502, is unused).  With standard
+backtracking the per-pair counts do not depend on the order, only the number of DFS nodes does,
+so any permutation keeps results exact.  This module:
+
+* logs join traces from plain runs (bms2_pipeline_trace): per query node the summed domain size
+  over its GMCR pairs, per query the GMCR pairs and DFS nodes, per pair the DFS nodes;
+* labels every query with the cheapest of a set of candidate orders (identity, the reference's
+  max-domain root, and a connected greedy order from every root) measured by realised DFS nodes,
+  and with its useful refinement depth (the smallest depth whose pruning is within 1% of depth 8);
+* trains the matching-order network (bms2_mlp_*: hand-written HIP forward, backward and Adam
+  kernels, f32) on per-query-node features;
+* plans a query batch (bf16 inference): per query a connected order by descending predicted
+  score; identity order takes over when the predicted cost ratio to identity is not below the
+  budget.
+
+The network's refinement-depth head is trained and reported but not applied: the pipeline's shell
+histograms at distance k >= 3 can remove true embeddings, so a per-query depth other than the
+run's K would change counts, which must stay equal to the plain pipeline's.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import sys
+import time
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+from . import _lib
+from .api import DeviceBuffer, Pipeline, Queue, _Handle, _L, check
+
+NUM_FEATURES = 32
+NUM_OUTPUTS = 10
+DEFAULT_WEIGHTS = Path(__file__).resolve().parent / "order_net.npz"
+# element codes of the project generator (workload.cpp): N, Cl, Br, O, C, S, F
+_ELEMENTS = (0, 1, 3, 7, 8, 9, 10)
+_WILDCARD = 255
+
+
+class OrderNet(_Handle):
+    """The matching-order network (bms2_mlp_*) on one queue."""
+
+    _destroy = "bms2_mlp_destroy"
+
+    def __init__(self, q: Queue, seed: int = 0):
+        self.q = q
+        h = C.c_void_p()
+        check(_L().bms2_mlp_create(q.h, int(seed), C.byref(h)))
+        self.h = h
+        self.num_params = int(_L().bms2_mlp_num_params())
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.num_params, np.float32)
+        check(_L().bms2_mlp_get_params(self.h, p.ctypes.data_as(C.c_void_p), self.num_params))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.num_params:
+            raise ValueError(f"expected {self.num_params} parameters, got {p.size}")
+        check(_L().bms2_mlp_set_params(self.h, p.ctypes.data_as(C.c_void_p), self.num_params))
+
+    def forward(self, x: np.ndarray, bf16: bool = True) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        rows = x.shape[0]
+        if rows == 0:
+            return np.zeros((0, NUM_OUTPUTS), np.float32)
+        dx = DeviceBuffer.from_array(self.q, x)
+        dy = DeviceBuffer(self.q, 4 * rows * NUM_OUTPUTS, zero=False)
+        check(_L().bms2_mlp_forward(self.h, C.c_void_p(dx.ptr), rows, C.c_void_p(dy.ptr),
+                                    int(bf16)))
+        return dy.to_numpy(np.float32, rows * NUM_OUTPUTS).reshape(rows, NUM_OUTPUTS)
+
+    def loss_grad(self, x: np.ndarray, t: np.ndarray, depth_weight: float = 1.0):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        dx, dt = DeviceBuffer.from_array(self.q, x), DeviceBuffer.from_array(self.q, t)
+        loss = C.c_float(0)
+        g = np.empty(self.num_params, np.float32)
+        check(_L().bms2_mlp_loss_grad(self.h, C.c_void_p(dx.ptr), C.c_void_p(dt.ptr), x.shape[0],
+                                      float(depth_weight), C.byref(loss),
+                                      g.ctypes.data_as(C.c_void_p)))
+        return float(loss.value), g
+
+    def train_step(self, dx: DeviceBuffer, dt: DeviceBuffer, rows: int, lr: float,
+                   depth_weight: float = 1.0, beta1=0.9, beta2=0.999, eps=1e-8) -> float:
+        opt = _lib.Adam(lr, beta1, beta2, eps)
+        loss = C.c_float(0)
+        check(_L().bms2_mlp_train_step(self.h, C.c_void_p(dx.ptr), C.c_void_p(dt.ptr), int(rows),
+                                       float(depth_weight), C.byref(opt), C.byref(loss)))
+        return float(loss.value)
+
+    def save(self, path: str | Path, meta: dict | None = None) -> None:
+        np.savez(path, params=self.get_params(),
+                 meta=np.frombuffer(json.dumps(meta or {}).encode(), np.uint8))
+
+    @staticmethod
+    def load(q: Queue, path: str | Path = DEFAULT_WEIGHTS) -> "OrderNet":
+        with np.load(path, allow_pickle=False) as z:
+            p = z["params"]
+        net = OrderNet(q)
+        net.set_params(p)
+        return net
+
+
+# ---------------------------------------------------------------- query structure and features
+
+@dataclass
+class QueryInfo:
+    n: int
+    adj: list            # neighbour sets (local ids)
+    deg: np.ndarray
+    dist: np.ndarray     # all-pairs BFS distances (n x n, large for unreachable)
+    labels: np.ndarray
+    on_cycle: np.ndarray
+
+
+def query_infos(queries) -> list[QueryInfo]:
+    go = np.asarray(queries.graph_offsets, np.int64)
+    ro = np.asarray(queries.row_offsets, np.int64)
+    ci = np.asarray(queries.column_indices, np.int64)
+    lab = np.asarray(queries.node_labels)
+    out = []
+    for q in range(queries.num_graphs):
+        b, e = go[q], go[q + 1]
+        n = int(e - b)
+        adj = [set((ci[ro[b + u]:ro[b + u + 1]] - b).tolist()) for u in range(n)]
+        deg = np.array([len(a) for a in adj], np.int64)
+        dist = np.full((n, n), 99, np.int64)
+        for s in range(n):
+            dist[s, s] = 0
+            frontier = [s]
+            while frontier:
+                nxt = []
+                for u in frontier:
+                    for v in adj[u]:
+                        if dist[s, v] == 99:
+                            dist[s, v] = dist[s, u] + 1
+                            nxt.append(v)
+                frontier = nxt
+        # a node lies on a cycle when one of its edges is not a bridge: u-v is a bridge iff v is
+        # unreachable from u once the edge is removed
+        on_cycle = np.zeros(n, bool)
+        for u in range(n):
+            for v in adj[u]:
+                seen, stack = {u}, [u]
+                while stack:
+                    x = stack.pop()
+                    for y in adj[x]:
+                        if (x, y) in ((u, v), (v, u)) or y in seen:
+                            continue
+                        seen.add(y)
+                        stack.append(y)
+                if v in seen:
+                    on_cycle[u] = True
+                    break
+        out.append(QueryInfo(n, adj, deg, dist, lab[b:e].copy(), on_cycle))
+    return out
+
+
+def features(infos: list[QueryInfo], domsum: np.ndarray, qpairs: np.ndarray) -> np.ndarray:
+    """[total query nodes][32] f32 features, rows in global query node order."""
+    total = sum(qi.n for qi in infos)
+    X = np.zeros((total, NUM_FEATURES), np.float32)
+    all_labels = np.concatenate([qi.labels for qi in infos]) if infos else np.zeros(0, np.uint8)
+    freq = np.bincount(all_labels, minlength=256).astype(np.float64) / max(len(all_labels), 1)
+    r = 0
+    for q, qi in enumerate(infos):
+        n = qi.n
+        pairs = float(qpairs[q])
+        dom = domsum[r:r + n].astype(np.float64) / pairs if pairs > 0 else np.zeros(n)
+        m = int(qi.deg.sum()) // 2
+        reach = np.where(qi.dist < 99, qi.dist, 0)
+        ecc = reach.max(axis=1) if n else np.zeros(0)
+        meandom = float(dom.mean()) if n else 0.0
+        deg_rank = np.argsort(np.argsort(-qi.deg, kind="stable"), kind="stable")
+        dom_rank = np.argsort(np.argsort(dom, kind="stable"), kind="stable")
+        amax_deg = int(np.argmax(qi.deg)) if n else -1
+        amin_dom = int(np.argmin(dom)) if n else -1
+        for u in range(n):
+            f = X[r + u]
+            f[0] = n / 16
+            f[1] = m / 16
+            f[2] = (m - n + 1) / 4
+            f[3] = qi.deg.max() / 4
+            f[4] = ecc.max() / 8
+            f[5] = len(set(qi.labels.tolist())) / 8
+            f[6] = np.log2(1 + pairs) / 16
+            f[7] = np.log2(1 + meandom) / 6
+            f[8] = qi.deg[u] / 4
+            f[9] = ecc[u] / 8
+            f[10] = reach[u].sum() / max(n - 1, 1) / 4
+            f[11] = np.log2(1 + dom[u]) / 6
+            f[12] = np.log2((1 + dom[u]) / (1 + meandom)) / 4
+            nb = list(qi.adj[u])
+            f[13] = sum(1 for v in nb if qi.deg[v] == 1) / 4
+            f[14] = sum(int(qi.deg[v]) for v in nb) / 16
+            f[15] = float(qi.on_cycle[u])
+            lab = int(qi.labels[u])
+            f[16] = float(lab == _WILDCARD)
+            if lab in _ELEMENTS:
+                f[17 + _ELEMENTS.index(lab)] = 1.0
+            elif lab != _WILDCARD:
+                f[24] = 1.0
+            f[25] = u / max(n - 1, 1)
+            f[26] = np.log2(1 + min(dom[v] for v in nb)) / 6 if nb else 0.0
+            f[27] = -np.log2(max(freq[lab], 1e-9)) / 8
+            f[28] = deg_rank[u] / max(n - 1, 1)
+            f[29] = dom_rank[u] / max(n - 1, 1)
+            f[30] = float(u == amax_deg)
+            f[31] = float(u == amin_dom)
+        r += n
+    return X
+
+
+# ---------------------------------------------------------------- orders
+
+def greedy_order(qi: QueryInfo, root: int, dom: np.ndarray) -> list[int]:
+    """Connected greedy order from root: next = most already-matched neighbours, then smaller
+    mean domain, then higher degree, then lower id."""
+    order, S = [root], {root}
+    while len(order) < qi.n:
+        best, key = -1, None
+        for u in range(qi.n):
+            if u in S:
+                continue
+            k = (-len(qi.adj[u] & S), dom[u], -int(qi.deg[u]), u)
+            if key is None or k < key:
+                best, key = u, k
+        order.append(best)
+        S.add(best)
+    return order
+
+
+def score_order(qi: QueryInfo, scores: np.ndarray) -> list[int]:
+    """Connected order by descending score: start at the best-scored node, then always the
+    best-scored node adjacent to the matched set (any remaining node when none is)."""
+    n = qi.n
+    order = [int(np.argmax(scores))]
+    S = set(order)
+    while len(order) < n:
+        front = [u for u in range(n) if u not in S and qi.adj[u] & S]
+        pool = front or [u for u in range(n) if u not in S]
+        u = max(pool, key=lambda v: (scores[v], -v))
+        order.append(u)
+        S.add(u)
+    return order
+
+
+def candidate_orders(infos: list[QueryInfo], domsum: np.ndarray, qpairs: np.ndarray):
+    """List of (name, order [Q][32] u8): identity, reference max-domain root, greedy from root r
+    (queries with fewer nodes repeat their identity order)."""
+    Q = len(infos)
+    maxn = max((qi.n for qi in infos), default=0)
+    offs = np.cumsum([0] + [qi.n for qi in infos])
+    doms = [domsum[offs[q]:offs[q + 1]].astype(np.float64) / max(float(qpairs[q]), 1.0)
+            for q in range(Q)]
+    ident = np.zeros((Q, 32), np.uint8)
+    for q, qi in enumerate(infos):
+        ident[q, :qi.n] = np.arange(qi.n)
+    out = [("identity", ident)]
+    ref = ident.copy()
+    for q, qi in enumerate(infos):
+        if qi.n:
+            root = int(np.argmax(doms[q]))  # defineMatchingOrder: max-domain root, then identity
+            ref[q, :qi.n] = [root] + [u for u in range(qi.n) if u != root]
+    out.append(("ref_maxdom", ref))
+    for r in range(maxn):
+        o = ident.copy()
+        for q, qi in enumerate(infos):
+            if r < qi.n:
+                o[q, :qi.n] = greedy_order(qi, r, doms[q])
+        out.append((f"greedy_root{r}", o))
+    return out
+
+
+# ---------------------------------------------------------------- traces and labels
+
+@dataclass
+class SeedTraces:
+    seed: int
+    X: np.ndarray
+    T: np.ndarray
+    pairs_traced: int
+    identity_nodes: int
+    best_nodes: int
+    info: dict = field(default_factory=dict)
+
+
+def _trace(pipe: Pipeline, graphs: int, qn: int, order=None, pair_nodes=False) -> dict:
+    pipe.set_order(order)
+    return pipe.trace(graphs, qn, pair_nodes=pair_nodes)
+
+
+def collect_seed(q: Queue, seed: int, num_data_graphs: int = 10000, K: int = 6,
+                 workload: str = "molecules-1m", depth_labels: bool = True) -> SeedTraces:
+    """Plain runs on one seed's workload: candidate orders and depths, as training rows."""
+    from . import graphs as G
+    from .api import DeviceBatchedCSRGraph
+    data, queries = G.generate(workload, seed=seed, num_data_graphs=num_data_graphs)
+    dq, dd = DeviceBatchedCSRGraph(q, queries), DeviceBatchedCSRGraph(q, data)
+    qn = queries.total_nodes
+    infos = query_infos(queries)
+    pipe = Pipeline(q, dq, dd, refinement_steps=K)
+    base = _trace(pipe, data.num_graphs, qn, None, pair_nodes=True)
+    domsum, qpairs = base["domsum"], base["qpairs"]
+    cands = candidate_orders(infos, domsum, qpairs)
+    nodes = np.zeros((len(cands), queries.num_graphs), np.float64)
+    for i, (name, o) in enumerate(cands):
+        tr = base if name == "identity" else _trace(pipe, data.num_graphs, qn, o)
+        if name != "identity" and not np.array_equal(tr["counts"], base["counts"]):
+            raise RuntimeError(f"seed {seed}: counts changed under order {name}")
+        nodes[i] = tr["qnodes"]
+    pipe.set_order(None)
+    del pipe
+    best = np.argmin(nodes, axis=0)  # ties keep the earliest candidate (identity first)
+    # useful refinement depth: smallest k whose GMCR pairs and DFS nodes are within 1% of k = 8
+    depth_idx = np.zeros(queries.num_graphs, np.int64)
+    if depth_labels:
+        per_k = []
+        for k in range(1, 9):
+            pk = Pipeline(q, dq, dd, refinement_steps=k)
+            tr = pk.trace(data.num_graphs, qn)
+            per_k.append((tr["qpairs"].astype(np.float64), tr["qnodes"].astype(np.float64)))
+            del pk
+        p8, n8 = per_k[-1]
+        for qq in range(queries.num_graphs):
+            for k in range(8):
+                pk_, nk_ = per_k[k][0][qq], per_k[k][1][qq]
+                if pk_ <= 1.01 * p8[qq] + 1 and nk_ <= 1.01 * n8[qq] + 1:
+                    depth_idx[qq] = k
+                    break
+    X = features(infos, domsum, qpairs)
+    T = np.zeros((X.shape[0], 4), np.float32)
+    r = 0
+    keep = np.zeros(X.shape[0], bool)
+    for qq, qi in enumerate(infos):
+        n = qi.n
+        o = cands[best[qq]][1][qq, :n]
+        pos = np.empty(n, np.int64)
+        pos[o] = np.arange(n)
+        ident_nodes, best_nodes = nodes[0, qq], nodes[best[qq], qq]
+        T[r:r + n, 0] = 1.0 - pos / max(n - 1, 1)
+        T[r:r + n, 1] = np.log2((best_nodes + 1) / (ident_nodes + 1))
+        T[r:r + n, 2] = depth_idx[qq]
+        T[r:r + n, 3] = np.log2(2 + ident_nodes)
+        keep[r:r + n] = qpairs[qq] > 0 and n > 1
+        r += n
+    return SeedTraces(seed, X[keep], T[keep], int((base["pair_nodes"] > 0).sum()),
+                      int(nodes[0].sum()), int(nodes[best, np.arange(queries.num_graphs)].sum()),
+                      dict(candidates=len(cands), queries=queries.num_graphs,
+                           best_counts={cands[i][0]: int((best == i).sum())
+                                        for i in range(len(cands)) if (best == i).any()}))
+
+
+def train(net: OrderNet, X: np.ndarray, T: np.ndarray, epochs: int = 40, batch: int = 4096,
+          lr: float = 2e-3, depth_weight: float = 0.25, seed: int = 0, log=None) -> list[float]:
+    rng = np.random.default_rng(seed)
+    rows = X.shape[0]
+    losses = []
+    for ep in range(epochs):
+        perm = rng.permutation(rows)
+        tot, nb = 0.0, 0
+        for s in range(0, rows, batch):
+            idx = perm[s:s + batch]
+            dx = DeviceBuffer.from_array(net.q, np.ascontiguousarray(X[idx], np.float32))
+            dt = DeviceBuffer.from_array(net.q, np.ascontiguousarray(T[idx], np.float32))
+            tot += net.train_step(dx, dt, len(idx), lr * (0.5 ** (ep // 15)), depth_weight)
+            nb += 1
+        losses.append(tot / max(nb, 1))
+        if log:
+            log(f"epoch {ep}: loss {losses[-1]:.4f}")
+    return losses
+
+
+# ---------------------------------------------------------------- planning
+
+@dataclass
+class Plan:
+    order: np.ndarray          # [Q][32] u8
+    learned: np.ndarray        # [Q] bool: learned order used (False: identity fallback)
+    depth: np.ndarray          # [Q] predicted useful refinement depth 1..8 (reported only)
+    predicted_log_ratio: np.ndarray
+    sample_graphs: int
+    seconds: float
+
+
+def plan(pipe: Pipeline, net: OrderNet, queries, sample_graphs: int = 16384,
+         budget_log_ratio: float = -0.02, infos: list[QueryInfo] | None = None) -> Plan:
+    """Traces a sample of the resident data graphs (identity order), predicts per query node
+    with the network in bf16, sets the pipeline's matching order and returns the plan."""
+    t0 = time.perf_counter()
+    infos = infos if infos is not None else query_infos(queries)
+    pipe.set_order(None)
+    tr = pipe.trace(sample_graphs, queries.total_nodes)
+    X = features(infos, tr["domsum"], tr["qpairs"])
+    Y = net.forward(X, bf16=True)
+    Q = len(infos)
+    order = np.zeros((Q, 32), np.uint8)
+    learned = np.zeros(Q, bool)
+    depth = np.zeros(Q, np.int64)
+    ratio = np.zeros(Q, np.float32)
+    r = 0
+    for qq, qi in enumerate(infos):
+        n = qi.n
+        order[qq, :n] = np.arange(n)
+        if n:
+            y = Y[r:r + n]
+            ratio[qq] = float(y[:, 1].mean())
+            depth[qq] = 1 + int(np.argmax(y[:, 2:].mean(axis=0)))
+            if n > 1 and ratio[qq] < budget_log_ratio:
+                order[qq, :n] = score_order(qi, y[:, 0])
+                learned[qq] = True
+        r += n
+    pipe.set_order(order)
+    return Plan(order, learned, depth, ratio, tr["graphs"], time.perf_counter() - t0)
+
+
+# ---------------------------------------------------------------- command line
+
+def _main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="python -m bms2_amd.learned")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train", help="collect traces on seeds and train the order network")
+    t.add_argument("--seeds", default="1-20")
+    t.add_argument("--graphs-per-seed", type=int, default=10000)
+    t.add_argument("--epochs", type=int, default=40)
+    t.add_argument("--out", default=str(DEFAULT_WEIGHTS))
+    t.add_argument("--no-depth", action="store_true")
+    e = sub.add_parser("evaluate", help="DFS nodes and counts, learned vs identity order")
+    e.add_argument("--seed", type=int, default=100)
+    e.add_argument("--graphs", type=int, default=100000)
+    e.add_argument("--weights", default=str(DEFAULT_WEIGHTS))
+    args = ap.parse_args(argv)
+    log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
+    q = Queue(0)
+    if args.cmd == "train":
+        a, b = (int(x) for x in args.seeds.split("-"))
+        Xs, Ts, meta = [], [], dict(seeds=args.seeds, graphs_per_seed=args.graphs_per_seed,
+                                    per_seed=[])
+        traced = 0
+        for s in range(a, b + 1):
+            t0 = time.perf_counter()
+            st = collect_seed(q, s, args.graphs_per_seed, depth_labels=not args.no_depth)
+            Xs.append(st.X)
+            Ts.append(st.T)
+            traced += st.pairs_traced
+            meta["per_seed"].append(dict(seed=s, rows=int(st.X.shape[0]), pairs=st.pairs_traced,
+                                         identity_nodes=st.identity_nodes,
+                                         best_nodes=st.best_nodes, **st.info))
+            log(f"seed {s}: {st.X.shape[0]} rows, {st.pairs_traced} traced pairs, DFS nodes "
+                f"identity {st.identity_nodes} best {st.best_nodes} "
+                f"({time.perf_counter() - t0:.1f}s)")
+        X, T = np.concatenate(Xs), np.concatenate(Ts)
+        net = OrderNet(q, seed=1)
+        losses = train(net, X, T, epochs=args.epochs, log=log)
+        meta.update(rows=int(X.shape[0]), traced_pairs=traced, final_loss=losses[-1],
+                    first_loss=losses[0])
+        net.save(args.out, meta)
+        print(json.dumps(dict(rows=int(X.shape[0]), traced_pairs=traced, first_loss=losses[0],
+                              final_loss=losses[-1], out=args.out)))
+        return 0
+    if args.cmd == "evaluate":
+        from . import graphs as G
+        from .api import DeviceBatchedCSRGraph
+        data, queries = G.generate("molecules-1m", seed=args.seed, num_data_graphs=args.graphs)
+        dq, dd = DeviceBatchedCSRGraph(q, queries), DeviceBatchedCSRGraph(q, data)
+        pipe = Pipeline(q, dq, dd, refinement_steps=6)
+        net = OrderNet.load(q, args.weights)
+        ident = pipe.trace(data.num_graphs, queries.total_nodes)
+        pl = plan(pipe, net, queries)
+        lrn = pipe.trace(data.num_graphs, queries.total_nodes)
+        same = bool(np.array_equal(ident["counts"], lrn["counts"]))
+        out = dict(seed=args.seed, graphs=data.num_graphs,
+                   identity_dfs_nodes=int(ident["qnodes"].sum()),
+                   learned_dfs_nodes=int(lrn["qnodes"].sum()), counts_equal=same,
+                   learned_queries=int(pl.learned.sum()), plan_seconds=round(pl.seconds, 3))
+        print(json.dumps(out))
+        return 0 if same else 1
+    return 2
+
+
+if __name__ == "__main__":
+    sys.exit(_main())

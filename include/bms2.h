@@ -255,6 +255,62 @@ int bms2_pipeline_destroy(bms2_pipeline p);
 int bms2_pipeline_run(bms2_pipeline p, uint64_t* d_counts, uint64_t* d_valid, uint64_t* d_stats,
                       bms2_event* ev, bms2_event* kernel_ev);
 
+/* Matching order of the join, per query: order[q*32 + d] = local query node matched at DFS
+ * depth d (a permutation of 0..n_q-1 per query; NULL restores identity, the reference's order,
+ * isomorphism.hpp:371-447).  Counts do not depend on the order (standard backtracking); only the
+ * number of DFS nodes does.  The reference's join fixes identity order; defineMatchingOrder
+ * (isomorphism.hpp:476-502) is its unused alternative. */
+int bms2_pipeline_set_order(bms2_pipeline p, const uint8_t* order);
+
+/* Join traces over the data graphs of the leading packs (at least max_graphs graphs, fewer when
+ * the batch is smaller): the pipeline runs on graphs [0, *graphs_out) and accumulates, per query
+ * node (global query node id), the summed domain size over its GMCR pairs (d_domsum u64[Qn]),
+ * per query the number of GMCR pairs (d_qpairs u64[Q]) and DFS nodes expanded (d_qnodes u64[Q]),
+ * and per pair the DFS nodes (d_pair_nodes u32[graphs][Q], saturating, nullable).  The caller
+ * zeroes the accumulators.  d_counts / d_valid are sized for the traced graphs as in
+ * bms2_pipeline_run.  No counterpart in the reference: it feeds the learned matching order. */
+int bms2_pipeline_trace(bms2_pipeline p, uint32_t max_graphs, uint64_t* d_counts,
+                        uint64_t* d_valid, uint64_t* d_domsum, uint64_t* d_qpairs,
+                        uint64_t* d_qnodes, uint32_t* d_pair_nodes, uint32_t* graphs_out);
+
+/* ---------------------------------------------------------------- matching-order network
+ * No counterpart in the reference (its join fixes identity order).  A multilayer perceptron
+ * 32 -> 64 -> 64 -> 64 -> 10 with ReLU on the three hidden layers, one row per query node:
+ * outputs [0] order score (higher = matched earlier), [1] predicted log2 of the DFS-node ratio
+ * learned / identity order for the node's query, [2..9] logits of the useful refinement depth
+ * 1..8.  Parameters are one f32 array: W1[32][64], b1[64], W2[64][64], b2[64], W3[64][64],
+ * b3[64], W4[64][10], b4[10] (W[i][j] connects input i to output j).
+ * Loss over a batch of rows r with targets t_r = {score, log2 ratio, depth index 0..7, weight w}:
+ *   L = sum_r w_r [(y0 - t0)^2 + (y1 - t1)^2 + depth_weight * CE(softmax(y2..9), t2)] / sum_r w_r.
+ * Training is f32 with Adam; inference is f32 or bf16 (weights and activations rounded to bf16
+ * to nearest even, f32 accumulation). */
+#define BMS2_MLP_IN 32
+#define BMS2_MLP_HIDDEN 64
+#define BMS2_MLP_OUT 10
+#define BMS2_MLP_TARGETS 4
+
+typedef struct bms2_adam {
+  float lr, beta1, beta2, eps;
+} bms2_adam;
+
+typedef struct bms2_mlp_s* bms2_mlp;
+uint64_t bms2_mlp_num_params(void);
+/* He-uniform weights from a splitmix64 stream of `seed`, zero biases, zero Adam state. */
+int bms2_mlp_create(bms2_queue q, uint64_t seed, bms2_mlp* out);
+int bms2_mlp_destroy(bms2_mlp m);
+int bms2_mlp_get_params(bms2_mlp m, float* host, uint64_t n);
+/* Replaces the parameters and resets the Adam state. */
+int bms2_mlp_set_params(bms2_mlp m, const float* host, uint64_t n);
+/* d_x: f32 [rows][32] (device); d_y: f32 [rows][10] (device). */
+int bms2_mlp_forward(bms2_mlp m, const float* d_x, uint32_t rows, float* d_y, int bf16);
+/* Loss and its gradient (grad_host: f32[num_params], nullable) at the current parameters;
+ * d_t: f32 [rows][4] targets (device). */
+int bms2_mlp_loss_grad(bms2_mlp m, const float* d_x, const float* d_t, uint32_t rows,
+                       float depth_weight, float* loss, float* grad_host);
+/* One Adam step on the batch; *loss (nullable) receives the loss before the step. */
+int bms2_mlp_train_step(bms2_mlp m, const float* d_x, const float* d_t, uint32_t rows,
+                        float depth_weight, const bms2_adam* opt, float* loss);
+
 #ifdef __cplusplus
 }
 #endif
