@@ -1,17 +1,21 @@
 """Throughput benchmark of the fused batched pipeline on one MI355X.
 
 Workload (default): 1,000,000 synthetic molecules (10-40 atoms, seed 0) x 1,024 queries of 3-16
-nodes (70% cut from the data graphs), K = 6 refinement steps, count-all, the reference's identity
-matching order.  One step = one bms2_pipeline_run over the resident batch: query signatures,
-data signatures, filter, 6 refinement levels, GMCR, join, one u64 count per (graph, query) pair
-and the GMCR bit rows, all written to HBM.
+nodes (70% cut from the data graphs), K = 6 refinement steps, count-all.  One step = one
+bms2_pipeline_run over the resident batch: query signatures, data signatures, filter, 6
+refinement levels, GMCR, join, one u64 count per (graph, query) pair and the GMCR bit rows, all
+written to HBM.
 
-Prints ONE JSON line.  `value` is screened (query, graph) pairs per second with the CSR batch
-already resident in HBM; `roofline` prices the pipeline's device span (screen + join kernels of
-every chunk, timed with HIP events on the pipeline's stream) against HBM bandwidth with the
-canonical algorithmic bytes; `cpu_baseline` times the oracle's fused CPU path (OpenMP) on a
-bounded prefix of the same data graphs with all queries, on this host, and checks the GPU counts
-of that prefix against it.
+Two legs run on the same resident batch: plain (the reference's identity matching order) and
+learned (matching order planned once for the query batch by the order network, trained on seeds
+1-20 and shipped in bms2_amd/order_net.npz; planning traces 16,384 data graphs and runs bf16
+inference, timed separately as plan_ms).  `value` is the learned leg's screened (query, graph)
+pairs per second with the CSR batch already resident in HBM; both legs must produce the same
+counts.  `roofline` prices the learned leg's device span (screen + join kernels of every chunk,
+timed with HIP events on the pipeline's stream) against HBM bandwidth with the canonical
+algorithmic bytes; `cpu_baseline` times the oracle's fused CPU path (OpenMP) on a bounded prefix
+of the same data graphs with all queries, on this host, and checks the GPU counts of that prefix
+against it.
 """
 from __future__ import annotations
 
@@ -59,15 +63,21 @@ def main() -> int:
     ap.add_argument("--data-graphs", type=int, default=0, help="override the number of data graphs")
     ap.add_argument("--refinement-steps", type=int, default=6)
     ap.add_argument("--find-first", action="store_true")
-    ap.add_argument("--cpu-sample-graphs", type=int, default=131072)
+    ap.add_argument("--cpu-sample-graphs", type=int, default=0,
+                    help="data graphs the CPU baseline runs on (0: all)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="time the plain leg only")
+    ap.add_argument("--skip-plain", action="store_true",
+                    help="time the learned leg only (profiling runs)")
+    ap.add_argument("--plan-graphs", type=int, default=16384,
+                    help="data graphs traced to plan the learned order")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              "(the pipeline shards over data graphs with no cross-GPU traffic, see DESIGN.md)",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: the pipeline "
+              "would shard data graphs across GPUs with no cross-GPU traffic during compute, and "
+              "this build runs and reports one GPU's share", file=sys.stderr)
         return 2
 
     from bms2_amd import api, graphs
@@ -86,27 +96,55 @@ def main() -> int:
     counts, valid, stats = pipe.alloc_outputs()
     q.wait()
     print("[bench] resident batch uploaded; warming up", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        pipe.run(counts, valid, stats)
-    q.wait()
-
-    kernel_events = []
-    q.wait()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        _, kev = pipe.run(counts, valid, stats, kernel_event=True)
-        kernel_events.append(kev)
-    q.wait()
-    t_total = time.perf_counter() - t_start
-    kernel_ms = [e.elapsed_ms() for e in kernel_events]
-    st = stats.to_numpy(np.uint64, 8)
-
     D, Q = data.num_graphs, queries.num_graphs
     pairs = D * Q
+
+    def leg(name):
+        for _ in range(args.warmup):
+            pipe.run(counts, valid, stats)
+        q.wait()
+        evs = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, kev = pipe.run(counts, valid, stats, kernel_event=True)
+            evs.append(kev)
+        q.wait()
+        t = time.perf_counter() - t0
+        st = stats.to_numpy(np.uint64, 8)
+        kms = [e.elapsed_ms() for e in evs]
+        print(f"[bench] {name}: {args.steps} steps, {1000 * t / args.steps:.2f} ms/step, "
+              f"{pairs * args.steps / t:.4g} pairs/s, device span {np.mean(kms):.2f} ms, DFS nodes "
+              f"{int(st[3])}", file=sys.stderr, flush=True)
+        return t, kms, st
+
+    order = "identity"
+    S = min(args.cpu_sample_graphs or D, D)
+    plain_sample = None
+    if not (args.skip_plain and not args.no_learned):
+        t_plain, kms_plain, st_plain = leg("plain (identity order)")
+        # plain counts of the leading graphs; the learned leg must reproduce them (and the totals)
+        plain_sample = pipe.results(counts, valid, rows=S)
+    plan_info = None
+    if not args.no_learned:
+        from bms2_amd import learned
+        net = learned.OrderNet.load(q)
+        pl = learned.plan(pipe, net, queries, sample_graphs=args.plan_graphs)
+        plan_info = dict(plan_ms=round(1000 * pl.seconds, 1), sample_graphs=pl.sample_graphs,
+                         queries_reordered=int(pl.learned.sum()))
+        print(f"[bench] learned plan: {plan_info}", file=sys.stderr, flush=True)
+        order = "learned"
+    t_total, kernel_ms, st = leg(f"{order} order")
+    if plan_info is not None and plain_sample is not None:
+        lc, lv = pipe.results(counts, valid, rows=plain_sample[0].shape[0])
+        same = (np.array_equal(lc, plain_sample[0]) and np.array_equal(lv, plain_sample[1])
+                and all(int(st[k]) == int(st_plain[k]) for k in range(3)))
+        if not same:
+            print("[bench] learned and plain counts differ", file=sys.stderr, flush=True)
+            return 1
+
     ms_per_step = 1000.0 * t_total / args.steps
     value = pairs * args.steps / t_total
-    print(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, {value:.4g} pairs/s, device "
-          f"span {np.mean(kernel_ms):.2f} ms avg", file=sys.stderr, flush=True)
+    plain_value = pairs * args.steps / t_plain if plain_sample is not None else None
 
     abytes = algorithmic_bytes(data, queries)
     k_avg_s = float(np.mean(kernel_ms)) / 1000.0
@@ -116,7 +154,7 @@ def main() -> int:
         "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
         "traffic": (traffic or {}).get("bytes_per_launch"),
-        "kernel": f"screen_kernel<{K}> + join_kernel (device span per run)",
+        "kernel": f"screen_kernel<{K}> + join_kernel (device span per run, {order} order)",
         "kernel_ms_avg": round(float(np.mean(kernel_ms)), 3),
         "algorithmic_bytes_per_launch": abytes,
         "bytes_per_pair": round(abytes / pairs, 4),
@@ -126,7 +164,6 @@ def main() -> int:
     parity = None
     if not args.no_cpu_baseline:
         import oracle
-        S = min(args.cpu_sample_graphs, D)
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         sub = data.subset(0, S)
         print(f"[bench] CPU baseline: oracle fused path on the first {S} graphs x {Q} queries, "
@@ -136,9 +173,13 @@ def main() -> int:
         cpu_s = time.perf_counter() - t
         cpu = {"value": round(S * Q / cpu_s, 1), "unit": "pairs/s", "cores": threads,
                "kind": "port", "seconds": round(cpu_s, 2),
-               "sample": f"first {S} of {D} data graphs x all {Q} queries, same K and mode"}
+               "sample": (f"all {D}" if S == D else f"first {S} of {D}") +
+                         f" data graphs x all {Q} queries, same K and mode, from host CSR"}
         gc, gv = pipe.results(counts, valid, rows=S)
         parity = bool(np.array_equal(gc, rc) and np.array_equal(gv, rv))
+        if plain_sample is not None:
+            parity = parity and bool(np.array_equal(plain_sample[0][:S], rc))
+        del rc, rv, gc, gv
         print(f"[bench] CPU {cpu['value']:.4g} pairs/s in {cpu_s:.1f}s; GPU/oracle counts on the "
               f"sample equal: {parity}", file=sys.stderr, flush=True)
 
@@ -149,14 +190,19 @@ def main() -> int:
         "data": "synthetic (project generator, seed 0)",
         "config": {
             "workload": f"{args.workload}: {D} molecules x {Q} queries, K={K}, "
-                        f"{'first-match' if args.find_first else 'count-all'}, identity order, "
-                        "plain HIP pipeline",
+                        f"{'first-match' if args.find_first else 'count-all'}, {order} matching "
+                        "order, batched HIP pipeline",
             "data_graphs": D, "queries": Q, "refinement_steps": K,
-            "mode": "first-match" if args.find_first else "count-all",
+            "mode": "first-match" if args.find_first else "count-all", "order": order,
         },
         "matched_pairs_per_s": round(float(st[1]) * args.steps / t_total, 1),
         "gmcr_pairs": int(st[0]), "matched_pairs": int(st[1]), "embeddings": int(st[2]),
         "dfs_nodes": int(st[3]),
+        "plain": None if plain_value is None else {
+            "value": round(plain_value, 1), "ms_per_step": round(1000 * t_plain / args.steps, 3),
+            "kernel_ms_avg": round(float(np.mean(kms_plain)), 3), "dfs_nodes": int(st_plain[3])},
+        "learned_vs_plain": None if plain_value is None else round(value / plain_value, 4),
+        "learned_plan": plan_info,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity_on_cpu_sample": parity,
