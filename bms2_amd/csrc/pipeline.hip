@@ -29,6 +29,8 @@ namespace bms2 {
 constexpr int kPackGraphs = 8;   // graphs per pack
 constexpr int kStageRecs = 32;   // LDS-staged pair records per wave
 constexpr int kRecDwords = 48;   // screening record of one query node
+constexpr int kPreDwords = 20;   // prefilter record of one query
+constexpr int kSumDwords = 16;   // prefilter summary of one data graph
 
 // Screening record layout (u32 words): [0] label | deg << 8 | local id << 16 | level mask << 24,
 // [1..8] path Bloom, [9..10] ring counts, [12 + 4k .. 15 + 4k] shell level k.  Shell words hold
@@ -43,7 +45,7 @@ struct ScreenParams {
   uint32_t pack_begin;
   uint32_t Q;
   const uint32_t* qhead;    // [Q][4]: first record, nq, valid (nq > 1, gmcr.hpp:64), 0
-  const uint32_t* qpre;     // [Q][16]: required presence bits [0..3], path union [4..11], valid [12]
+  const uint32_t* qpre;     // [Q][kPreDwords] prefilter records
   const uint32_t* qrec;     // [Qn][kRecDwords] records in screening order per query
   const uint8_t* jorder;    // [Q][32] local query node matched at each DFS depth
   uint64_t* adj_out;        // [Dn] adjacency rows over pack lanes
@@ -128,14 +130,15 @@ __global__ __launch_bounds__(64) void query_prep_kernel(QueryPrepParams p) {
   for (int w = 12 + 4 * kMaxLevels; w < kRecDwords; ++w) r[w] = 0;
 }
 
-// Per-query prefilter record: [0..3] presence bits every node needs (label < 16 at its degree
-// threshold, capped at 7), [4..11] union of the nodes' path Bloom filters, [12] nq > 1.
+// Per-query prefilter record (kPreDwords): [0..3] presence bits every node needs (label < 16 at
+// its degree threshold, capped at 7), [4..11] union of the nodes' path Bloom filters, [12..15]
+// the largest of each ring-count field over the nodes, [16] nq > 1, [17] first record, [18] nq.
 __global__ void query_pre_kernel(const uint32_t* __restrict__ qhead, const uint32_t* __restrict__ qrec,
                                  uint32_t Q, uint32_t* __restrict__ qpre) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= Q) return;
   const uint32_t rb = qhead[4 * q], nq = qhead[4 * q + 1];
-  uint32_t out[16] = {};
+  uint32_t out[kPreDwords] = {};
   for (uint32_t i = 0; i < nq; ++i) {
     const uint32_t* r = qrec + size_t(rb + i) * kRecDwords;
     const uint32_t l = r[0] & 255u, d = (r[0] >> 8) & 255u;
@@ -144,9 +147,15 @@ __global__ void query_pre_kernel(const uint32_t* __restrict__ qhead, const uint3
       out[bit >> 5] |= 1u << (bit & 31);
     }
     for (int w = 0; w < 8; ++w) out[4 + w] |= r[1 + w];
+    out[12] = max(out[12], r[9] & 0xffffu);
+    out[13] = max(out[13], r[9] >> 16);
+    out[14] = max(out[14], r[10] & 0xffffu);
+    out[15] = max(out[15], r[10] >> 16);
   }
-  out[12] = nq > 1 ? 1u : 0u;
-  for (int w = 0; w < 16; ++w) qpre[size_t(q) * 16 + w] = out[w];
+  out[16] = nq > 1 ? 1u : 0u;
+  out[17] = rb;
+  out[18] = nq;
+  for (int w = 0; w < kPreDwords; ++w) qpre[size_t(q) * kPreDwords + w] = out[w];
 }
 
 // ------------------------------------------------------------------ screening
@@ -155,7 +164,7 @@ template <int K>
 __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
   __shared__ uint64_t adj[64];
   __shared__ uint8_t lbl[64];
-  __shared__ uint64_t dom[32];
+  __shared__ __align__(16) uint32_t gsum_lds[kPackGraphs * kSumDwords];
   extern __shared__ __align__(16) uint64_t stage[];  // kStageRecs records; Bloom scratch first
   const uint32_t lane = lane_id();
   const uint32_t pack = p.pack_begin + blockIdx.x;
@@ -189,12 +198,13 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
 
   // Graph segments: node lanes of a graph are contiguous.  mytop = last lane of the node's graph
   // (64 for lanes past the pack); graph-slot lanes j < ng hold the graph id and its top lane.
-  uint32_t mytop = 64, mygraph = 0, slot_top = 64, slot_gid = 0;
+  uint32_t mytop = 64, mygraph = 0, myslot = 0, slot_top = 64, slot_gid = 0;
   for (uint32_t j = 0; j < ng; ++j) {
     const uint32_t s = p.graph_offsets[gb + j] - n0, e = p.graph_offsets[gb + j + 1] - n0;
     if (lane >= s && lane < e) {
       mytop = e - 1;
       mygraph = gb + j;
+      myslot = j;
     }
     if (lane == j) {
       slot_gid = gb + j;
@@ -224,92 +234,109 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
     nstage = 0;
   };
 
-  // Pack summary for the query prefilter: presence bit (label l < 16, degree threshold t < 8)
-  // set when some node has label l and degree >= t; union of the path Bloom filters.
-  uint32_t pres[4] = {0, 0, 0, 0}, punion[8];
-  if (lane < nn && t.label < 16) {
-    const uint32_t top = min(t.deg, 7u), base = t.label * 8;
-    const uint64_t bitsl = (uint64_t(2) << top) - 1;  // thresholds 0..top
+  // Per-graph summaries for the query prefilter (LDS, kSumDwords per graph slot): presence bit
+  // (label l < 16, degree threshold t < 8) set when some node of the graph has label l and degree
+  // >= t; union of the nodes' path Bloom filters; largest value of each ring-count field.
+  uint32_t* gsum = gsum_lds;
+  for (uint32_t w = lane; w < uint32_t(kPackGraphs * kSumDwords); w += 64) gsum[w] = 0;
+  __syncthreads();
+  if (lane < nn) {
+    uint32_t* gs = gsum + myslot * kSumDwords;
+    if (t.label < 16) {
+      const uint32_t top = min(t.deg, 7u), base = t.label * 8;
+      const uint64_t bitsl = (uint64_t(2) << top) - 1;  // thresholds 0..top
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      const int lo = 32 * w;
-      if (base + 8 > uint32_t(lo) && base < uint32_t(lo + 32))
-        pres[w] = uint32_t(bitsl << (base - lo));
+      for (int w = 0; w < 4; ++w) {
+        const int lo = 32 * w;
+        if (base + 8 > uint32_t(lo) && base < uint32_t(lo + 32))
+          atomicOr(gs + w, uint32_t(bitsl << (base - lo)));
+      }
     }
+#pragma unroll
+    for (int w = 0; w < 8; ++w) atomicOr(gs + 4 + w, t.path[w]);
+    atomicMax(gs + 12, uint32_t(t.cycle) & 0xffffu);
+    atomicMax(gs + 13, uint32_t(t.cycle) >> 16);
+    atomicMax(gs + 14, uint32_t(t.cycle >> 32) & 0xffffu);
+    atomicMax(gs + 15, uint32_t(t.cycle >> 48));
   }
-#pragma unroll
-  for (int w = 0; w < 8; ++w) punion[w] = lane < nn ? t.path[w] : 0u;
-  for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-    for (int w = 0; w < 4; ++w) pres[w] |= __shfl_xor(pres[w], off);
-#pragma unroll
-    for (int w = 0; w < 8; ++w) punion[w] |= __shfl_xor(punion[w], off);
-  }
+  __syncthreads();
+  const bool root_lane = lane < nn && lane == mytop;
 
   for (uint32_t qb = 0; qb < p.Q; qb += 64) {
-    // 64 queries at once, one per lane: every node's (label, degree) present in the pack and
-    // every node's path bits inside the pack's union (both necessary for a candidate).
+    // 64 queries at once, one per lane, against every graph of the pack: every node's (label,
+    // degree) present in the graph, every node's path bits inside the graph's union and every
+    // ring field within the graph's largest (all necessary for a candidate of each node).
     const uint32_t qq = qb + lane;
-    bool pass = false;
+    uint32_t gpass = 0, q_rb = 0, q_nq = 0;
     if (qq < p.Q) {
-      const uint4* pr = reinterpret_cast<const uint4*>(p.qpre + size_t(qq) * 16);
-      const uint4 a = pr[0], b = pr[1], c = pr[2], e = pr[3];
-      pass = e.x != 0 && !((a.x & ~pres[0]) | (a.y & ~pres[1]) | (a.z & ~pres[2]) | (a.w & ~pres[3]) |
-                           (b.x & ~punion[0]) | (b.y & ~punion[1]) | (b.z & ~punion[2]) |
-                           (b.w & ~punion[3]) | (c.x & ~punion[4]) | (c.y & ~punion[5]) |
-                           (c.z & ~punion[6]) | (c.w & ~punion[7]));
+      const uint4* pr = reinterpret_cast<const uint4*>(p.qpre + size_t(qq) * kPreDwords);
+      const uint4 a = pr[0], b = pr[1], c = pr[2], rg = pr[3], e = pr[4];
+      q_rb = e.y;
+      q_nq = e.z;
+      if (e.x) {
+        for (uint32_t j = 0; j < ng; ++j) {
+          const uint4* gs = reinterpret_cast<const uint4*>(gsum + j * kSumDwords);
+          const uint4 ga = gs[0], gb4 = gs[1], gc = gs[2], gr = gs[3];
+          const uint32_t miss = (a.x & ~ga.x) | (a.y & ~ga.y) | (a.z & ~ga.z) | (a.w & ~ga.w) |
+                                (b.x & ~gb4.x) | (b.y & ~gb4.y) | (b.z & ~gb4.z) | (b.w & ~gb4.w) |
+                                (c.x & ~gc.x) | (c.y & ~gc.y) | (c.z & ~gc.z) | (c.w & ~gc.w);
+          const bool rings = rg.x <= gr.x && rg.y <= gr.y && rg.z <= gr.z && rg.w <= gr.w;
+          if (miss == 0 && rings) gpass |= 1u << j;
+        }
+      }
     }
-    uint64_t cand = ballot(pass);
+    uint64_t cand = ballot(gpass != 0);
     while (cand) {
-      const uint32_t qu = qb + uint32_t(ctz64(cand));
+      const uint32_t li = uint32_t(ctz64(cand));
       cand &= cand - 1;
-      const uint32_t rb = cload(p.qhead, 4 * qu), nq = cload(p.qhead, 4 * qu + 1);
-      uint64_t tops = T;
-      uint64_t lanes = node_mask;
-      for (uint32_t ti = 0; ti < nq && tops; ++ti) {
-        const uint32_t* r = p.qrec + size_t(rb + ti) * kRecDwords;
-        // one batch of scalar loads for label, degree, path and ring words
-        uint32_t rr[12];
+      const uint32_t qu = qb + li;
+      const uint32_t gm = __builtin_amdgcn_readlane(gpass, li);
+      const uint32_t rb = __builtin_amdgcn_readlane(q_rb, li);
+      const uint32_t nq = __builtin_amdgcn_readlane(q_nq, li);
+      const bool in_graph = lane < nn && ((gm >> myslot) & 1);
+      uint64_t lanes = ballot(in_graph);
+      uint64_t tops = ballot(in_graph && root_lane);
+      // domains by local query node id, lane u holding node u's (VGPRs)
+      uint32_t dom_lo = 0, dom_hi = 0;
+      const uint32_t* r = p.qrec + size_t(rb) * kRecDwords;
+      for (uint32_t ti = 0; ti < nq; ++ti, r += kRecDwords) {
+        // the whole record in one batch of scalar loads, then a branch-free test
+        uint32_t w[12 + 4 * K];
 #pragma unroll
-        for (int w = 0; w < 12; ++w) rr[w] = cload(r, w);
-        const uint32_t w0 = rr[0];
-        const uint32_t ql = w0 & 255u, qd = (w0 >> 8) & 255u, local = (w0 >> 16) & 255u;
-        const uint32_t lm = w0 >> 24;
-        uint64_t m = ballot((d_label == ql || ql == kWildcard) && d_deg >= qd) & lanes;
-        if (m) {
-          uint32_t acc = 0;
+        for (int x = 0; x < 12 + 4 * K; ++x) w[x] = cload(r, x);
+        const uint32_t ql = w[0] & 255u, qd = (w[0] >> 8) & 255u, local = (w[0] >> 16) & 255u;
+        uint32_t acc = 0;
 #pragma unroll
-          for (int w = 0; w < 8; ++w) acc |= rr[1 + w] & d_pnot[w];
-          m &= ballot(acc == 0);
-        }
-        if (m) {
-          const uint32_t c0 = d_cyc0 - rr[9], c1 = d_cyc1 - rr[10];
-          m &= ballot(((c0 & c1) & 0x80008000u) == 0x80008000u);
-        }
-        if (K > 0 && m && lm) {
-          uint32_t fail = 0;
+        for (int x = 0; x < 8; ++x) acc |= w[1 + x] & d_pnot[x];
+        const uint32_t c0 = d_cyc0 - w[9], c1 = d_cyc1 - w[10];
+        uint32_t fail = ((c0 & c1) & 0x80008000u) ^ 0x80008000u;  // a ring field below the query's
 #pragma unroll
-          for (int k = 0; k < K; ++k) {
-            if ((lm >> k) & 1) {
-              fail |= (cload(r, 12 + 4 * k) - d_hist[k][0]) | (cload(r, 13 + 4 * k) - d_hist[k][1]) |
-                      (cload(r, 14 + 4 * k) - d_hist[k][2]) | (cload(r, 15 + 4 * k) - d_hist[k][3]);
-            }
-          }
-          m &= ballot((fail & 0x10101010u) == 0);
+        for (int k = 0; k < K; ++k)
+          fail |= ((w[12 + 4 * k] - d_hist[k][0]) | (w[13 + 4 * k] - d_hist[k][1]) |
+                   (w[14 + 4 * k] - d_hist[k][2]) | (w[15 + 4 * k] - d_hist[k][3])) & 0x10101010u;
+        const bool ok = (d_label == ql || ql == kWildcard) && d_deg >= qd && (acc | fail) == 0;
+        const uint64_t m = ballot(ok) & lanes;
+        if (lane == local) {
+          dom_lo = uint32_t(m);
+          dom_hi = uint32_t(m >> 32);
         }
-        if (lane == 0) dom[local] = m;
         // Graphs left with a candidate: per segment, (m | top) - first has its top bit set
         // unless every non-top bit of m is clear, in which case m's own top bit decides.
         const uint64_t u = (m | T) - S;
         const uint64_t alive = ((u | m) & T) & tops;
         if (alive != tops) {
           tops = alive;
+          if (!tops) break;
           lanes = ballot(mytop < 64 && ((tops >> mytop) & 1)) & node_mask;
         }
       }
       if (!tops) continue;
-      __syncthreads();  // dom[] visible to all lanes
       if (lane < ng && slot_top < 64 && ((tops >> slot_top) & 1)) vbits |= uint64_t(1) << (qu & 63);
+      // domains in matching order: lane d takes the domain of the node matched at depth d
+      const uint32_t jl = lane < nq ? uint32_t(p.jorder[size_t(qu) * 32 + lane]) : 0u;
+      const uint64_t jdom = (uint64_t(uint32_t(__shfl(int(dom_hi), int(jl)))) << 32) |
+                            uint32_t(__shfl(int(dom_lo), int(jl)));
+      const uint64_t own = (uint64_t(dom_hi) << 32) | dom_lo;  // lane = local node id
       // GMCR pairs -> staged records {q | g << 32, n0 | nq << 32, domains in matching order}
       uint64_t at = tops;
       while (at) {
@@ -322,11 +349,11 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
           rec[0] = uint64_t(qu) | (uint64_t(g) << 32);
           rec[1] = uint64_t(n0) | (uint64_t(nq) << 32);
         }
-        if (lane < nq) rec[2 + lane] = dom[p.jorder[size_t(qu) * 32 + lane]] & seg;
+        if (lane < nq) rec[2 + lane] = jdom & seg;
         if (p.domsum) {
           if (lane < nq)
             atomicAdd(reinterpret_cast<unsigned long long*>(p.domsum + rb + lane),
-                      (unsigned long long)popc64(dom[lane] & seg));
+                      (unsigned long long)popc64(own & seg));
           if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(p.qpairs + qu), 1ull);
         }
         ++nstage;
@@ -686,7 +713,7 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
   p->num_qpacks = uint32_t(qpacks.size() - 1);
   if ((rc = device_alloc((void**)&p->d_qpacks, 4ull * qpacks.size())) ||
       (rc = device_alloc((void**)&p->d_qhead, 4ull * std::max<size_t>(qhead.size(), 4))) ||
-      (rc = device_alloc((void**)&p->d_qpre, 64ull * std::max<uint32_t>(Q, 1))) ||
+      (rc = device_alloc((void**)&p->d_qpre, 4ull * kPreDwords * std::max<uint32_t>(Q, 1))) ||
       (rc = device_alloc((void**)&p->d_qrec, 4ull * kRecDwords * std::max<uint64_t>(Qn, 1))) ||
       (rc = device_alloc((void**)&p->d_qpos, 4ull * qpos.size())) ||
       (rc = device_alloc((void**)&p->d_jorder, std::max<size_t>(size_t(Q) * 32, 32))) ||
