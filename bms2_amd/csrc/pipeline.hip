@@ -27,10 +27,14 @@
 namespace bms2 {
 
 constexpr int kPackGraphs = 8;   // graphs per pack
-constexpr int kStageRecs = 32;   // LDS-staged pair records per wave
+constexpr int kStageSmall = 96;  // LDS-staged small pair records per wave
+constexpr int kStageLarge = 16;  // LDS-staged large pair records per wave
+constexpr int kSegRecs = 32;     // small records per segment (a flush writes several)
 constexpr int kRecDwords = 48;   // screening record of one query node
 constexpr int kPreDwords = 20;   // prefilter record of one query
 constexpr int kSumDwords = 16;   // prefilter summary of one data graph
+constexpr int kSmallNq = 8;      // queries of up to 8 nodes use the small records and join
+constexpr int kSmallWords = 2 + kSmallNq;
 
 // Screening record layout (u32 words): [0] label | deg << 8 | local id << 16 | level mask << 24,
 // [1..8] path Bloom, [9..10] ring counts, [12 + 4k .. 15 + 4k] shell level k.  Shell words hold
@@ -48,20 +52,35 @@ struct ScreenParams {
   const uint32_t* qpre;     // [Q][kPreDwords] prefilter records
   const uint32_t* qrec;     // [Qn][kRecDwords] records in screening order per query
   const uint8_t* jorder;    // [Q][32] local query node matched at each DFS depth
+  const uint64_t* qadj8;    // [Q] byte d: earlier depths adjacent to depth d (nq <= kSmallNq)
   uint64_t* adj_out;        // [Dn] adjacency rows over pack lanes
   uint64_t* counts;         // [D][Q]
   uint64_t* valid;          // [D][VW]
   uint32_t VW;
-  uint32_t rec_words;       // u64 words per pair record: 2 + nq_cap
-  uint64_t* recs;
-  uint64_t rec_cap;
-  uint64_t* ctl;            // [0] records reserved, [1] overflow, [2] join cursor, [3..6] stats
+  uint64_t* recs_s;         // small-query records (nq <= kSmallNq), kSmallWords u64 each
+  uint64_t cap_s;
+  uint4* segs;              // one descriptor per flush of small records: {first, count, n0, nn}
+  uint64_t cap_seg;
+  uint64_t* recs_l;         // large-query records, rec_words_l u64 each
+  uint64_t cap_l;
+  uint32_t rec_words_l;     // 2 + nq_cap
+  uint64_t* ctl;            // kCtl* slots
   uint64_t* domsum;         // trace (nullable): [Qn] summed domain sizes over GMCR pairs
   uint64_t* qpairs;         // trace (nullable): [Q] GMCR pairs
 };
 
+// Control words of one chunk (u64).
+constexpr int kCtlSmall = 0;      // small records reserved (low 40 bits), segments (high 24)
+constexpr int kCtlOverflow = 1;   // a record or segment buffer overflowed
+constexpr int kCtlSegCursor = 2;  // small join: segments claimed
+constexpr int kCtlPairs = 3;      // stats: GMCR pairs, matched pairs, embeddings, DFS nodes
+constexpr int kCtlLarge = 7;      // large records reserved
+constexpr int kCtlLargeCursor = 8;
+constexpr int kSegShift = 40;
+constexpr int kCtlWords = 16;
+
 struct JoinParams {
-  const uint64_t* recs;
+  const uint64_t* recs;     // large-query records
   uint64_t rec_cap;
   uint32_t rec_words;
   uint32_t Q;
@@ -165,7 +184,7 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
   __shared__ uint64_t adj[64];
   __shared__ uint8_t lbl[64];
   __shared__ __align__(16) uint32_t gsum_lds[kPackGraphs * kSumDwords];
-  extern __shared__ __align__(16) uint64_t stage[];  // kStageRecs records; Bloom scratch first
+  extern __shared__ __align__(16) uint64_t stage[];  // staged records; Bloom scratch first
   const uint32_t lane = lane_id();
   const uint32_t pack = p.pack_begin + blockIdx.x;
   const uint32_t gb = p.pack_gbegin[pack], ge = p.pack_gbegin[pack + 1];
@@ -214,24 +233,52 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
   const uint64_t T = ballot(lane < nn && lane == mytop);  // top lane of every non-empty graph
   const uint64_t S = ((T << 1) | 1) & node_mask;           // first lane of every non-empty graph
 
-  const uint32_t RW = p.rec_words;
-  uint32_t nstage = 0;
+  // LDS staging of pair records: small records (queries of up to kSmallNq nodes) first, then
+  // large records; each flush appends them to HBM with one atomic per class, and the small
+  // records of a flush (all of this pack) get a segment descriptor for the small join.
+  const uint32_t RWL = p.rec_words_l;
+  uint64_t* stage_s = stage;
+  uint64_t* stage_l = stage + kStageSmall * kSmallWords;
+  uint32_t ns = 0, nl = 0;
   uint64_t vbits = 0, pairs = 0;
 
   auto flush = [&]() {
-    if (!nstage) return;
+    if (!(ns | nl)) return;
     __syncthreads();
-    uint64_t base = 0;
-    if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl), nstage);
-    base = uniform64(readlane64(base, 0));
-    if (base + nstage > p.rec_cap) {
-      if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(p.ctl + 1), 1ull);
-    } else {
-      uint64_t* dst = p.recs + base * RW;
-      for (uint32_t w = lane; w < nstage * RW; w += 64) dst[w] = stage[w];
+    if (ns) {
+      // one atomic reserves the records and their segments of up to kSegRecs records
+      const uint32_t nsg = (ns + kSegRecs - 1) / kSegRecs;
+      uint64_t old = 0;
+      if (lane == 0)
+        old = atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + kCtlSmall),
+                        (unsigned long long)(ns | (uint64_t(nsg) << kSegShift)));
+      old = uniform64(readlane64(old, 0));
+      const uint64_t base = old & low_mask(kSegShift), si = old >> kSegShift;
+      if (base + ns > p.cap_s || si + nsg > p.cap_seg) {
+        if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(p.ctl + kCtlOverflow), 1ull);
+      } else {
+        uint64_t* dst = p.recs_s + base * kSmallWords;
+        for (uint32_t w = lane; w < ns * kSmallWords; w += 64) dst[w] = stage_s[w];
+        if (lane < nsg)
+          p.segs[si + lane] = make_uint4(uint32_t(base) + lane * kSegRecs,
+                                         min(uint32_t(kSegRecs), ns - lane * kSegRecs), n0, nn);
+      }
+    }
+    if (nl) {
+      uint64_t base = 0;
+      if (lane == 0)
+        base = atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + kCtlLarge), (unsigned long long)nl);
+      base = uniform64(readlane64(base, 0));
+      if (base + nl > p.cap_l) {
+        if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(p.ctl + kCtlOverflow), 1ull);
+      } else {
+        uint64_t* dst = p.recs_l + base * RWL;
+        for (uint32_t w = lane; w < nl * RWL; w += 64) dst[w] = stage_l[w];
+      }
     }
     __syncthreads();
-    nstage = 0;
+    ns = 0;
+    nl = 0;
   };
 
   // Per-graph summaries for the query prefilter (LDS, kSumDwords per graph slot): presence bit
@@ -337,6 +384,7 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
       const uint64_t jdom = (uint64_t(uint32_t(__shfl(int(dom_hi), int(jl)))) << 32) |
                             uint32_t(__shfl(int(dom_lo), int(jl)));
       const uint64_t own = (uint64_t(dom_hi) << 32) | dom_lo;  // lane = local node id
+      const uint64_t qa8 = nq <= uint32_t(kSmallNq) ? p.qadj8[qu] : 0;
       // GMCR pairs -> staged records {q | g << 32, n0 | nq << 32, domains in matching order}
       uint64_t at = tops;
       while (at) {
@@ -344,10 +392,13 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
         at &= at - 1;
         const uint64_t seg = ballot(mytop == uint32_t(tb));
         const uint32_t g = __builtin_amdgcn_readlane(mygraph, tb);
-        uint64_t* rec = stage + size_t(nstage) * RW;
+        const bool small = nq <= uint32_t(kSmallNq);
+        uint64_t* rec = small ? stage_s + ns * kSmallWords : stage_l + nl * RWL;
         if (lane == 0) {
           rec[0] = uint64_t(qu) | (uint64_t(g) << 32);
-          rec[1] = uint64_t(n0) | (uint64_t(nq) << 32);
+          // small records carry the query adjacency by depth (its byte 0 is always 0) and nq;
+          // large records the pack's first node and nq
+          rec[1] = small ? (qa8 | nq) : (uint64_t(n0) | (uint64_t(nq) << 32));
         }
         if (lane < nq) rec[2 + lane] = jdom & seg;
         if (p.domsum) {
@@ -356,10 +407,13 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
                       (unsigned long long)popc64(own & seg));
           if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(p.qpairs + qu), 1ull);
         }
-        ++nstage;
+        if (small)
+          ++ns;
+        else
+          ++nl;
         ++pairs;
       }
-      if (nstage > uint32_t(kStageRecs - kPackGraphs)) flush();
+      if (ns > uint32_t(kStageSmall - kPackGraphs) || nl > uint32_t(kStageLarge - kPackGraphs)) flush();
     }
     // GMCR bit rows of the block, and count 0 for every pair of the block outside the GMCR
     // (one coalesced 512-byte store per graph)
@@ -372,14 +426,15 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
     vbits = 0;
   }
   flush();
-  if (lane == 0 && pairs) atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + 3), (unsigned long long)pairs);
+  if (lane == 0 && pairs) atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + kCtlPairs), (unsigned long long)pairs);
 }
 
 // ------------------------------------------------------------------ join
 
-// Depth-first join, one (query, graph) pair per lane (isomorphism.hpp:291-472 semantics with
-// standard backtracking); LDS stacks per depth: remaining candidates R, matched node F and its
-// neighbour row NF.
+// Depth-first join of the large-query records (more than kSmallNq nodes), one (query, graph)
+// pair per lane (isomorphism.hpp:291-472 semantics with standard backtracking); LDS stacks per
+// depth: remaining candidates R, matched node F and its neighbour row NF; domain words, query
+// adjacency and data adjacency rows read from global memory per step.
 template <int NQ>
 __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
   __shared__ uint64_t R[NQ][64];   // remaining candidates per depth
@@ -388,8 +443,8 @@ __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
   const uint32_t lane = lane_id();
   // after an overflow the screen skipped whole flushes, so records below ctl[0] may be stale;
   // the host redoes the chunk in smaller pieces
-  if (p.ctl[1]) return;
-  const uint64_t total = min(p.ctl[0], p.rec_cap);
+  if (p.ctl[kCtlOverflow]) return;
+  const uint64_t total = min(p.ctl[kCtlLarge], p.rec_cap);
   const uint32_t RW = p.rec_words;
   bool busy = false, exhausted = false;
   uint64_t rec = 0, used = 0, cnt = 0, nodes = 0;
@@ -406,7 +461,7 @@ __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
       if (local_next >= local_end && !exhausted) {
         uint64_t base = 0;
         if (lane == 0)
-          base = atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + 2), (unsigned long long)kClaim);
+          base = atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + kCtlLargeCursor), (unsigned long long)kClaim);
         base = uniform64(readlane64(base, 0));
         local_next = base;
         local_end = base >= total ? base : min(base + kClaim, total);
@@ -487,9 +542,173 @@ __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
     st_nodes += __shfl_xor(st_nodes, off);
   }
   if (lane == 0 && st_matched + st_emb + st_nodes) {
-    atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + 4), (unsigned long long)st_matched);
-    atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + 5), (unsigned long long)st_emb);
-    atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + 6), (unsigned long long)st_nodes);
+    atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + kCtlPairs + 1), (unsigned long long)st_matched);
+    atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + kCtlPairs + 2), (unsigned long long)st_emb);
+    atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + kCtlPairs + 3), (unsigned long long)st_nodes);
+  }
+}
+
+struct JoinSmallParams {
+  const uint64_t* recs;     // small-query records, kSmallWords u64 each
+  const uint4* segs;        // {first record, count, n0, nn} per flush
+  uint64_t cap_seg;
+  uint32_t Q;
+  const uint64_t* adj;      // [Dn] adjacency rows over pack lanes
+  uint64_t* counts;
+  uint64_t* ctl;
+  uint32_t find_first;
+  uint64_t* qnodes;         // trace (nullable)
+  uint32_t* pair_nodes;     // trace (nullable)
+};
+
+// Depth-first join of the small-query records (up to kSmallNq nodes), one (query, graph) pair
+// per lane (isomorphism.hpp:291-472 semantics with standard backtracking).  A wave claims whole
+// segments (the small records one screen flush wrote, all of one pack) into S slots, stages each
+// slot's pack adjacency rows in LDS and queues the segment's records in an LDS ring; idle lanes
+// take records from the ring once kBatch of them are idle, so the wave pays the record loads'
+// latency once per batch.  A record {q | g << 32, query adjacency by depth | nq, domains} is
+// read in one round of independent loads; its domain words go to LDS and its query adjacency to
+// one register, so a DFS step touches LDS only.  USE[s] counts the records of slot s not yet
+// finished; a slot takes a new segment when it reaches zero.
+template <int S>
+__global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
+  constexpr int NQ = kSmallNq;
+  constexpr uint32_t kRing = 256;  // >= S * (largest segment)
+  constexpr uint32_t kBatch = 16;  // idle lanes that trigger a handout
+  static_assert(S * kSegRecs <= int(kRing), "ring must hold every slot's segment");
+  __shared__ uint64_t R[NQ][64];   // remaining candidates per depth
+  __shared__ uint64_t NF[NQ][64];  // neighbour row of the node matched per depth
+  __shared__ uint64_t D[NQ][64];   // domain per depth (matching order)
+  __shared__ uint64_t ADJ[S][64];  // adjacency rows of each slot's pack
+  __shared__ uint32_t RING[kRing];  // queued record index | slot << 29
+  __shared__ uint32_t USE[S];
+  __shared__ uint8_t F[NQ][64];    // node matched per depth
+  const uint32_t lane = lane_id();
+  if (p.ctl[kCtlOverflow]) return;  // the host redoes the chunk
+  const uint64_t nseg = min(p.ctl[kCtlSmall] >> kSegShift, p.cap_seg);
+  if (lane < uint32_t(S)) USE[lane] = 0;
+  __syncthreads();
+  uint32_t qh = 0, qt = 0;  // ring head and tail (uniform)
+  bool exhausted = false, busy = false;
+  uint32_t slot = 0, q = 0, g = 0, nq = 0;
+  int depth = 0;
+  uint64_t used = 0, cnt = 0, nodes = 0, qa = 0;
+  uint64_t st_matched = 0, st_emb = 0, st_nodes = 0;
+  while (true) {
+    const uint64_t idle = ballot(!busy);
+    const uint32_t nidle = uint32_t(popc64(idle));
+    if (nidle >= kBatch || nidle == 64) {
+      if (!exhausted && qt - qh < nidle) {
+        // refill free slots (nothing queued or running) with new segments
+        bool loaded = false;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          if (!exhausted && qt - qh < nidle + 64 && uniform(USE[s]) == 0) {
+            uint64_t si = 0;
+            if (lane == 0)
+              si = atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + kCtlSegCursor), 1ull);
+            si = uniform64(readlane64(si, 0));
+            if (si >= nseg) {
+              exhausted = true;
+            } else {
+              const uint4 sd = p.segs[si];
+              const uint32_t first = uniform(sd.x), count = uniform(sd.y);
+              ADJ[s][lane] = lane < sd.w ? p.adj[sd.z + lane] : 0;
+              for (uint32_t e = lane; e < count; e += 64)
+                RING[(qt + e) % kRing] = (first + e) | (uint32_t(s) << 29);
+              if (lane == 0) USE[s] = count;
+              qt += count;
+              loaded = true;
+            }
+          }
+        }
+        if (loaded) __syncthreads();  // ADJ rows, ring entries and USE visible to every lane
+      }
+      const uint32_t take = min(qt - qh, nidle);
+      const uint32_t rank = uint32_t(popc64(idle & low_mask(lane)));
+      if (!busy && rank < take) {
+        busy = true;
+        const uint32_t ent = RING[(qh + rank) % kRing];
+        slot = ent >> 29;
+        const uint64_t* r = p.recs + uint64_t(ent & ((1u << 29) - 1)) * kSmallWords;
+        uint64_t w[kSmallWords];
+#pragma unroll
+        for (int x = 0; x < kSmallWords; ++x) w[x] = r[x];  // words past 2 + nq are unused
+        q = uint32_t(w[0]);
+        g = uint32_t(w[0] >> 32);
+        nq = uint32_t(w[1] & 0xffu);
+        qa = w[1];  // byte d >= 1: earlier depths adjacent to depth d
+#pragma unroll
+        for (int d = 0; d < NQ; ++d) D[d][lane] = w[2 + d];
+        R[0][lane] = w[2];
+        depth = 0;
+        used = 0;
+        cnt = 0;
+        nodes = 0;
+      }
+      qh += take;
+      if (!ballot(busy)) {
+        if (exhausted && qh == qt) break;
+        continue;
+      }
+    }
+    bool finished = false;
+    if (busy) {
+      const uint64_t Rd = R[depth][lane];
+      if (Rd == 0) {
+        if (depth == 0) {
+          finished = true;
+        } else {
+          --depth;
+          used &= ~(uint64_t(1) << F[depth][lane]);
+        }
+      } else {
+        const int v = ctz64(Rd);
+        R[depth][lane] = Rd & (Rd - 1);
+        F[depth][lane] = uint8_t(v);
+        const uint64_t nv = ADJ[slot][v];
+        NF[depth][lane] = nv;
+        ++nodes;
+        const int nd = depth + 1;
+        const uint64_t vbit = uint64_t(1) << v;
+        uint64_t Rn = D[nd][lane] & ~(used | vbit);
+        const uint32_t am = uint32_t(qa >> (8 * nd)) & 0xffu;
+        Rn &= ((am >> depth) & 1) ? nv : ~nv;
+        for (int i = 0; i < depth; ++i) {
+          const uint64_t a = NF[i][lane];
+          Rn &= ((am >> i) & 1) ? a : ~a;
+        }
+        if (uint32_t(nd) == nq - 1) {
+          cnt += uint64_t(popc64(Rn));
+          if (p.find_first && cnt) finished = true;
+        } else {
+          used |= vbit;
+          depth = nd;
+          R[nd][lane] = Rn;
+        }
+      }
+    }
+    if (finished) {
+      const uint64_t c = p.find_first ? (cnt ? 1 : 0) : cnt;
+      p.counts[uint64_t(g) * p.Q + q] = c;
+      if (p.qnodes) atomicAdd(reinterpret_cast<unsigned long long*>(p.qnodes + q), (unsigned long long)nodes);
+      if (p.pair_nodes) p.pair_nodes[uint64_t(g) * p.Q + q] = uint32_t(min(nodes, uint64_t(0xffffffffu)));
+      atomicSub(&USE[slot], 1u);
+      st_matched += c ? 1 : 0;
+      st_emb += c;
+      st_nodes += nodes;
+      busy = false;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    st_matched += __shfl_xor(st_matched, off);
+    st_emb += __shfl_xor(st_emb, off);
+    st_nodes += __shfl_xor(st_nodes, off);
+  }
+  if (lane == 0 && st_matched + st_emb + st_nodes) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + kCtlPairs + 1), (unsigned long long)st_matched);
+    atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + kCtlPairs + 2), (unsigned long long)st_emb);
+    atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + kCtlPairs + 3), (unsigned long long)st_nodes);
   }
 }
 
@@ -543,10 +762,12 @@ struct bms2_pipeline_s {
   bms2_queue q;
   bms2_device_csr data, query;
   bms2_pipeline_options opt;
-  uint32_t num_packs = 0, num_qpacks = 0, nq_cap = 0, rec_words = 0;
-  uint64_t rec_cap = 0;
-  double pairs_per_pack = 0.0;  // measured GMCR density, sizes the chunks
-  int join_grid = 0;
+  uint32_t num_packs = 0, num_qpacks = 0, nq_cap = 0, rec_words_l = 0;
+  uint32_t max_nq = 0;
+  uint64_t cap_s = 0, cap_seg = 0, cap_l = 0;
+  // measured records / segments per pack of each class, sizes the chunks
+  double dens_s = 0.0, dens_seg = 0.0, dens_l = 0.0;
+  int join_grid = 0, join_small_grid = 0;
   uint32_t* d_packs = nullptr;
   uint32_t* d_qpacks = nullptr;
   uint32_t* d_qhead = nullptr;
@@ -556,7 +777,10 @@ struct bms2_pipeline_s {
   uint8_t* d_jorder = nullptr;
   uint32_t* d_qadj = nullptr;
   uint64_t* d_adj = nullptr;
-  uint64_t* d_recs = nullptr;
+  uint64_t* d_qadj8 = nullptr;
+  uint64_t* d_recs_s = nullptr;
+  uint4* d_segs = nullptr;
+  uint64_t* d_recs_l = nullptr;
   uint64_t* d_ctl = nullptr;
   // host copy of the query batch (matching orders are derived from it)
   std::vector<uint32_t> qgo, qro, qci;
@@ -575,7 +799,10 @@ void pipeline_free(bms2_pipeline p) {
   (void)hipFree(p->d_jorder);
   (void)hipFree(p->d_qadj);
   (void)hipFree(p->d_adj);
-  (void)hipFree(p->d_recs);
+  (void)hipFree(p->d_qadj8);
+  (void)hipFree(p->d_recs_s);
+  (void)hipFree(p->d_segs);
+  (void)hipFree(p->d_recs_l);
   (void)hipFree(p->d_ctl);
   delete p;
 }
@@ -594,6 +821,7 @@ int upload_order(bms2_pipeline_s* p, const uint8_t* order) {
   if (!Q) return BMS2_OK;
   std::vector<uint8_t> jorder(size_t(Q) * 32, 0);
   std::vector<uint32_t> qadj(size_t(Q) * 32, 0);
+  std::vector<uint64_t> qadj8(Q, 0);
   for (uint32_t g = 0; g < Q; ++g) {
     const uint32_t b = p->qgo[g], n = p->qgo[g + 1] - b;
     uint8_t* o = jorder.data() + size_t(g) * 32;
@@ -622,11 +850,13 @@ int upload_order(bms2_pipeline_s* p, const uint8_t* order) {
           }
       }
       qadj[size_t(g) * 32 + d] = w;
+      if (n <= uint32_t(kSmallNq)) qadj8[g] |= uint64_t(w) << (8 * d);
     }
   }
   hipStream_t s = p->q->stream;
   BMS2_HIP(hipMemcpyAsync(p->d_jorder, jorder.data(), jorder.size(), hipMemcpyHostToDevice, s));
   BMS2_HIP(hipMemcpyAsync(p->d_qadj, qadj.data(), 4ull * qadj.size(), hipMemcpyHostToDevice, s));
+  BMS2_HIP(hipMemcpyAsync(p->d_qadj8, qadj8.data(), 8ull * qadj8.size(), hipMemcpyHostToDevice, s));
   BMS2_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
   return BMS2_OK;
 }
@@ -659,7 +889,8 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
   p->query = *query;
   p->opt = *opt;
   p->nq_cap = query->max_graph_nodes <= 16 ? 16 : 32;
-  p->rec_words = 2 + std::max<uint32_t>(query->max_graph_nodes, 2);
+  p->rec_words_l = 2 + std::max<uint32_t>(query->max_graph_nodes, 2);
+  p->max_nq = query->max_graph_nodes;
   hipStream_t s = q->stream;
   int rc = BMS2_OK;
   auto fail = [&](int code) {
@@ -675,7 +906,7 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
   p->num_packs = uint32_t(packs.size() - 1);
   if ((rc = device_alloc((void**)&p->d_packs, 4ull * packs.size())) ||
       (rc = device_alloc((void**)&p->d_adj, 8ull * std::max<uint64_t>(data->total_nodes, 1))) ||
-      (rc = device_alloc((void**)&p->d_ctl, 8 * 8)))
+      (rc = device_alloc((void**)&p->d_ctl, 8 * kCtlWords)))
     return fail(rc);
   BMS2_HIP(hipMemcpyAsync(p->d_packs, packs.data(), 4ull * packs.size(), hipMemcpyHostToDevice, s));
 
@@ -717,7 +948,8 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
       (rc = device_alloc((void**)&p->d_qrec, 4ull * kRecDwords * std::max<uint64_t>(Qn, 1))) ||
       (rc = device_alloc((void**)&p->d_qpos, 4ull * qpos.size())) ||
       (rc = device_alloc((void**)&p->d_jorder, std::max<size_t>(size_t(Q) * 32, 32))) ||
-      (rc = device_alloc((void**)&p->d_qadj, 4ull * std::max<size_t>(size_t(Q) * 32, 32))))
+      (rc = device_alloc((void**)&p->d_qadj, 4ull * std::max<size_t>(size_t(Q) * 32, 32))) ||
+      (rc = device_alloc((void**)&p->d_qadj8, 8ull * std::max<uint32_t>(Q, 1))))
     return fail(rc);
   BMS2_HIP(hipMemcpyAsync(p->d_qpacks, qpacks.data(), 4ull * qpacks.size(), hipMemcpyHostToDevice, s));
   BMS2_HIP(hipMemcpyAsync(p->d_qpos, qpos.data(), 4ull * qpos.size(), hipMemcpyHostToDevice, s));
@@ -728,16 +960,30 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
   p->qro = std::move(qro);
   p->qci = std::move(qci);
   if ((rc = upload_order(p, nullptr))) return fail(rc);
-  // pair records: worst case or the budget, whichever is smaller
-  const uint64_t worst = uint64_t(data->num_graphs) * Q;
-  const uint64_t rec_bytes = 8ull * p->rec_words;
-  p->rec_cap = std::max<uint64_t>(1, std::min<uint64_t>(worst, record_budget_bytes() / rec_bytes));
-  if ((rc = device_alloc((void**)&p->d_recs, p->rec_cap * rec_bytes))) return fail(rc);
+  // pair records of both classes and the small-record segments: worst case or a share of the
+  // budget, whichever is smaller (the chunks are sized to what a run measures)
+  uint32_t q_small = 0;
+  for (uint32_t g = 0; g < Q; ++g) q_small += (p->qgo[g + 1] - p->qgo[g]) <= uint32_t(kSmallNq);
+  const uint64_t budget = record_budget_bytes();
+  // large queries hold few GMCR pairs (a node of every large query needs a candidate)
+  const double share_l = q_small == Q ? 0.0 : (q_small == 0 ? 0.9 : 0.15);
+  const uint64_t worst_s = uint64_t(data->num_graphs) * q_small;
+  const uint64_t worst_l = uint64_t(data->num_graphs) * (Q - q_small);
+  // record indices travel in 29 bits through the small join's queue
+  p->cap_s = std::max<uint64_t>(1, std::min<uint64_t>({worst_s, uint64_t(0.95 * (1 - share_l) * budget) / (8 * kSmallWords), (1ull << 29) - 1}));
+  p->cap_seg = std::max<uint64_t>(1, std::min<uint64_t>({p->cap_s, uint64_t(0.05 * (1 - share_l) * budget) / 16,
+                                                          (uint64_t(1) << (64 - kSegShift)) - 1}));
+  p->cap_l = std::max<uint64_t>(1, std::min<uint64_t>(worst_l, uint64_t(share_l * budget) / (8ull * p->rec_words_l)));
+  if ((rc = device_alloc((void**)&p->d_recs_s, p->cap_s * 8 * kSmallWords)) ||
+      (rc = device_alloc((void**)&p->d_segs, p->cap_seg * sizeof(uint4))) ||
+      (rc = device_alloc((void**)&p->d_recs_l, p->cap_l * 8ull * p->rec_words_l)))
+    return fail(rc);
   int dev = 0, cus = 0;
   (void)hipGetDevice(&dev);
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
   p->join_grid = cus * (p->nq_cap <= 16 ? 12 : 6);
+  p->join_small_grid = cus * 8;  // 16.5 KB of LDS per wave: 9 waves per CU
   BMS2_HIP(hipStreamSynchronize(s));
   *out = p;
   return BMS2_OK;
@@ -781,20 +1027,25 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint64_t* d_counts, uint64_t* 
   sp.qpre = p->d_qpre;
   sp.qrec = p->d_qrec;
   sp.jorder = p->d_jorder;
+  sp.qadj8 = p->d_qadj8;
   sp.adj_out = p->d_adj;
   sp.counts = d_counts;
   sp.valid = d_valid;
   sp.VW = (Q + 63) / 64;
-  sp.rec_words = p->rec_words;
-  sp.recs = p->d_recs;
-  sp.rec_cap = p->rec_cap;
+  sp.recs_s = p->d_recs_s;
+  sp.cap_s = p->cap_s;
+  sp.segs = p->d_segs;
+  sp.cap_seg = p->cap_seg;
+  sp.recs_l = p->d_recs_l;
+  sp.cap_l = p->cap_l;
+  sp.rec_words_l = p->rec_words_l;
   sp.ctl = p->d_ctl;
   sp.domsum = tr.domsum;
   sp.qpairs = tr.qpairs;
   JoinParams jp{};
-  jp.recs = p->d_recs;
-  jp.rec_cap = p->rec_cap;
-  jp.rec_words = p->rec_words;
+  jp.recs = p->d_recs_l;
+  jp.rec_cap = p->cap_l;
+  jp.rec_words = p->rec_words_l;
   jp.Q = Q;
   jp.adj = p->d_adj;
   jp.qadj = p->d_qadj;
@@ -803,10 +1054,22 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint64_t* d_counts, uint64_t* 
   jp.find_first = p->opt.find_first ? 1 : 0;
   jp.qnodes = tr.qnodes;
   jp.pair_nodes = tr.pair_nodes;
+  JoinSmallParams js{};
+  js.recs = p->d_recs_s;
+  js.segs = p->d_segs;
+  js.cap_seg = p->cap_seg;
+  js.Q = Q;
+  js.adj = p->d_adj;
+  js.counts = d_counts;
+  js.ctl = p->d_ctl;
+  js.find_first = jp.find_first;
+  js.qnodes = tr.qnodes;
+  js.pair_nodes = tr.pair_nodes;
   QueryPrepParams qp{p->query.graph_offsets, p->query.row_offsets, p->query.column_indices,
                      p->query.node_labels, p->d_qpacks, p->d_qpos, p->d_qrec};
-  const size_t stage_bytes =
-      std::max<size_t>(size_t(kStageRecs) * p->rec_words * 8, 64 * 8 * sizeof(uint32_t));
+  const size_t stage_bytes = std::max<size_t>(
+      (size_t(kStageSmall) * kSmallWords + size_t(kStageLarge) * p->rec_words_l) * 8,
+      64 * 8 * sizeof(uint32_t));
   return dispatch_levels(p->opt.refinement_steps, [&](auto kc) -> int {
     constexpr int K = decltype(kc)::value;
     hipLaunchKernelGGL(query_prep_kernel<K>, dim3(p->num_qpacks), dim3(64), 0, s, qp);
@@ -817,40 +1080,51 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint64_t* d_counts, uint64_t* 
     if (int krc = ks.begin()) return krc;
     uint32_t p0 = 0;
     while (p0 < pack_end) {
-      // chunk of packs sized so the expected GMCR pairs fit the record buffer
+      // chunk of packs sized so the expected records of both classes and the segments fit
       uint64_t chunk = pack_end - p0;
-      if (p->pairs_per_pack > 0)
-        chunk = std::min<uint64_t>(chunk, std::max<uint64_t>(1, uint64_t(0.9 * p->rec_cap / p->pairs_per_pack)));
+      auto fit = [&](double dens, uint64_t cap) {
+        if (dens > 0) chunk = std::min<uint64_t>(chunk, std::max<uint64_t>(1, uint64_t(0.9 * cap / dens)));
+      };
+      fit(p->dens_s, p->cap_s);
+      fit(p->dens_seg, p->cap_seg);
+      fit(p->dens_l, p->cap_l);
       const uint32_t p1 = p0 + uint32_t(chunk);
-      hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_ctl, 8u);
+      hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_ctl, uint32_t(kCtlWords));
       sp.pack_begin = p0;
       hipLaunchKernelGGL(screen_kernel<K>, dim3(p1 - p0), dim3(64), stage_bytes, s, sp);
       BMS2_CHECK_LAUNCH("screen_kernel");
-      if (p->nq_cap <= 16)
-        hipLaunchKernelGGL(join_kernel<16>, dim3(p->join_grid), dim3(64), 0, s, jp);
-      else
-        hipLaunchKernelGGL(join_kernel<32>, dim3(p->join_grid), dim3(64), 0, s, jp);
-      BMS2_CHECK_LAUNCH("join_kernel");
-      uint64_t ctl[8];
+      hipLaunchKernelGGL(join_small_kernel<8>, dim3(p->join_small_grid), dim3(64), 0, s, js);
+      BMS2_CHECK_LAUNCH("join_small_kernel");
+      if (p->max_nq > uint32_t(kSmallNq)) {
+        if (p->nq_cap <= 16)
+          hipLaunchKernelGGL(join_kernel<16>, dim3(p->join_grid), dim3(64), 0, s, jp);
+        else
+          hipLaunchKernelGGL(join_kernel<32>, dim3(p->join_grid), dim3(64), 0, s, jp);
+        BMS2_CHECK_LAUNCH("join_kernel");
+      }
+      uint64_t ctl[kCtlWords];
       BMS2_HIP(hipMemcpyAsync(ctl, p->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, s));
       BMS2_HIP(hipStreamSynchronize(s));
-      const double density = double(ctl[0]) / double(p1 - p0);
-      if (ctl[1]) {
-        // record buffer overflowed: measure, shrink the chunk, redo it (results are rewritten)
+      const double np = double(p1 - p0);
+      const double ds = (ctl[kCtlSmall] & low_mask(kSegShift)) / np;
+      const double dg = (ctl[kCtlSmall] >> kSegShift) / np, dl = ctl[kCtlLarge] / np;
+      if (ctl[kCtlOverflow]) {
+        // a buffer overflowed: measure, shrink the chunk, redo it (results are rewritten)
         if (chunk == 1)
-          return set_error(BMS2_ERR_CAPACITY, "pair record buffer (%llu records) too small",
-                           (unsigned long long)p->rec_cap);
+          return set_error(BMS2_ERR_CAPACITY, "pair record buffers (%llu + %llu records) too small",
+                           (unsigned long long)p->cap_s, (unsigned long long)p->cap_l);
         if (tr.domsum || tr.qnodes)
           return set_error(BMS2_ERR_CAPACITY,
                            "pair record buffer overflowed during a trace; trace fewer graphs");
-        p->pairs_per_pack = std::max(p->pairs_per_pack * 2.0, density);
+        p->dens_s = std::max(p->dens_s * 2.0, ds);
+        p->dens_seg = std::max(p->dens_seg * 2.0, dg);
+        p->dens_l = std::max(p->dens_l * 2.0, dl);
         continue;
       }
-      p->pairs_per_pack = std::max(p->pairs_per_pack, density);
-      stats[0] += ctl[3];
-      stats[1] += ctl[4];
-      stats[2] += ctl[5];
-      stats[3] += ctl[6];
+      p->dens_s = std::max(p->dens_s, ds);
+      p->dens_seg = std::max(p->dens_seg, dg);
+      p->dens_l = std::max(p->dens_l, dl);
+      for (int k = 0; k < 4; ++k) stats[k] += ctl[kCtlPairs + k];
       p0 = p1;
     }
     return ks.end();

@@ -89,3 +89,22 @@ def test_fused_1m_shard(queue, molecules_1m_shard):
     assert np.array_equal(valid, rv)
     assert np.array_equal(counts, rc)
     assert np.array_equal(stats[:4], rs)
+
+
+@pytest.mark.parametrize("which", ["small", "large"])
+@pytest.mark.parametrize("find_first", [False, True])
+def test_fused_query_size_classes(queue, molecules_1m_shard, which, find_first):
+    """Queries of up to 8 nodes go to the small-record join, larger ones to the large-record
+    join; a batch of only one class exercises each join alone."""
+    data, queries = molecules_1m_shard
+    n = np.diff(np.asarray(queries.graph_offsets))
+    sel = np.flatnonzero(n <= 8 if which == "small" else n > 8)[:200]
+    qs = graphs.GraphBatch.from_graphs([queries.graph(int(i)) for i in sel])
+    sub = data.subset(0, 1500)
+    counts, valid, stats = api.run_pipeline(queue, qs, sub, 6, find_first)
+    rc, rv, rs = oracle.pipeline(qs, sub, 6, find_first)
+    assert np.array_equal(valid, rv)
+    assert np.array_equal(counts, rc)
+    assert int(stats[0]) == int(rs[0]) and int(stats[1]) == int(rs[1])
+    if not find_first:
+        assert np.array_equal(stats[:4], rs)
