@@ -27,7 +27,7 @@
 namespace bms2 {
 
 constexpr int kPackGraphs = 8;   // graphs per pack
-constexpr int kStageSmall = 96;  // LDS-staged small pair records per wave
+constexpr int kStageSmall = 64;  // LDS-staged small pair records per wave
 constexpr int kStageLarge = 16;  // LDS-staged large pair records per wave
 constexpr int kSegRecs = 32;     // small records per segment (a flush writes several)
 constexpr int kRecDwords = 48;   // screening record of one query node
@@ -351,18 +351,25 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
         uint32_t w[12 + 4 * K];
 #pragma unroll
         for (int x = 0; x < 12 + 4 * K; ++x) w[x] = cload(r, x);
+        // keep the record's loads in one batch ahead of the uniform branches below
+#pragma unroll
+        for (int x = 0; x < 12 + 4 * K; ++x) asm volatile("" ::"s"(w[x]));
         const uint32_t ql = w[0] & 255u, qd = (w[0] >> 8) & 255u, local = (w[0] >> 16) & 255u;
-        uint32_t acc = 0;
+        const uint32_t lm = w[0] >> 24;  // shell levels with a non-empty query histogram
+        uint64_t m = ballot((d_label == ql || ql == kWildcard) && d_deg >= qd) & lanes;
+        if (m) {
+          uint32_t acc = 0;
 #pragma unroll
-        for (int x = 0; x < 8; ++x) acc |= w[1 + x] & d_pnot[x];
-        const uint32_t c0 = d_cyc0 - w[9], c1 = d_cyc1 - w[10];
-        uint32_t fail = ((c0 & c1) & 0x80008000u) ^ 0x80008000u;  // a ring field below the query's
+          for (int x = 0; x < 8; ++x) acc |= w[1 + x] & d_pnot[x];
+          const uint32_t c0 = d_cyc0 - w[9], c1 = d_cyc1 - w[10];
+          uint32_t fail = ((c0 & c1) & 0x80008000u) ^ 0x80008000u;  // a ring field below the query's
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-          fail |= ((w[12 + 4 * k] - d_hist[k][0]) | (w[13 + 4 * k] - d_hist[k][1]) |
-                   (w[14 + 4 * k] - d_hist[k][2]) | (w[15 + 4 * k] - d_hist[k][3])) & 0x10101010u;
-        const bool ok = (d_label == ql || ql == kWildcard) && d_deg >= qd && (acc | fail) == 0;
-        const uint64_t m = ballot(ok) & lanes;
+          for (int k = 0; k < K; ++k)
+            if ((lm >> k) & 1)  // an empty query shell passes every data node
+              fail |= ((w[12 + 4 * k] - d_hist[k][0]) | (w[13 + 4 * k] - d_hist[k][1]) |
+                       (w[14 + 4 * k] - d_hist[k][2]) | (w[15 + 4 * k] - d_hist[k][3])) & 0x10101010u;
+          m &= ballot((acc | fail) == 0);
+        }
         if (lane == local) {
           dom_lo = uint32_t(m);
           dom_hi = uint32_t(m >> 32);
