@@ -190,6 +190,7 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
   const uint32_t gb = p.pack_gbegin[pack], ge = p.pack_gbegin[pack + 1];
   const uint32_t ng = ge - gb;
   const uint32_t n0 = p.graph_offsets[gb], nn = p.graph_offsets[ge] - n0;
+  if (nn > 64) return;  // a graph above 64 nodes: the staged path serves it
   const uint64_t node_mask = low_mask(nn);
 
   Tuple t;
@@ -725,17 +726,22 @@ __global__ void zero_kernel(uint64_t* p, uint32_t n) {
 
 // ------------------------------------------------------------------ host side
 
-// Greedy packing of consecutive graphs: at most 64 nodes and kPackGraphs graphs per pack.
+// Greedy packing of consecutive graphs: at most 64 nodes and kPackGraphs graphs per pack.  A
+// graph above 64 nodes gets a pack of its own, which the screen skips (the staged path serves it).
 std::vector<uint32_t> build_packs(const std::vector<uint32_t>& graph_offsets) {
   const uint32_t G = uint32_t(graph_offsets.size() - 1);
   std::vector<uint32_t> begins;
   uint32_t g = 0;
   while (g < G) {
     begins.push_back(g);
+    if (graph_offsets[g + 1] - graph_offsets[g] > 64) {
+      ++g;
+      continue;
+    }
     uint32_t nodes = 0, cnt = 0;
     while (g < G && cnt < uint32_t(kPackGraphs)) {
       const uint32_t n = graph_offsets[g + 1] - graph_offsets[g];
-      if (cnt > 0 && nodes + n > 64) break;
+      if (n > 64 || (cnt > 0 && nodes + n > 64)) break;
       nodes += n;
       ++cnt;
       ++g;
@@ -743,6 +749,59 @@ std::vector<uint32_t> build_packs(const std::vector<uint32_t>& graph_offsets) {
   }
   begins.push_back(G);
   return begins;
+}
+
+// Data graphs above 64 nodes (staged path): zero their count and GMCR rows.
+__global__ void big_zero_kernel(const uint32_t* __restrict__ big_gid, uint32_t nbig, uint32_t Q,
+                                uint32_t VW, uint32_t g_limit, uint64_t* __restrict__ counts,
+                                uint64_t* __restrict__ valid) {
+  const uint32_t b = blockIdx.x;
+  if (b >= nbig) return;
+  const uint32_t g = big_gid[b];
+  if (g >= g_limit) return;
+  for (uint32_t q = threadIdx.x; q < Q; q += blockDim.x) counts[uint64_t(g) * Q + q] = 0;
+  for (uint32_t w = threadIdx.x; w < VW; w += blockDim.x) valid[uint64_t(g) * VW + w] = 0;
+}
+
+// GMCR entries of the big graphs into the dense outputs, with their statistics
+// {GMCR pairs, matched pairs, embeddings, DFS nodes}.
+__global__ void big_scatter_kernel(const uint32_t* __restrict__ goff, const uint32_t* __restrict__ gidx,
+                                   uint64_t T, const uint32_t* __restrict__ big_gid, uint32_t nbig,
+                                   const uint64_t* __restrict__ pc, const uint64_t* __restrict__ pn,
+                                   uint32_t Q, uint32_t VW, uint32_t g_limit,
+                                   uint64_t* __restrict__ counts, uint64_t* __restrict__ valid,
+                                   uint64_t* __restrict__ stats) {
+  const uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  if (t < T) {
+    uint32_t lo = 0, hi = nbig + 1;  // big graph of entry t: last b with goff[b] <= t
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (t >= goff[mid]) lo = mid + 1; else hi = mid;
+    }
+    const uint32_t g = big_gid[lo - 1], q = gidx[t];
+    if (g < g_limit) {
+      counts[uint64_t(g) * Q + q] = pc[t];
+      atomicOr(reinterpret_cast<unsigned long long*>(valid + uint64_t(g) * VW + (q >> 6)),
+               (unsigned long long)(uint64_t(1) << (q & 63)));
+      s0 = 1;
+      s1 = pc[t] ? 1 : 0;
+      s2 = pc[t];
+      s3 = pn[t];
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off);
+    s1 += __shfl_xor(s1, off);
+    s2 += __shfl_xor(s2, off);
+    s3 += __shfl_xor(s3, off);
+  }
+  if (lane_id() == 0 && s0) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(stats + 0), (unsigned long long)s0);
+    atomicAdd(reinterpret_cast<unsigned long long*>(stats + 1), (unsigned long long)s1);
+    atomicAdd(reinterpret_cast<unsigned long long*>(stats + 2), (unsigned long long)s2);
+    atomicAdd(reinterpret_cast<unsigned long long*>(stats + 3), (unsigned long long)s3);
+  }
 }
 
 template <class F>
@@ -791,6 +850,19 @@ struct bms2_pipeline_s {
   uint64_t* d_ctl = nullptr;
   // host copy of the query batch (matching orders are derived from it)
   std::vector<uint32_t> qgo, qro, qci;
+  // data graphs above 64 nodes: their own compact batch through the staged kernels
+  uint32_t num_big = 0;
+  bms2_device_csr big{};
+  uint32_t* d_big_gid = nullptr;
+  bms2_signature big_sig = nullptr;
+  bms2_path_signature big_path = nullptr;
+  bms2_cycle_signature big_cyc = nullptr;
+  bms2_candidates big_cand = nullptr;
+  bms2_gmcr big_gmcr = nullptr;
+  uint64_t big_cap = 0;
+  uint64_t* d_big_pc = nullptr;     // per GMCR entry counts
+  uint64_t* d_big_pn = nullptr;     // per GMCR entry DFS nodes
+  uint64_t* d_big_stats = nullptr;  // [0..3] statistics, [4] join total
 };
 
 namespace {
@@ -811,6 +883,16 @@ void pipeline_free(bms2_pipeline p) {
   (void)hipFree(p->d_segs);
   (void)hipFree(p->d_recs_l);
   (void)hipFree(p->d_ctl);
+  (void)hipFree(p->d_big_gid);
+  (void)hipFree(p->d_big_pc);
+  (void)hipFree(p->d_big_pn);
+  (void)hipFree(p->d_big_stats);
+  if (p->big_sig) bms2_signature_destroy(p->big_sig);
+  if (p->big_path) bms2_path_signature_destroy(p->big_path);
+  if (p->big_cyc) bms2_cycle_signature_destroy(p->big_cyc);
+  if (p->big_cand) bms2_candidates_destroy(p->big_cand);
+  if (p->big_gmcr) bms2_gmcr_destroy(p->big_gmcr);
+  if (p->big.graph_offsets) bms2_device_csr_destroy(p->q, &p->big);
   delete p;
 }
 
@@ -819,6 +901,115 @@ uint64_t record_budget_bytes() {
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 1ull << 30;
   return std::min<uint64_t>(free_b / 4, 16ull << 30);
+}
+
+// Data graphs above 64 nodes: a compact device batch of them (node ids renumbered), their
+// original graph ids, and the staged objects that serve them on every run.
+int build_big_batch(bms2_pipeline_s* p, const std::vector<uint32_t>& go) {
+  const uint32_t G = uint32_t(go.size() - 1);
+  std::vector<uint32_t> gid;
+  for (uint32_t g = 0; g < G; ++g)
+    if (go[g + 1] - go[g] > 64) gid.push_back(g);
+  p->num_big = uint32_t(gid.size());
+  if (gid.empty()) return BMS2_OK;
+  const bms2_device_csr& d = p->data;
+  std::vector<uint32_t> ro(d.total_nodes + 1), ci(d.total_edges);
+  std::vector<uint8_t> lab(d.total_nodes);
+  BMS2_HIP(hipMemcpy(ro.data(), d.row_offsets, 4ull * ro.size(), hipMemcpyDeviceToHost));
+  if (!ci.empty()) BMS2_HIP(hipMemcpy(ci.data(), d.column_indices, 4ull * ci.size(), hipMemcpyDeviceToHost));
+  BMS2_HIP(hipMemcpy(lab.data(), d.node_labels, lab.size(), hipMemcpyDeviceToHost));
+  std::vector<uint32_t> bgo{0}, bro{0}, bci;
+  std::vector<uint8_t> blab, belab;
+  for (uint32_t g : gid) {
+    const uint32_t s = go[g], e = go[g + 1], nb = bgo.back();
+    for (uint32_t u = s; u < e; ++u) {
+      for (uint32_t k = ro[u]; k < ro[u + 1]; ++k) bci.push_back(ci[k] - s + nb);
+      bro.push_back(uint32_t(bci.size()));
+      blab.push_back(lab[u]);
+    }
+    bgo.push_back(nb + (e - s));
+  }
+  belab.assign(bci.size(), 0);  // edge labels are never read by matching (R5)
+  bms2_host_csr h{};
+  h.num_graphs = p->num_big;
+  h.total_nodes = bgo.back();
+  h.total_edges = bci.size();
+  h.graph_offsets = bgo.data();
+  h.row_offsets = bro.data();
+  h.column_indices = bci.data();
+  h.node_labels = blab.data();
+  h.edge_labels = belab.data();
+  if (int rc = bms2_device_csr_create(p->q, &h, &p->big)) return rc;
+  if (int rc = device_alloc((void**)&p->d_big_gid, 4ull * gid.size())) return rc;
+  if (int rc = device_alloc((void**)&p->d_big_stats, 8 * 8)) return rc;
+  BMS2_HIP(hipMemcpy(p->d_big_gid, gid.data(), 4ull * gid.size(), hipMemcpyHostToDevice));
+  const uint64_t dn = p->big.total_nodes, qn = p->query.total_nodes;
+  if (int rc = bms2_signature_create(p->q, dn, qn, &p->big_sig)) return rc;
+  if (int rc = bms2_path_signature_create(p->q, dn, qn, &p->big_path)) return rc;
+  if (int rc = bms2_cycle_signature_create(p->q, dn, qn, &p->big_cyc)) return rc;
+  if (int rc = bms2_candidates_create(p->q, qn, dn, &p->big_cand)) return rc;
+  if (int rc = bms2_gmcr_create(p->q, &p->big_gmcr)) return rc;
+  return BMS2_OK;
+}
+
+// Runs the stage sequence on the big graphs and writes their rows of the dense outputs (graphs
+// with id below g_limit); adds their statistics to stats.
+int run_big(bms2_pipeline_s* p, uint64_t* d_counts, uint64_t* d_valid, uint32_t g_limit,
+            uint64_t stats[8]) {
+  if (!p->num_big) return BMS2_OK;
+  bms2_queue q = p->q;
+  hipStream_t s = q->stream;
+  const bms2_device_csr* dq = &p->query;
+  const bms2_device_csr* db = &p->big;
+  if (int rc = staged_reset(p->big_sig, p->big_cand)) return rc;
+  if (int rc = bms2_signature_generate(p->big_sig, db, BMS2_SCOPE_DATA, nullptr)) return rc;
+  if (int rc = bms2_path_signature_generate(p->big_path, db, BMS2_SCOPE_DATA, nullptr)) return rc;
+  if (int rc = bms2_cycle_signature_generate(p->big_cyc, db, BMS2_SCOPE_DATA, nullptr)) return rc;
+  if (int rc = bms2_signature_generate(p->big_sig, dq, BMS2_SCOPE_QUERY, nullptr)) return rc;
+  if (int rc = bms2_path_signature_generate(p->big_path, dq, BMS2_SCOPE_QUERY, nullptr)) return rc;
+  if (int rc = bms2_cycle_signature_generate(p->big_cyc, dq, BMS2_SCOPE_QUERY, nullptr)) return rc;
+  if (int rc = bms2_filter_candidates(q, dq, db, p->big_sig, p->big_path, p->big_cyc, p->big_cand,
+                                      nullptr))
+    return rc;
+  for (uint32_t k = 1; k <= p->opt.refinement_steps; ++k) {
+    if (int rc = bms2_signature_refine(p->big_sig, db, k, BMS2_SCOPE_DATA, nullptr)) return rc;
+    if (int rc = bms2_signature_refine(p->big_sig, dq, k, BMS2_SCOPE_QUERY, nullptr)) return rc;
+    if (int rc = bms2_refine_candidates(q, dq, db, p->big_sig, p->big_cand, nullptr)) return rc;
+  }
+  if (int rc = bms2_gmcr_generate(p->big_gmcr, dq, db, p->big_cand, nullptr)) return rc;
+  const uint64_t T = bms2_gmcr_total(p->big_gmcr);
+  if (T > p->big_cap) {
+    BMS2_HIP(hipStreamSynchronize(s));
+    (void)hipFree(p->d_big_pc);
+    (void)hipFree(p->d_big_pn);
+    p->d_big_pc = p->d_big_pn = nullptr;
+    p->big_cap = 0;
+    if (int rc = device_alloc((void**)&p->d_big_pc, 8 * T)) return rc;
+    if (int rc = device_alloc((void**)&p->d_big_pn, 8 * T)) return rc;
+    p->big_cap = T;
+  }
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_big_stats, 8u);
+  if (int rc = staged_join(q, dq, db, p->big_cand, p->big_gmcr, int(p->opt.find_first),
+                           p->d_jorder, p->d_big_stats + 4, p->d_big_pc, p->d_big_pn))
+    return rc;
+  const uint32_t Q = p->query.num_graphs, VW = (Q + 63) / 64;
+  hipLaunchKernelGGL(big_zero_kernel, dim3(p->num_big), dim3(256), 0, s, p->d_big_gid, p->num_big,
+                     Q, VW, g_limit, d_counts, d_valid);
+  BMS2_CHECK_LAUNCH("big_zero_kernel");
+  if (T) {
+    const uint32_t* goff = nullptr;
+    const uint32_t* gidx = nullptr;
+    if (int rc = bms2_gmcr_device_ptrs(p->big_gmcr, &goff, &gidx)) return rc;
+    hipLaunchKernelGGL(big_scatter_kernel, dim3(uint32_t((T + 255) / 256)), dim3(256), 0, s, goff,
+                       gidx, T, p->d_big_gid, p->num_big, p->d_big_pc, p->d_big_pn, Q, VW, g_limit,
+                       d_counts, d_valid, p->d_big_stats);
+    BMS2_CHECK_LAUNCH("big_scatter_kernel");
+  }
+  uint64_t st[8];
+  BMS2_HIP(hipMemcpyAsync(st, p->d_big_stats, sizeof(st), hipMemcpyDeviceToHost, s));
+  BMS2_HIP(hipStreamSynchronize(s));
+  for (int k = 0; k < 4; ++k) stats[k] += st[k];
+  return BMS2_OK;
 }
 
 // Matching order per query (identity when order is null, the reference's, isomorphism.hpp:
@@ -878,11 +1069,9 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
   if (opt->refinement_steps > uint32_t(kMaxLevels))
     return set_error(BMS2_ERR_UNSUPPORTED, "refinement steps %u > %d", opt->refinement_steps,
                      kMaxLevels);
-  if (data->max_graph_nodes > 64)
-    return set_error(BMS2_ERR_UNSUPPORTED,
-                     "fused pipeline handles data graphs of up to 64 nodes (largest: %u); use the "
-                     "staged entry points",
-                     data->max_graph_nodes);
+  if (data->max_graph_nodes > BMS2_MAX_GRAPH_NODES)
+    return set_error(BMS2_ERR_UNSUPPORTED, "data graph of %u nodes (limit %u)",
+                     data->max_graph_nodes, BMS2_MAX_GRAPH_NODES);
   if (data->max_degree > 27 || query->max_degree > 27)
     return set_error(BMS2_ERR_UNSUPPORTED,
                      "node degree above 27: the reference's 128-entry ring-count stack could "
@@ -911,6 +1100,7 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
     return fail(set_error(BMS2_ERR_HIP, "copy of data graph offsets failed"));
   std::vector<uint32_t> packs = build_packs(go);
   p->num_packs = uint32_t(packs.size() - 1);
+  if (int brc = build_big_batch(p, go)) return fail(brc);
   if ((rc = device_alloc((void**)&p->d_packs, 4ull * packs.size())) ||
       (rc = device_alloc((void**)&p->d_adj, 8ull * std::max<uint64_t>(data->total_nodes, 1))) ||
       (rc = device_alloc((void**)&p->d_ctl, 8 * kCtlWords)))
@@ -1015,8 +1205,8 @@ struct Trace {
 
 // Screens and joins packs [0, pack_end) in chunks that fit the record buffer; stats (host)
 // accumulates {GMCR pairs, matched pairs, embeddings, DFS nodes}.
-int run_packs(bms2_pipeline p, uint32_t pack_end, uint64_t* d_counts, uint64_t* d_valid,
-              const Trace& tr, EventScope& ks, uint64_t stats[8]) {
+int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_counts,
+              uint64_t* d_valid, const Trace& tr, EventScope& ks, uint64_t stats[8]) {
   hipStream_t s = p->q->stream;
   const uint32_t Q = p->query.num_graphs;
   if (pack_end == 0 || Q == 0) {
@@ -1134,6 +1324,7 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint64_t* d_counts, uint64_t* 
       for (int k = 0; k < 4; ++k) stats[k] += ctl[kCtlPairs + k];
       p0 = p1;
     }
+    if (int brc = run_big(p, d_counts, d_valid, g_limit, stats)) return brc;
     return ks.end();
   });
 }
@@ -1150,7 +1341,8 @@ extern "C" int bms2_pipeline_run(bms2_pipeline p, uint64_t* d_counts, uint64_t* 
   if (int rc = es.begin()) return rc;
   EventScope ks{p->q, kernel_ev};
   uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (int rc = run_packs(p, p->num_packs, d_counts, d_valid, Trace{}, ks, stats)) return rc;
+  if (int rc = run_packs(p, p->num_packs, p->data.num_graphs, d_counts, d_valid, Trace{}, ks, stats))
+    return rc;
   if (d_stats) BMS2_HIP(hipMemcpyAsync(d_stats, stats, sizeof(stats), hipMemcpyHostToDevice, s));
   if (d_stats) BMS2_HIP(hipStreamSynchronize(s));
   return es.end();
@@ -1180,7 +1372,7 @@ extern "C" int bms2_pipeline_trace(bms2_pipeline p, uint32_t max_graphs, uint64_
   Trace tr{d_domsum, d_qpairs, d_qnodes, d_pair_nodes};
   EventScope ks{p->q, nullptr};
   uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (int rc = run_packs(p, pe, d_counts, d_valid, tr, ks, stats)) return rc;
+  if (int rc = run_packs(p, pe, packs[pe], d_counts, d_valid, tr, ks, stats)) return rc;
   BMS2_HIP(hipStreamSynchronize(p->q->stream));
   return BMS2_OK;
 }

@@ -46,4 +46,14 @@ struct EventScope {
 
 int device_alloc(void** p, uint64_t bytes);
 
+// Staged join (staged.hip) with an optional matching order (d_jorder [Q][32], null = identity)
+// and optional per-pair counts and DFS nodes; enqueued on q, no event.
+int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
+                bms2_candidates cand, bms2_gmcr gmcr, int find_first, const uint8_t* d_jorder,
+                uint64_t* d_total, uint64_t* d_pair_counts, uint64_t* d_pair_nodes);
+
+// Zeroes a Signature's label signatures and reachability state and a Candidates matrix (fill
+// kernels on the signature's queue), so the staged sequence can run again on the same objects.
+int staged_reset(bms2_signature sig, bms2_candidates cand);
+
 }  // namespace bms2

@@ -363,7 +363,9 @@ __global__ __launch_bounds__(256) void gmcr_fill_kernel(const uint64_t* __restri
 // ------------------------------------------------------------------ join
 
 // JoinCandidates3Optimized (isomorphism.hpp:309-468): one lane per GMCR pair, standard
-// backtracking over bitsets of the data graph's local ids (up to 4 words = 256 nodes).
+// backtracking over bitsets of the data graph's local ids (up to 4 words = 256 nodes).  jorder
+// (nullable, [Q][32]) gives the local query node matched at each depth (identity when null, the
+// reference's order); per_pair_nodes (nullable) receives the DFS nodes expanded per pair.
 template <int NW>
 __global__ __launch_bounds__(256) void join_kernel(
     const uint64_t* __restrict__ cand, uint64_t W, const uint32_t* __restrict__ qgo,
@@ -371,7 +373,8 @@ __global__ __launch_bounds__(256) void join_kernel(
     const uint32_t* __restrict__ dgo, uint32_t D, const uint32_t* __restrict__ dro,
     const uint32_t* __restrict__ dci, const uint32_t* __restrict__ goff,
     const uint32_t* __restrict__ gidx, uint64_t T, int find_first, uint64_t* __restrict__ total,
-    uint64_t* __restrict__ per_pair) {
+    uint64_t* __restrict__ per_pair, const uint8_t* __restrict__ jorder,
+    uint64_t* __restrict__ per_pair_nodes) {
   const uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   uint64_t c = 0;
   if (t < T) {
@@ -386,12 +389,15 @@ __global__ __launch_bounds__(256) void join_kernel(
     const uint32_t s = dgo[g], e = dgo[g + 1], n = e - s;
     const uint32_t qb = qgo[q], nq = qgo[q + 1] - qb;
     uint64_t dom[32][NW];
-    uint32_t qadj[32];
+    uint32_t qadj[32];   // bit j of qadj[i]: depths j and i adjacent in the query
+    uint32_t pos[32];    // depth of each local query node
+    for (uint32_t i = 0; i < nq; ++i) pos[jorder ? jorder[size_t(q) * 32 + i] : i] = i;
     for (uint32_t i = 0; i < nq; ++i) {
-      const uint64_t* row = cand + uint64_t(qb + i) * W;
+      const uint32_t u = jorder ? jorder[size_t(q) * 32 + i] : i;
+      const uint64_t* row = cand + uint64_t(qb + u) * W;
       for (int k = 0; k < NW; ++k) dom[i][k] = row_bits(row, s, e, k);
       uint32_t a = 0;
-      for (uint32_t k = qro[qb + i]; k < qro[qb + i + 1]; ++k) a |= 1u << (qci[k] - qb);
+      for (uint32_t k = qro[qb + u]; k < qro[qb + u + 1]; ++k) a |= 1u << pos[qci[k] - qb];
       qadj[i] = a;
     }
     auto nrow = [&](uint32_t x, uint64_t* out) {
@@ -411,6 +417,7 @@ __global__ __launch_bounds__(256) void join_kernel(
     }
     int depth = 0;
     bool stop = false;
+    uint64_t nodes = 0;
     while (!stop) {
       int k0 = -1;
       for (int k = 0; k < NW; ++k)
@@ -427,6 +434,7 @@ __global__ __launch_bounds__(256) void join_kernel(
       const uint32_t v = uint32_t(k0 * 64 + ctz64(R[depth][k0]));
       R[depth][k0] &= R[depth][k0] - 1;
       f[depth] = v;
+      ++nodes;
       const int nd = depth + 1;
       uint64_t Rn[NW];
       for (int k = 0; k < NW; ++k) Rn[k] = dom[nd][k] & ~used[k];
@@ -448,6 +456,7 @@ __global__ __launch_bounds__(256) void join_kernel(
     }
     if (find_first) c = c ? 1 : 0;
     if (per_pair) per_pair[t] = c;
+    if (per_pair_nodes) per_pair_nodes[t] = nodes;
   }
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
   if (lane_id() == 0 && c) atomicAdd(reinterpret_cast<unsigned long long*>(total), (unsigned long long)c);
@@ -514,6 +523,55 @@ int check_graph(const bms2_device_csr* g, uint64_t nodes) {
 }
 
 }  // namespace
+
+namespace bms2 {
+
+__global__ void fill_zero_kernel(uint64_t* __restrict__ p, uint64_t n) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * blockDim.x)
+    p[i] = 0;
+}
+
+int staged_reset(bms2_signature sig, bms2_candidates cand) {
+  hipStream_t st = sig->q->stream;
+  for (int sc = 0; sc < 2; ++sc) {
+    if (sig->n[sc]) {
+      hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(sig->n[sc], 256, 4096)), dim3(256), 0, st,
+                         sig->reach[sc], 4 * sig->n[sc]);
+      hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(sig->n[sc], 256, 4096)), dim3(256), 0, st,
+                         sig->sig[sc], sig->n[sc]);
+    }
+  }
+  const uint64_t words = cand->rows * cand->W;
+  if (words)
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(words, 256, 1u << 16)), dim3(256), 0, st,
+                       cand->bits, words);
+  BMS2_CHECK_LAUNCH("fill_zero_kernel");
+  return BMS2_OK;
+}
+
+int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
+                bms2_candidates cand, bms2_gmcr gmcr, int find_first, const uint8_t* d_jorder,
+                uint64_t* d_total, uint64_t* d_pair_counts, uint64_t* d_pair_nodes) {
+  const uint64_t T = gmcr->total;
+  if (T) {
+    auto args = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(grid_for(T, 256, 1u << 30)), dim3(256), 0, q->stream, cand->bits,
+                         cand->W, query->graph_offsets, query->row_offsets, query->column_indices,
+                         data->graph_offsets, data->num_graphs, data->row_offsets,
+                         data->column_indices, gmcr->offsets, gmcr->indices, T, find_first,
+                         d_total, d_pair_counts, d_jorder, d_pair_nodes);
+    };
+    if (data->max_graph_nodes <= 64)
+      args(join_kernel<1>);
+    else
+      args(join_kernel<4>);
+    BMS2_CHECK_LAUNCH("join_kernel");
+  }
+  return BMS2_OK;
+}
+
+}  // namespace bms2
 
 extern "C" {
 
@@ -936,21 +994,9 @@ int bms2_join_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_
                      query->max_graph_nodes, BMS2_MAX_QUERY_NODES);
   EventScope es{q, ev};
   if (int rc = es.begin()) return rc;
-  const uint64_t T = gmcr->total;
-  if (T) {
-    auto args = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(grid_for(T, 256, 1u << 30)), dim3(256), 0, q->stream, cand->bits,
-                         cand->W, query->graph_offsets, query->row_offsets, query->column_indices,
-                         data->graph_offsets, data->num_graphs, data->row_offsets,
-                         data->column_indices, gmcr->offsets, gmcr->indices, T, find_first,
-                         d_num_matches, d_pair_counts);
-    };
-    if (data->max_graph_nodes <= 64)
-      args(join_kernel<1>);
-    else
-      args(join_kernel<4>);
-  }
-  BMS2_CHECK_LAUNCH("join_kernel");
+  if (int rc = staged_join(q, query, data, cand, gmcr, find_first, nullptr, d_num_matches,
+                           d_pair_counts, nullptr))
+    return rc;
   return es.end();
 }
 

@@ -258,8 +258,10 @@ int bms2_pipeline_run(bms2_pipeline p, uint64_t* d_counts, uint64_t* d_valid, ui
 /* Matching order of the join, per query: order[q*32 + d] = local query node matched at DFS
  * depth d (a permutation of 0..n_q-1 per query; NULL restores identity, the reference's order,
  * isomorphism.hpp:371-447).  Counts do not depend on the order (standard backtracking); only the
- * number of DFS nodes does.  The reference's join fixes identity order; defineMatchingOrder
- * (isomorphism.hpp:476-502) is its unused alternative. */
+ * number of DFS nodes does, and an order that matches a node before any of its neighbours leaves
+ * that depth unconstrained, which on large queries can grow the search combinatorially.  The
+ * reference's join fixes identity order; defineMatchingOrder (isomorphism.hpp:476-502) is its
+ * unused alternative. */
 int bms2_pipeline_set_order(bms2_pipeline p, const uint8_t* order);
 
 /* Join traces over the data graphs of the leading packs (at least max_graphs graphs, fewer when

@@ -108,3 +108,52 @@ def test_fused_query_size_classes(queue, molecules_1m_shard, which, find_first):
     assert int(stats[0]) == int(rs[0]) and int(stats[1]) == int(rs[1])
     if not find_first:
         assert np.array_equal(stats[:4], rs)
+
+
+@pytest.fixture(scope="module")
+def uneven_2k():
+    """2,000 graphs of the uneven generator (lognormal sizes up to 250 nodes, 10% fused ring
+    systems; 55 graphs above 64 nodes) with its 1,024 queries of 3-32 nodes (seed 0)."""
+    return graphs.generate("uneven-1m", seed=0, num_data_graphs=2000)
+
+
+@pytest.mark.parametrize("K", [0, 2, 6])
+@pytest.mark.parametrize("find_first", [False, True])
+def test_fused_uneven_sizes(queue, uneven_2k, K, find_first):
+    """Graphs above 64 nodes take the staged kernels inside bms2_pipeline_run, queries above 16
+    nodes the 32-deep join; all counts, GMCR rows and statistics equal the oracle's."""
+    data, queries = uneven_2k
+    assert np.diff(np.asarray(data.graph_offsets)).max() > 64
+    assert np.diff(np.asarray(queries.graph_offsets)).max() > 16
+    counts, valid, stats = api.run_pipeline(queue, queries, data, K, find_first)
+    rc, rv, rs = oracle.pipeline(queries, data, K, find_first)
+    assert np.array_equal(valid, rv)
+    assert np.array_equal(counts, rc)
+    assert np.array_equal(stats[:4], rs)
+
+
+def test_fused_uneven_order_and_trace(queue, uneven_2k):
+    data, queries = uneven_2k
+    dq = api.DeviceBatchedCSRGraph(queue, queries)
+    dd = api.DeviceBatchedCSRGraph(queue, data)
+    pipe = api.Pipeline(queue, dq, dd, refinement_steps=6)
+    rc, rv, _ = oracle.pipeline(queries, data, 6, False)
+    # connected random orders (random scores through the planner's connected ordering): a
+    # disconnected order leaves early depths unconstrained and the search of a 32-node query on
+    # a 250-node graph grows combinatorially
+    from bms2_amd import learned
+    rng = np.random.default_rng(3)
+    order = np.zeros((queries.num_graphs, 32), np.uint8)
+    for q, qi in enumerate(learned.query_infos(queries)):
+        order[q, :qi.n] = learned.score_order(qi, rng.normal(size=qi.n))
+    pipe.set_order(order)
+    counts, valid, stats = pipe.alloc_outputs()
+    for _ in range(2):  # the staged objects are reused across runs
+        pipe.run(counts, valid, stats)
+        queue.wait()
+        c, v = pipe.results(counts, valid)
+        assert np.array_equal(c, rc) and np.array_equal(v, rv)
+    # a trace over a prefix holding big graphs writes exactly the prefix's counts
+    tr = pipe.trace(700, queries.total_nodes)
+    g = tr["graphs"]
+    assert np.array_equal(tr["counts"], rc[:g])
