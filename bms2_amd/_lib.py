@@ -80,7 +80,8 @@ class PipelineOptions(C.Structure):
         ("find_first", C.c_uint32),
         ("block_waves", C.c_uint32),
         ("join_waves", C.c_uint32),
-        ("reserved", C.c_uint32 * 4),
+        ("no_graph", C.c_uint32),
+        ("reserved", C.c_uint32 * 3),
     ]
 
 

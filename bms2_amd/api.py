@@ -364,7 +364,8 @@ class Pipeline(_Handle):
 
     _destroy = "bms2_pipeline_destroy"
 
-    def __init__(self, q: Queue, query, data, refinement_steps: int = 0, find_first: bool = False):
+    def __init__(self, q: Queue, query, data, refinement_steps: int = 0, find_first: bool = False,
+                 use_graph: bool = True):
         self.q = q
         self.Q = query.num_graphs
         self.D = data.num_graphs
@@ -373,6 +374,7 @@ class Pipeline(_Handle):
         o = _lib.PipelineOptions()
         o.refinement_steps = refinement_steps
         o.find_first = int(find_first)
+        o.no_graph = 0 if use_graph else 1
         h = C.c_void_p()
         check(_L().bms2_pipeline_create(q.h, C.byref(query.c), C.byref(data.c), C.byref(o),
                                         C.byref(h)))

@@ -863,6 +863,11 @@ struct bms2_pipeline_s {
   uint64_t* d_big_pc = nullptr;     // per GMCR entry counts
   uint64_t* d_big_pn = nullptr;     // per GMCR entry DFS nodes
   uint64_t* d_big_stats = nullptr;  // [0..3] statistics, [4] join total
+  // the single-chunk kernels captured in a hipGraph, valid for these outputs and K
+  hipGraphExec_t graph = nullptr;
+  uint64_t* graph_counts = nullptr;
+  uint64_t* graph_valid = nullptr;
+  uint32_t graph_packs = 0;
 };
 
 namespace {
@@ -883,6 +888,7 @@ void pipeline_free(bms2_pipeline p) {
   (void)hipFree(p->d_segs);
   (void)hipFree(p->d_recs_l);
   (void)hipFree(p->d_ctl);
+  if (p->graph) (void)hipGraphExecDestroy(p->graph);
   (void)hipFree(p->d_big_gid);
   (void)hipFree(p->d_big_pc);
   (void)hipFree(p->d_big_pn);
@@ -1275,17 +1281,8 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
                        p->d_qrec, Q, p->d_qpre);
     BMS2_CHECK_LAUNCH("query_pre_kernel");
     if (int krc = ks.begin()) return krc;
-    uint32_t p0 = 0;
-    while (p0 < pack_end) {
-      // chunk of packs sized so the expected records of both classes and the segments fit
-      uint64_t chunk = pack_end - p0;
-      auto fit = [&](double dens, uint64_t cap) {
-        if (dens > 0) chunk = std::min<uint64_t>(chunk, std::max<uint64_t>(1, uint64_t(0.9 * cap / dens)));
-      };
-      fit(p->dens_s, p->cap_s);
-      fit(p->dens_seg, p->cap_seg);
-      fit(p->dens_l, p->cap_l);
-      const uint32_t p1 = p0 + uint32_t(chunk);
+    // kernels of one chunk of packs [p0, p1)
+    auto launch_chunk = [&](uint32_t p0, uint32_t p1) -> int {
       hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_ctl, uint32_t(kCtlWords));
       sp.pack_begin = p0;
       hipLaunchKernelGGL(screen_kernel<K>, dim3(p1 - p0), dim3(64), stage_bytes, s, sp);
@@ -1298,6 +1295,45 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
         else
           hipLaunchKernelGGL(join_kernel<32>, dim3(p->join_grid), dim3(64), 0, s, jp);
         BMS2_CHECK_LAUNCH("join_kernel");
+      }
+      return BMS2_OK;
+    };
+    const bool tracing = tr.domsum || tr.qnodes || tr.pair_nodes;
+    uint32_t p0 = 0;
+    while (p0 < pack_end) {
+      // chunk of packs sized so the expected records of both classes and the segments fit
+      uint64_t chunk = pack_end - p0;
+      auto fit = [&](double dens, uint64_t cap) {
+        if (dens > 0) chunk = std::min<uint64_t>(chunk, std::max<uint64_t>(1, uint64_t(0.9 * cap / dens)));
+      };
+      fit(p->dens_s, p->cap_s);
+      fit(p->dens_seg, p->cap_seg);
+      fit(p->dens_l, p->cap_l);
+      const uint32_t p1 = p0 + uint32_t(chunk);
+      // a measured batch that fits one chunk replays its captured kernels
+      const bool graphable = !p->opt.no_graph && !tracing && p0 == 0 && p1 == pack_end &&
+                             p->dens_s + p->dens_l > 0;
+      if (graphable && !(p->graph && p->graph_counts == d_counts && p->graph_valid == d_valid &&
+                         p->graph_packs == pack_end)) {
+        if (p->graph) (void)hipGraphExecDestroy(p->graph);
+        p->graph = nullptr;
+        hipGraph_t g = nullptr;
+        BMS2_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int lrc = launch_chunk(p0, p1);
+        const hipError_t erc = hipStreamEndCapture(s, &g);
+        if (lrc) return lrc;
+        BMS2_HIP(erc);
+        const hipError_t irc = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        BMS2_HIP(irc);
+        p->graph_counts = d_counts;
+        p->graph_valid = d_valid;
+        p->graph_packs = pack_end;
+      }
+      if (graphable) {
+        BMS2_HIP(hipGraphLaunch(p->graph, s));
+      } else if (int lrc = launch_chunk(p0, p1)) {
+        return lrc;
       }
       uint64_t ctl[kCtlWords];
       BMS2_HIP(hipMemcpyAsync(ctl, p->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, s));
@@ -1316,6 +1352,10 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
         p->dens_s = std::max(p->dens_s * 2.0, ds);
         p->dens_seg = std::max(p->dens_seg * 2.0, dg);
         p->dens_l = std::max(p->dens_l * 2.0, dl);
+        if (graphable) {
+          (void)hipGraphExecDestroy(p->graph);
+          p->graph = nullptr;
+        }
         continue;
       }
       p->dens_s = std::max(p->dens_s, ds);

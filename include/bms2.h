@@ -238,16 +238,20 @@ int bms2_join_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_
 typedef struct bms2_pipeline_options {
   uint32_t refinement_steps; /* K (BMS2 -i) */
   uint32_t find_first;       /* 1: first-match, 0: count-all (BMS2 --find-all) */
-  uint32_t block_waves;      /* waves per workgroup of the fused filter/join kernel; 0 = tuned */
-  uint32_t join_waves;       /* 0 = tuned */
-  uint32_t reserved[4];
+  uint32_t block_waves;      /* waves per CU of the small-query join; 0 = tuned */
+  uint32_t join_waves;       /* waves per CU of the large-query join; 0 = tuned */
+  uint32_t no_graph;         /* 1: never capture the chunk kernels in a hipGraph */
+  uint32_t reserved[3];
 } bms2_pipeline_options;
 
 typedef struct bms2_pipeline_s* bms2_pipeline;
 int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
                          const bms2_pipeline_options* opt, bms2_pipeline* out);
 int bms2_pipeline_destroy(bms2_pipeline p);
-/* Runs the whole sequence.  d_counts: u64[num_data_graphs * num_query_graphs], data-graph major
+/* Runs the whole sequence.  Once a run has measured how many pair records the batch produces
+ * and they fit one chunk, the chunk's kernels are captured in a hipGraph (per output buffers
+ * and K) and later runs replay it; a replay that overflows drops the graph and redoes the run.
+ * d_counts: u64[num_data_graphs * num_query_graphs], data-graph major
  * (pair (g, q) at g*Q + q), 0 for pairs outside the GMCR.  d_valid: u64 bit rows
  * [num_data_graphs][ceil(Q/64)], bit q set when (q, g) is a GMCR pair.  d_stats (nullable)
  * u64[8]: {GMCR pairs, matched pairs, total embeddings, DFS nodes expanded, 0...}.  ev brackets

@@ -157,3 +157,33 @@ def test_fused_uneven_order_and_trace(queue, uneven_2k):
     tr = pipe.trace(700, queries.total_nodes)
     g = tr["graphs"]
     assert np.array_equal(tr["counts"], rc[:g])
+
+
+@pytest.mark.parametrize("workload", ["molecules", "uneven"])
+def test_graph_replays_match_eager(queue, molecules_1m_shard, uneven_2k, workload):
+    """From the second run on, the chunk kernels replay from a hipGraph; every replay, with
+    eager work (traces, order changes, another pipeline) between replays, equals the oracle and
+    the pipeline that never captures."""
+    data, queries = molecules_1m_shard if workload == "molecules" else uneven_2k
+    rc, rv, rs = oracle.pipeline(queries, data, 6, False)
+    dq = api.DeviceBatchedCSRGraph(queue, queries)
+    dd = api.DeviceBatchedCSRGraph(queue, data)
+    eager = api.Pipeline(queue, dq, dd, refinement_steps=6, use_graph=False)
+    ec, ev, es = eager.alloc_outputs()
+    eager.run(ec, ev, es)
+    queue.wait()
+    assert np.array_equal(eager.results(ec, ev)[0], rc)
+    pipe = api.Pipeline(queue, dq, dd, refinement_steps=6)
+    counts, valid, stats = pipe.alloc_outputs()
+    for i in range(4):
+        counts.zero()
+        pipe.run(counts, valid, stats)
+        queue.wait()
+        c, v = pipe.results(counts, valid)
+        assert np.array_equal(c, rc), f"run {i}"
+        assert np.array_equal(v, rv), f"run {i}"
+        assert np.array_equal(stats.to_numpy(np.uint64, 8)[:4], rs), f"run {i}"
+        # eager work between replays
+        pipe.trace(300, queries.total_nodes)
+        eager.run(ec, ev, es)
+        queue.wait()
