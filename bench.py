@@ -68,6 +68,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="time the plain leg only")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--skip-plain", action="store_true",
                     help="time the learned leg only (profiling runs)")
     ap.add_argument("--plan-graphs", type=int, default=16384,
@@ -117,6 +118,12 @@ def main() -> int:
               f"{int(st[3])}", file=sys.stderr, flush=True)
         return t, kms, st
 
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tune = pipe.autotune(counts, valid, runs=1)
+        tune["seconds"] = round(time.perf_counter() - t0, 2)
+        print(f"[bench] autotuned launch shape: {tune}", file=sys.stderr, flush=True)
     order = "identity"
     S = min(args.cpu_sample_graphs or D, D)
     plain_sample = None
@@ -203,6 +210,7 @@ def main() -> int:
             "kernel_ms_avg": round(float(np.mean(kms_plain)), 3), "dfs_nodes": int(st_plain[3])},
         "learned_vs_plain": None if plain_value is None else round(value / plain_value, 4),
         "learned_plan": plan_info,
+        "autotune": tune,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity_on_cpu_sample": parity,

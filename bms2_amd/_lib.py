@@ -85,6 +85,11 @@ class PipelineOptions(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("small_waves", C.c_uint32), ("large_waves", C.c_uint32), ("slots", C.c_uint32),
+                ("candidates", C.c_uint32), ("best_ms", C.c_float)]
+
+
 class Adam(C.Structure):
     _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float)]
 
@@ -152,6 +157,7 @@ PROTOTYPES: dict[str, tuple] = {
     "bms2_pipeline_destroy": (i32, [vp]),
     "bms2_pipeline_run": (i32, [vp, vp, vp, vp, vpp, vpp]),
     "bms2_pipeline_set_order": (i32, [vp, vp]),
+    "bms2_pipeline_autotune": (i32, [vp, vp, vp, u32, C.POINTER(Tuning)]),
     "bms2_mlp_num_params": (u64, []),
     "bms2_mlp_create": (i32, [vp, u64, vpp]),
     "bms2_mlp_destroy": (i32, [vp]),

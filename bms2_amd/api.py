@@ -408,6 +408,15 @@ class Pipeline(_Handle):
         v = valid.to_numpy(np.uint64, rows * self.VW).reshape(rows, self.VW)
         return c, v
 
+    def autotune(self, counts: DeviceBuffer, valid: DeviceBuffer, runs: int = 1) -> dict:
+        """Times the join launch shapes on this batch and keeps the fastest
+        (bms2_pipeline_autotune); the outputs hold the last run's (identical) results."""
+        t = _lib.Tuning()
+        check(_L().bms2_pipeline_autotune(self.h, C.c_void_p(counts.ptr), C.c_void_p(valid.ptr),
+                                          int(runs), C.byref(t)))
+        return dict(small_waves=t.small_waves, large_waves=t.large_waves, slots=t.slots,
+                    candidates=t.candidates, best_ms=round(float(t.best_ms), 3))
+
     def set_order(self, order: np.ndarray | None) -> None:
         """Join matching order: order[q, d] = local query node matched at depth d (u8 [Q, 32]);
         None restores identity."""

@@ -833,7 +833,9 @@ struct bms2_pipeline_s {
   uint64_t cap_s = 0, cap_seg = 0, cap_l = 0;
   // measured records / segments per pack of each class, sizes the chunks
   double dens_s = 0.0, dens_seg = 0.0, dens_l = 0.0;
-  int join_grid = 0, join_small_grid = 0;
+  // launch shape (autotuned): waves per CU of each join, LDS slots per small-join wave
+  int cus = 256;
+  uint32_t small_waves = 8, large_waves = 12, slots = 8;
   uint32_t* d_packs = nullptr;
   uint32_t* d_qpacks = nullptr;
   uint32_t* d_qhead = nullptr;
@@ -1185,8 +1187,11 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
   (void)hipGetDevice(&dev);
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
-  p->join_grid = cus * (p->nq_cap <= 16 ? 12 : 6);
-  p->join_small_grid = cus * 8;  // 16.5 KB of LDS per wave: 9 waves per CU
+  p->cus = cus;
+  // defaults: as many waves as LDS allows (small join ~18 KB, large join 17 / 34 KB per wave)
+  p->small_waves = opt->block_waves ? opt->block_waves : 8;
+  p->large_waves = opt->join_waves ? opt->join_waves : (p->nq_cap <= 16 ? 12 : 6);
+  p->slots = 8;
   BMS2_HIP(hipStreamSynchronize(s));
   *out = p;
   return BMS2_OK;
@@ -1287,13 +1292,17 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
       sp.pack_begin = p0;
       hipLaunchKernelGGL(screen_kernel<K>, dim3(p1 - p0), dim3(64), stage_bytes, s, sp);
       BMS2_CHECK_LAUNCH("screen_kernel");
-      hipLaunchKernelGGL(join_small_kernel<8>, dim3(p->join_small_grid), dim3(64), 0, s, js);
+      const dim3 gs(uint32_t(p->cus) * p->small_waves), gl(uint32_t(p->cus) * p->large_waves);
+      if (p->slots == 4)
+        hipLaunchKernelGGL(join_small_kernel<4>, gs, dim3(64), 0, s, js);
+      else
+        hipLaunchKernelGGL(join_small_kernel<8>, gs, dim3(64), 0, s, js);
       BMS2_CHECK_LAUNCH("join_small_kernel");
       if (p->max_nq > uint32_t(kSmallNq)) {
         if (p->nq_cap <= 16)
-          hipLaunchKernelGGL(join_kernel<16>, dim3(p->join_grid), dim3(64), 0, s, jp);
+          hipLaunchKernelGGL(join_kernel<16>, gl, dim3(64), 0, s, jp);
         else
-          hipLaunchKernelGGL(join_kernel<32>, dim3(p->join_grid), dim3(64), 0, s, jp);
+          hipLaunchKernelGGL(join_kernel<32>, gl, dim3(64), 0, s, jp);
         BMS2_CHECK_LAUNCH("join_kernel");
       }
       return BMS2_OK;
@@ -1414,5 +1423,70 @@ extern "C" int bms2_pipeline_trace(bms2_pipeline p, uint32_t max_graphs, uint64_
   uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (int rc = run_packs(p, pe, packs[pe], d_counts, d_valid, tr, ks, stats)) return rc;
   BMS2_HIP(hipStreamSynchronize(p->q->stream));
+  return BMS2_OK;
+}
+
+extern "C" int bms2_pipeline_autotune(bms2_pipeline p, uint64_t* d_counts, uint64_t* d_valid,
+                                      uint32_t runs, bms2_pipeline_tuning* out) {
+  if (!p) return set_error(BMS2_ERR_INVALID, "pipeline is null");
+  if (int rc = use_device(p->q)) return rc;
+  if (!d_counts || !d_valid) return set_error(BMS2_ERR_INVALID, "null output buffer");
+  runs = std::max<uint32_t>(runs, 1);
+  struct Cand {
+    uint32_t small_waves, large_waves, slots;
+  };
+  std::vector<Cand> cands;
+  const uint32_t lw0 = p->nq_cap <= 16 ? 12 : 6;
+  for (uint32_t sw : {4u, 6u, 8u})
+    for (uint32_t sl : {4u, 8u}) cands.push_back({sw, lw0, sl});
+  for (uint32_t lw : {lw0 / 2, lw0 * 3 / 4}) cands.push_back({8, lw, 8});
+  const Cand keep{p->small_waves, p->large_waves, p->slots};
+  float best = 0.f;
+  Cand best_c = keep;
+  bms2_event ev = nullptr;
+  for (size_t i = 0; i < cands.size(); ++i) {
+    const Cand& c = cands[i];
+    if (c.small_waves != p->small_waves || c.large_waves != p->large_waves || c.slots != p->slots) {
+      if (p->graph) (void)hipGraphExecDestroy(p->graph);
+      p->graph = nullptr;  // grid shapes are baked into the graph
+    }
+    p->small_waves = c.small_waves;
+    p->large_waves = c.large_waves;
+    p->slots = c.slots;
+    // one untimed run (measures the batch, captures the graph), then the timed runs
+    for (uint32_t r = 0; r <= runs; ++r) {
+      if (int rc = bms2_pipeline_run(p, d_counts, d_valid, nullptr, nullptr, r ? &ev : nullptr)) {
+        p->small_waves = keep.small_waves;
+        p->large_waves = keep.large_waves;
+        p->slots = keep.slots;
+        return rc;
+      }
+      if (r) {
+        float ms = 0.f;
+        if (int rc = bms2_event_elapsed_ms(ev, &ms)) return rc;
+        bms2_event_destroy(ev);
+        ev = nullptr;
+        if (best == 0.f || ms < best) {
+          best = ms;
+          best_c = c;
+        }
+      }
+    }
+  }
+  if (best_c.small_waves != p->small_waves || best_c.large_waves != p->large_waves ||
+      best_c.slots != p->slots) {
+    if (p->graph) (void)hipGraphExecDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  p->small_waves = best_c.small_waves;
+  p->large_waves = best_c.large_waves;
+  p->slots = best_c.slots;
+  if (out) {
+    out->small_waves = best_c.small_waves;
+    out->large_waves = best_c.large_waves;
+    out->slots = best_c.slots;
+    out->candidates = uint32_t(cands.size());
+    out->best_ms = best;
+  }
   return BMS2_OK;
 }

@@ -259,6 +259,23 @@ int bms2_pipeline_destroy(bms2_pipeline p);
 int bms2_pipeline_run(bms2_pipeline p, uint64_t* d_counts, uint64_t* d_valid, uint64_t* d_stats,
                       bms2_event* ev, bms2_event* kernel_ev);
 
+/* Launch shape of the joins, picked by bms2_pipeline_autotune. */
+typedef struct bms2_pipeline_tuning {
+  uint32_t small_waves; /* waves per CU of the small-query join */
+  uint32_t large_waves; /* waves per CU of the large-query join */
+  uint32_t slots;       /* pack-adjacency slots per small-join wave (4 or 8) */
+  uint32_t candidates;  /* configurations timed */
+  float best_ms;        /* fastest kernel span of one run */
+} bms2_pipeline_tuning;
+
+/* Times whole runs of the resident batch under a set of launch shapes (waves per CU of each join,
+ * pack-adjacency slots per small-join wave), `runs` timed runs each after one untimed run, and
+ * keeps the fastest for later runs.  The outputs hold the last run's results (identical under
+ * every shape).  Refinement depth is not tuned: it changes the reference's results (shells at
+ * distance >= 3 can remove true embeddings).  No counterpart in the reference. */
+int bms2_pipeline_autotune(bms2_pipeline p, uint64_t* d_counts, uint64_t* d_valid, uint32_t runs,
+                           bms2_pipeline_tuning* out);
+
 /* Matching order of the join, per query: order[q*32 + d] = local query node matched at DFS
  * depth d (a permutation of 0..n_q-1 per query; NULL restores identity, the reference's order,
  * isomorphism.hpp:371-447).  Counts do not depend on the order (standard backtracking); only the

@@ -187,3 +187,21 @@ def test_graph_replays_match_eager(queue, molecules_1m_shard, uneven_2k, workloa
         pipe.trace(300, queries.total_nodes)
         eager.run(ec, ev, es)
         queue.wait()
+
+
+def test_autotune_keeps_results(queue, uneven_2k):
+    data, queries = uneven_2k
+    rc, rv, rs = oracle.pipeline(queries, data, 2, False)
+    pipe = api.Pipeline(queue, api.DeviceBatchedCSRGraph(queue, queries),
+                        api.DeviceBatchedCSRGraph(queue, data), refinement_steps=2)
+    counts, valid, stats = pipe.alloc_outputs()
+    t = pipe.autotune(counts, valid, runs=1)
+    assert t["candidates"] >= 6 and t["slots"] in (4, 8) and t["best_ms"] > 0
+    c, v = pipe.results(counts, valid)
+    assert np.array_equal(c, rc) and np.array_equal(v, rv)
+    counts.zero()
+    pipe.run(counts, valid, stats)
+    queue.wait()
+    c, v = pipe.results(counts, valid)
+    assert np.array_equal(c, rc) and np.array_equal(v, rv)
+    assert np.array_equal(stats.to_numpy(np.uint64, 8)[:4], rs)
