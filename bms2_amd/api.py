@@ -408,6 +408,21 @@ class Pipeline(_Handle):
         v = valid.to_numpy(np.uint64, rows * self.VW).reshape(rows, self.VW)
         return c, v
 
+    def set_queries(self, query) -> None:
+        """Binds another query batch to the resident data batch (bms2_pipeline_set_queries)."""
+        check(_L().bms2_pipeline_set_queries(self.h, C.byref(query.c)))
+        self.query_graphs = query
+        self.Q = query.num_graphs
+        self.VW = (self.Q + 63) // 64
+
+    def query_totals(self, counts: DeviceBuffer) -> tuple[np.ndarray, np.ndarray]:
+        """Per query of the bound batch: data graphs matched and summed counts of the last run."""
+        m = DeviceBuffer(self.q, 8 * max(self.Q, 1), zero=False)
+        e = DeviceBuffer(self.q, 8 * max(self.Q, 1), zero=False)
+        check(_L().bms2_pipeline_query_totals(self.h, C.c_void_p(counts.ptr), C.c_void_p(m.ptr),
+                                              C.c_void_p(e.ptr)))
+        return m.to_numpy(np.uint64, self.Q), e.to_numpy(np.uint64, self.Q)
+
     def autotune(self, counts: DeviceBuffer, valid: DeviceBuffer, runs: int = 1) -> dict:
         """Times the join launch shapes on this batch and keeps the fastest
         (bms2_pipeline_autotune); the outputs hold the last run's (identical) results."""

@@ -831,6 +831,7 @@ struct bms2_pipeline_s {
   uint32_t num_packs = 0, num_qpacks = 0, nq_cap = 0, rec_words_l = 0;
   uint32_t max_nq = 0;
   uint64_t cap_s = 0, cap_seg = 0, cap_l = 0;
+  uint64_t alloc_s = 0, alloc_seg = 0, alloc_l_words = 0;  // allocated record buffer sizes
   // measured records / segments per pack of each class, sizes the chunks
   double dens_s = 0.0, dens_seg = 0.0, dens_l = 0.0;
   // launch shape (autotuned): waves per CU of each join, LDS slots per small-join wave
@@ -904,15 +905,16 @@ void pipeline_free(bms2_pipeline p) {
   delete p;
 }
 
-// Pair-record budget: a quarter of free device memory, at most 16 GiB.
-uint64_t record_budget_bytes() {
+// Pair-record budget: a quarter of free device memory (counting record buffers already held),
+// at most 16 GiB.
+uint64_t record_budget_bytes(uint64_t held = 0) {
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 1ull << 30;
-  return std::min<uint64_t>(free_b / 4, 16ull << 30);
+  return std::min<uint64_t>((free_b + held) / 4, 16ull << 30);
 }
 
-// Data graphs above 64 nodes: a compact device batch of them (node ids renumbered), their
-// original graph ids, and the staged objects that serve them on every run.
+// Data graphs above 64 nodes: a compact device batch of them (node ids renumbered) and their
+// original graph ids; the staged objects that serve them depend on the query batch.
 int build_big_batch(bms2_pipeline_s* p, const std::vector<uint32_t>& go) {
   const uint32_t G = uint32_t(go.size() - 1);
   std::vector<uint32_t> gid;
@@ -951,12 +953,6 @@ int build_big_batch(bms2_pipeline_s* p, const std::vector<uint32_t>& go) {
   if (int rc = device_alloc((void**)&p->d_big_gid, 4ull * gid.size())) return rc;
   if (int rc = device_alloc((void**)&p->d_big_stats, 8 * 8)) return rc;
   BMS2_HIP(hipMemcpy(p->d_big_gid, gid.data(), 4ull * gid.size(), hipMemcpyHostToDevice));
-  const uint64_t dn = p->big.total_nodes, qn = p->query.total_nodes;
-  if (int rc = bms2_signature_create(p->q, dn, qn, &p->big_sig)) return rc;
-  if (int rc = bms2_path_signature_create(p->q, dn, qn, &p->big_path)) return rc;
-  if (int rc = bms2_cycle_signature_create(p->q, dn, qn, &p->big_cyc)) return rc;
-  if (int rc = bms2_candidates_create(p->q, qn, dn, &p->big_cand)) return rc;
-  if (int rc = bms2_gmcr_create(p->q, &p->big_gmcr)) return rc;
   return BMS2_OK;
 }
 
@@ -1069,53 +1065,64 @@ int upload_order(bms2_pipeline_s* p, const uint8_t* order) {
 
 }  // namespace
 
-extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
-                                    const bms2_device_csr* data, const bms2_pipeline_options* opt,
-                                    bms2_pipeline* out) {
-  if (int rc = use_device(q)) return rc;
-  if (!query || !data || !opt || !out) return set_error(BMS2_ERR_INVALID, "null argument");
-  if (opt->refinement_steps > uint32_t(kMaxLevels))
-    return set_error(BMS2_ERR_UNSUPPORTED, "refinement steps %u > %d", opt->refinement_steps,
-                     kMaxLevels);
-  if (data->max_graph_nodes > BMS2_MAX_GRAPH_NODES)
-    return set_error(BMS2_ERR_UNSUPPORTED, "data graph of %u nodes (limit %u)",
-                     data->max_graph_nodes, BMS2_MAX_GRAPH_NODES);
-  if (data->max_degree > 27 || query->max_degree > 27)
+namespace {
+
+int check_queries(const bms2_device_csr* query) {
+  if (query->max_degree > 27)
     return set_error(BMS2_ERR_UNSUPPORTED,
-                     "node degree above 27: the reference's 128-entry ring-count stack could "
+                     "query node degree above 27: the reference's 128-entry ring-count stack could "
                      "overflow (signature.hpp:638,697), which the fused ring counts do not model");
   if (query->max_graph_nodes > BMS2_MAX_QUERY_NODES)
     return set_error(BMS2_ERR_UNSUPPORTED, "query with %u nodes (limit %u)",
                      query->max_graph_nodes, BMS2_MAX_QUERY_NODES);
-  auto* p = new bms2_pipeline_s{};
-  p->q = q;
-  p->data = *data;
+  return BMS2_OK;
+}
+
+// Releases everything that depends on the query batch (the data plan stays).
+void free_query_side(bms2_pipeline_s* p) {
+  for (void* b : {(void*)p->d_qpacks, (void*)p->d_qhead, (void*)p->d_qpre, (void*)p->d_qrec,
+                  (void*)p->d_qpos, (void*)p->d_jorder, (void*)p->d_qadj, (void*)p->d_qadj8,
+                  (void*)p->d_big_pc, (void*)p->d_big_pn})
+    (void)hipFree(b);
+  p->d_qpacks = p->d_qhead = p->d_qpre = p->d_qrec = p->d_qpos = p->d_qadj = nullptr;
+  p->d_jorder = nullptr;
+  p->d_qadj8 = p->d_big_pc = p->d_big_pn = nullptr;
+  p->big_cap = 0;
+  if (p->big_sig) bms2_signature_destroy(p->big_sig);
+  if (p->big_path) bms2_path_signature_destroy(p->big_path);
+  if (p->big_cyc) bms2_cycle_signature_destroy(p->big_cyc);
+  if (p->big_cand) bms2_candidates_destroy(p->big_cand);
+  if (p->big_gmcr) bms2_gmcr_destroy(p->big_gmcr);
+  p->big_sig = nullptr;
+  p->big_path = nullptr;
+  p->big_cyc = nullptr;
+  p->big_cand = nullptr;
+  p->big_gmcr = nullptr;
+  if (p->graph) (void)hipGraphExecDestroy(p->graph);
+  p->graph = nullptr;
+  p->dens_s = p->dens_seg = p->dens_l = 0.0;
+}
+
+// Grows a record buffer to hold `want` items of `item` bytes (kept when already large enough).
+int ensure_buffer(void** ptr, uint64_t* have, uint64_t want, uint64_t item) {
+  if (*ptr && *have >= want) return BMS2_OK;
+  (void)hipFree(*ptr);
+  *ptr = nullptr;
+  *have = 0;
+  if (int rc = device_alloc(ptr, want * item)) return rc;
+  *have = want;
+  return BMS2_OK;
+}
+
+// Binds a query batch: screening records, orders, record buffers, and the big graphs' staged
+// objects (sized by query nodes).
+int bind_queries(bms2_pipeline_s* p, const bms2_device_csr* query) {
+  hipStream_t s = p->q->stream;
+  int rc = BMS2_OK;
   p->query = *query;
-  p->opt = *opt;
   p->nq_cap = query->max_graph_nodes <= 16 ? 16 : 32;
   p->rec_words_l = 2 + std::max<uint32_t>(query->max_graph_nodes, 2);
   p->max_nq = query->max_graph_nodes;
-  hipStream_t s = q->stream;
-  int rc = BMS2_OK;
-  auto fail = [&](int code) {
-    pipeline_free(p);
-    return code;
-  };
-  // data packs: the plan of one resident batch
-  std::vector<uint32_t> go(data->num_graphs + 1);
-  if (hipMemcpy(go.data(), data->graph_offsets, 4ull * go.size(), hipMemcpyDeviceToHost) !=
-      hipSuccess)
-    return fail(set_error(BMS2_ERR_HIP, "copy of data graph offsets failed"));
-  std::vector<uint32_t> packs = build_packs(go);
-  p->num_packs = uint32_t(packs.size() - 1);
-  if (int brc = build_big_batch(p, go)) return fail(brc);
-  if ((rc = device_alloc((void**)&p->d_packs, 4ull * packs.size())) ||
-      (rc = device_alloc((void**)&p->d_adj, 8ull * std::max<uint64_t>(data->total_nodes, 1))) ||
-      (rc = device_alloc((void**)&p->d_ctl, 8 * kCtlWords)))
-    return fail(rc);
-  BMS2_HIP(hipMemcpyAsync(p->d_packs, packs.data(), 4ull * packs.size(), hipMemcpyHostToDevice, s));
-
-  // queries: host copy of the (small) query batch for orders and adjacency
   const uint32_t Q = query->num_graphs;
   const uint64_t Qn = query->total_nodes;
   std::vector<uint32_t> qgo(Q + 1), qro(Qn + 1), qci(query->total_edges);
@@ -1125,7 +1132,7 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
       (qci.size() && hipMemcpy(qci.data(), query->column_indices, 4ull * qci.size(),
                                hipMemcpyDeviceToHost)) ||
       (qlab.size() && hipMemcpy(qlab.data(), query->node_labels, qlab.size(), hipMemcpyDeviceToHost)))
-    return fail(set_error(BMS2_ERR_HIP, "copy of the query batch failed"));
+    return set_error(BMS2_ERR_HIP, "copy of the query batch failed");
   std::vector<uint32_t> label_freq(256, 0);
   for (uint8_t l : qlab) label_freq[l]++;
   std::vector<uint32_t> qhead(size_t(Q) * 4, 0), qpos(std::max<uint64_t>(Qn, 1), 0);
@@ -1155,46 +1162,111 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
       (rc = device_alloc((void**)&p->d_jorder, std::max<size_t>(size_t(Q) * 32, 32))) ||
       (rc = device_alloc((void**)&p->d_qadj, 4ull * std::max<size_t>(size_t(Q) * 32, 32))) ||
       (rc = device_alloc((void**)&p->d_qadj8, 8ull * std::max<uint32_t>(Q, 1))))
-    return fail(rc);
+    return rc;
   BMS2_HIP(hipMemcpyAsync(p->d_qpacks, qpacks.data(), 4ull * qpacks.size(), hipMemcpyHostToDevice, s));
   BMS2_HIP(hipMemcpyAsync(p->d_qpos, qpos.data(), 4ull * qpos.size(), hipMemcpyHostToDevice, s));
-  if (Q) {
-    BMS2_HIP(hipMemcpyAsync(p->d_qhead, qhead.data(), 4ull * qhead.size(), hipMemcpyHostToDevice, s));
-  }
+  if (Q) BMS2_HIP(hipMemcpyAsync(p->d_qhead, qhead.data(), 4ull * qhead.size(), hipMemcpyHostToDevice, s));
   p->qgo = std::move(qgo);
   p->qro = std::move(qro);
   p->qci = std::move(qci);
-  if ((rc = upload_order(p, nullptr))) return fail(rc);
+  if ((rc = upload_order(p, nullptr))) return rc;  // synchronises: the host vectors above are done
   // pair records of both classes and the small-record segments: worst case or a share of the
-  // budget, whichever is smaller (the chunks are sized to what a run measures)
+  // budget, whichever is smaller (the chunks are sized to what a run measures); buffers large
+  // enough from an earlier batch are kept
   uint32_t q_small = 0;
   for (uint32_t g = 0; g < Q; ++g) q_small += (p->qgo[g + 1] - p->qgo[g]) <= uint32_t(kSmallNq);
-  const uint64_t budget = record_budget_bytes();
+  const uint64_t budget = record_budget_bytes(8 * kSmallWords * p->alloc_s +
+                                              sizeof(uint4) * p->alloc_seg + 8ull * p->alloc_l_words);
   // large queries hold few GMCR pairs (a node of every large query needs a candidate)
   const double share_l = q_small == Q ? 0.0 : (q_small == 0 ? 0.9 : 0.15);
-  const uint64_t worst_s = uint64_t(data->num_graphs) * q_small;
-  const uint64_t worst_l = uint64_t(data->num_graphs) * (Q - q_small);
+  const uint64_t D = p->data.num_graphs;
+  const uint64_t worst_s = D * q_small, worst_l = D * (Q - q_small);
   // record indices travel in 29 bits through the small join's queue
   p->cap_s = std::max<uint64_t>(1, std::min<uint64_t>({worst_s, uint64_t(0.95 * (1 - share_l) * budget) / (8 * kSmallWords), (1ull << 29) - 1}));
   p->cap_seg = std::max<uint64_t>(1, std::min<uint64_t>({p->cap_s, uint64_t(0.05 * (1 - share_l) * budget) / 16,
                                                           (uint64_t(1) << (64 - kSegShift)) - 1}));
   p->cap_l = std::max<uint64_t>(1, std::min<uint64_t>(worst_l, uint64_t(share_l * budget) / (8ull * p->rec_words_l)));
-  if ((rc = device_alloc((void**)&p->d_recs_s, p->cap_s * 8 * kSmallWords)) ||
-      (rc = device_alloc((void**)&p->d_segs, p->cap_seg * sizeof(uint4))) ||
-      (rc = device_alloc((void**)&p->d_recs_l, p->cap_l * 8ull * p->rec_words_l)))
+  uint64_t l_words = p->alloc_l_words;
+  if ((rc = ensure_buffer((void**)&p->d_recs_s, &p->alloc_s, p->cap_s, 8 * kSmallWords)) ||
+      (rc = ensure_buffer((void**)&p->d_segs, &p->alloc_seg, p->cap_seg, sizeof(uint4))) ||
+      (rc = ensure_buffer((void**)&p->d_recs_l, &l_words, p->cap_l * p->rec_words_l, 8)))
+    return rc;
+  p->alloc_l_words = l_words;
+  if (p->num_big) {
+    const uint64_t dn = p->big.total_nodes;
+    if ((rc = bms2_signature_create(p->q, dn, Qn, &p->big_sig)) ||
+        (rc = bms2_path_signature_create(p->q, dn, Qn, &p->big_path)) ||
+        (rc = bms2_cycle_signature_create(p->q, dn, Qn, &p->big_cyc)) ||
+        (rc = bms2_candidates_create(p->q, Qn, dn, &p->big_cand)) ||
+        (rc = bms2_gmcr_create(p->q, &p->big_gmcr)))
+      return rc;
+  }
+  p->large_waves = p->opt.join_waves ? p->opt.join_waves : (p->nq_cap <= 16 ? 12 : 6);
+  BMS2_HIP(hipStreamSynchronize(s));
+  return BMS2_OK;
+}
+
+}  // namespace
+
+extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
+                                    const bms2_device_csr* data, const bms2_pipeline_options* opt,
+                                    bms2_pipeline* out) {
+  if (int rc = use_device(q)) return rc;
+  if (!query || !data || !opt || !out) return set_error(BMS2_ERR_INVALID, "null argument");
+  if (opt->refinement_steps > uint32_t(kMaxLevels))
+    return set_error(BMS2_ERR_UNSUPPORTED, "refinement steps %u > %d", opt->refinement_steps,
+                     kMaxLevels);
+  if (data->max_graph_nodes > BMS2_MAX_GRAPH_NODES)
+    return set_error(BMS2_ERR_UNSUPPORTED, "data graph of %u nodes (limit %u)",
+                     data->max_graph_nodes, BMS2_MAX_GRAPH_NODES);
+  if (data->max_degree > 27)
+    return set_error(BMS2_ERR_UNSUPPORTED,
+                     "node degree above 27: the reference's 128-entry ring-count stack could "
+                     "overflow (signature.hpp:638,697), which the fused ring counts do not model");
+  if (int rc = check_queries(query)) return rc;
+  auto* p = new bms2_pipeline_s{};
+  p->q = q;
+  p->data = *data;
+  p->opt = *opt;
+  hipStream_t s = q->stream;
+  int rc = BMS2_OK;
+  auto fail = [&](int code) {
+    pipeline_free(p);
+    return code;
+  };
+  // data packs: the plan of one resident batch
+  std::vector<uint32_t> go(data->num_graphs + 1);
+  if (hipMemcpy(go.data(), data->graph_offsets, 4ull * go.size(), hipMemcpyDeviceToHost) !=
+      hipSuccess)
+    return fail(set_error(BMS2_ERR_HIP, "copy of data graph offsets failed"));
+  std::vector<uint32_t> packs = build_packs(go);
+  p->num_packs = uint32_t(packs.size() - 1);
+  if ((rc = build_big_batch(p, go))) return fail(rc);
+  if ((rc = device_alloc((void**)&p->d_packs, 4ull * packs.size())) ||
+      (rc = device_alloc((void**)&p->d_adj, 8ull * std::max<uint64_t>(data->total_nodes, 1))) ||
+      (rc = device_alloc((void**)&p->d_ctl, 8 * kCtlWords)))
     return fail(rc);
+  BMS2_HIP(hipMemcpyAsync(p->d_packs, packs.data(), 4ull * packs.size(), hipMemcpyHostToDevice, s));
   int dev = 0, cus = 0;
   (void)hipGetDevice(&dev);
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
   p->cus = cus;
-  // defaults: as many waves as LDS allows (small join ~18 KB, large join 17 / 34 KB per wave)
+  // defaults: as many waves as LDS allows (small join ~18 KB per wave)
   p->small_waves = opt->block_waves ? opt->block_waves : 8;
-  p->large_waves = opt->join_waves ? opt->join_waves : (p->nq_cap <= 16 ? 12 : 6);
   p->slots = 8;
-  BMS2_HIP(hipStreamSynchronize(s));
+  if ((rc = bind_queries(p, query))) return fail(rc);
   *out = p;
   return BMS2_OK;
+}
+
+extern "C" int bms2_pipeline_set_queries(bms2_pipeline p, const bms2_device_csr* query) {
+  if (!p || !query) return set_error(BMS2_ERR_INVALID, "null argument");
+  if (int rc = use_device(p->q)) return rc;
+  if (int rc = check_queries(query)) return rc;
+  BMS2_HIP(hipStreamSynchronize(p->q->stream));
+  free_query_side(p);
+  return bind_queries(p, query);
 }
 
 extern "C" int bms2_pipeline_destroy(bms2_pipeline p) {
@@ -1488,5 +1560,53 @@ extern "C" int bms2_pipeline_autotune(bms2_pipeline p, uint64_t* d_counts, uint6
     out->candidates = uint32_t(cands.size());
     out->best_ms = best;
   }
+  return BMS2_OK;
+}
+
+namespace bms2 {
+namespace {
+
+// Per query: data graphs with a non-zero count and the summed counts, over a range of graphs
+// per block (coalesced along the query axis of the data-graph-major counts).
+__global__ void query_totals_kernel(const uint64_t* __restrict__ counts, uint64_t D, uint32_t Q,
+                                    uint32_t graphs_per_block, uint64_t* __restrict__ matched,
+                                    uint64_t* __restrict__ emb) {
+  const uint64_t g0 = uint64_t(blockIdx.x) * graphs_per_block;
+  const uint64_t g1 = min(D, g0 + graphs_per_block);
+  for (uint32_t q = threadIdx.x; q < Q; q += blockDim.x) {
+    uint64_t m = 0, e = 0;
+    for (uint64_t g = g0; g < g1; ++g) {
+      const uint64_t c = counts[g * Q + q];
+      m += c ? 1 : 0;
+      e += c;
+    }
+    if (m) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(matched + q), (unsigned long long)m);
+      atomicAdd(reinterpret_cast<unsigned long long*>(emb + q), (unsigned long long)e);
+    }
+  }
+}
+
+}  // namespace
+}  // namespace bms2
+
+extern "C" int bms2_pipeline_query_totals(bms2_pipeline p, const uint64_t* d_counts,
+                                          uint64_t* d_matched, uint64_t* d_embeddings) {
+  if (!p || !d_counts || !d_matched || !d_embeddings)
+    return set_error(BMS2_ERR_INVALID, "null argument");
+  if (int rc = use_device(p->q)) return rc;
+  const uint64_t D = p->data.num_graphs;
+  const uint32_t Q = p->query.num_graphs;
+  hipStream_t s = p->q->stream;
+  if (Q) {
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(256), 0, s, d_matched, Q);
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(256), 0, s, d_embeddings, Q);
+  }
+  if (D && Q) {
+    const uint32_t per = 2048;
+    hipLaunchKernelGGL(query_totals_kernel, dim3(uint32_t((D + per - 1) / per)), dim3(256), 0, s,
+                       d_counts, D, Q, per, d_matched, d_embeddings);
+  }
+  BMS2_CHECK_LAUNCH("query_totals_kernel");
   return BMS2_OK;
 }

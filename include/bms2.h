@@ -259,6 +259,19 @@ int bms2_pipeline_destroy(bms2_pipeline p);
 int bms2_pipeline_run(bms2_pipeline p, uint64_t* d_counts, uint64_t* d_valid, uint64_t* d_stats,
                       bms2_event* ev, bms2_event* kernel_ev);
 
+/* Binds another query batch to the pipeline's resident data batch (the data plan, its packs and
+ * adjacency rows are kept; the matching order returns to identity, the launch shape of the
+ * small-query join is kept).  The reference regenerates query signatures per data batch
+ * (BMS2_batch.cpp:162-283); here the data side is the resident part.  The caller keeps the query
+ * CSR alive until the next bind or destroy. */
+int bms2_pipeline_set_queries(bms2_pipeline p, const bms2_device_csr* query);
+
+/* Per query of the bound batch, over d_counts of the last run (u64[D*Q]): the number of data
+ * graphs with a non-zero count (d_matched u64[Q]) and the summed counts (d_embeddings u64[Q]).
+ * Serving callers read these instead of the dense counts.  No counterpart in the reference. */
+int bms2_pipeline_query_totals(bms2_pipeline p, const uint64_t* d_counts, uint64_t* d_matched,
+                               uint64_t* d_embeddings);
+
 /* Launch shape of the joins, picked by bms2_pipeline_autotune. */
 typedef struct bms2_pipeline_tuning {
   uint32_t small_waves; /* waves per CU of the small-query join */
