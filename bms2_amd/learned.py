@@ -119,6 +119,9 @@ class QueryInfo:
     dist: np.ndarray     # all-pairs BFS distances (n x n, large for unreachable)
     labels: np.ndarray
     on_cycle: np.ndarray
+    static: np.ndarray | None = None   # structure-only feature columns, computed once
+    nbr: np.ndarray | None = None      # neighbour lists flattened (local ids)
+    nbr_ptr: np.ndarray | None = None  # [n + 1] offsets into nbr
 
 
 def query_infos(queries) -> list[QueryInfo]:
@@ -164,58 +167,88 @@ def query_infos(queries) -> list[QueryInfo]:
     return out
 
 
-def features(infos: list[QueryInfo], domsum: np.ndarray, qpairs: np.ndarray) -> np.ndarray:
-    """[total query nodes][32] f32 features, rows in global query node order."""
-    total = sum(qi.n for qi in infos)
-    X = np.zeros((total, NUM_FEATURES), np.float32)
-    all_labels = np.concatenate([qi.labels for qi in infos]) if infos else np.zeros(0, np.uint8)
-    freq = np.bincount(all_labels, minlength=256).astype(np.float64) / max(len(all_labels), 1)
-    r = 0
-    for q, qi in enumerate(infos):
-        n = qi.n
-        pairs = float(qpairs[q])
-        dom = domsum[r:r + n].astype(np.float64) / pairs if pairs > 0 else np.zeros(n)
+def _static_features(qi: QueryInfo) -> np.ndarray:
+    """Feature columns of one query that depend on its structure only (cached on the info);
+    the label-rarity and domain columns are filled per batch."""
+    if qi.static is not None:
+        return qi.static
+    n = qi.n
+    X = np.zeros((n, NUM_FEATURES), np.float32)
+    qi.nbr = np.array([v for u in range(n) for v in sorted(qi.adj[u])], np.int64)
+    qi.nbr_ptr = np.concatenate([[0], np.cumsum([len(a) for a in qi.adj])]).astype(np.int64)
+    if n:
         m = int(qi.deg.sum()) // 2
         reach = np.where(qi.dist < 99, qi.dist, 0)
-        ecc = reach.max(axis=1) if n else np.zeros(0)
-        meandom = float(dom.mean()) if n else 0.0
-        deg_rank = np.argsort(np.argsort(-qi.deg, kind="stable"), kind="stable")
-        dom_rank = np.argsort(np.argsort(dom, kind="stable"), kind="stable")
-        amax_deg = int(np.argmax(qi.deg)) if n else -1
-        amin_dom = int(np.argmin(dom)) if n else -1
+        ecc = reach.max(axis=1)
+        X[:, 0] = n / 16
+        X[:, 1] = m / 16
+        X[:, 2] = (m - n + 1) / 4
+        X[:, 3] = qi.deg.max() / 4
+        X[:, 4] = ecc.max() / 8
+        X[:, 5] = len(set(qi.labels.tolist())) / 8
+        X[:, 8] = qi.deg / 4
+        X[:, 9] = ecc / 8
+        X[:, 10] = reach.sum(axis=1) / max(n - 1, 1) / 4
         for u in range(n):
-            f = X[r + u]
-            f[0] = n / 16
-            f[1] = m / 16
-            f[2] = (m - n + 1) / 4
-            f[3] = qi.deg.max() / 4
-            f[4] = ecc.max() / 8
-            f[5] = len(set(qi.labels.tolist())) / 8
-            f[6] = np.log2(1 + pairs) / 16
-            f[7] = np.log2(1 + meandom) / 6
-            f[8] = qi.deg[u] / 4
-            f[9] = ecc[u] / 8
-            f[10] = reach[u].sum() / max(n - 1, 1) / 4
-            f[11] = np.log2(1 + dom[u]) / 6
-            f[12] = np.log2((1 + dom[u]) / (1 + meandom)) / 4
             nb = list(qi.adj[u])
-            f[13] = sum(1 for v in nb if qi.deg[v] == 1) / 4
-            f[14] = sum(int(qi.deg[v]) for v in nb) / 16
-            f[15] = float(qi.on_cycle[u])
+            X[u, 13] = sum(1 for v in nb if qi.deg[v] == 1) / 4
+            X[u, 14] = sum(int(qi.deg[v]) for v in nb) / 16
             lab = int(qi.labels[u])
-            f[16] = float(lab == _WILDCARD)
+            X[u, 16] = float(lab == _WILDCARD)
             if lab in _ELEMENTS:
-                f[17 + _ELEMENTS.index(lab)] = 1.0
+                X[u, 17 + _ELEMENTS.index(lab)] = 1.0
             elif lab != _WILDCARD:
-                f[24] = 1.0
-            f[25] = u / max(n - 1, 1)
-            f[26] = np.log2(1 + min(dom[v] for v in nb)) / 6 if nb else 0.0
-            f[27] = -np.log2(max(freq[lab], 1e-9)) / 8
-            f[28] = deg_rank[u] / max(n - 1, 1)
-            f[29] = dom_rank[u] / max(n - 1, 1)
-            f[30] = float(u == amax_deg)
-            f[31] = float(u == amin_dom)
-        r += n
+                X[u, 24] = 1.0
+        X[:, 15] = qi.on_cycle.astype(np.float32)
+        X[:, 25] = np.arange(n) / max(n - 1, 1)
+        X[:, 28] = np.argsort(np.argsort(-qi.deg, kind="stable"), kind="stable") / max(n - 1, 1)
+        X[int(np.argmax(qi.deg)), 30] = 1.0
+    qi.static = X
+    return X
+
+
+def features(infos: list[QueryInfo], domsum: np.ndarray, qpairs: np.ndarray) -> np.ndarray:
+    """[total query nodes][32] f32 features, rows in global query node order: query size, edges,
+    cycle rank, largest degree, diameter, distinct labels, traced GMCR pairs and mean domain;
+    per node degree, eccentricity, closeness, domain size and its ratio to the query mean, leaf
+    neighbours, neighbour degrees, ring membership, wildcard / element one-hot, position, smallest
+    neighbour domain, label rarity in the batch, degree and domain ranks, largest-degree and
+    smallest-domain flags."""
+    total = sum(qi.n for qi in infos)
+    if total == 0:
+        return np.zeros((0, NUM_FEATURES), np.float32)
+    X = np.concatenate([_static_features(qi) for qi in infos])
+    labels = np.concatenate([qi.labels for qi in infos]).astype(np.int64)
+    freq = np.bincount(labels, minlength=256).astype(np.float64) / total
+    X[:, 27] = -np.log2(np.maximum(freq[labels], 1e-9)) / 8
+    sizes = np.array([qi.n for qi in infos], np.int64)
+    qid = np.repeat(np.arange(len(infos)), sizes)
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    pairs = np.asarray(qpairs, np.float64)[: len(infos)]
+    dom = np.where(pairs[qid] > 0,
+                   np.asarray(domsum[:total], np.float64) / np.maximum(pairs[qid], 1), 0.0)
+    nz = sizes > 0
+    meandom = np.zeros(len(infos))
+    meandom[nz] = np.add.reduceat(dom, starts[nz]) / sizes[nz]
+    X[:, 6] = (np.log2(1 + pairs) / 16)[qid]
+    X[:, 7] = (np.log2(1 + meandom) / 6)[qid]
+    X[:, 11] = np.log2(1 + dom) / 6
+    X[:, 12] = np.log2((1 + dom) / (1 + meandom[qid])) / 4
+    # smallest neighbour domain: neighbour lists of all queries in global node ids
+    nbr = np.concatenate([qi.nbr + st for qi, st in zip(infos, starts)])
+    deg = np.concatenate([np.diff(qi.nbr_ptr) for qi in infos])
+    has = deg > 0
+    if nbr.size:
+        ptr = np.concatenate([[0], np.cumsum(deg)[:-1]])
+        mins = np.minimum.reduceat(dom[nbr], ptr[has])
+        X[has, 26] = np.log2(1 + mins) / 6
+    # domain rank inside each query (ties by local id) and the smallest-domain flag
+    local = np.arange(total) - starts[qid]
+    order = np.lexsort((local, dom, qid))
+    rank = np.empty(total, np.int64)
+    rank[order] = np.arange(total) - starts[qid[order]]
+    X[:, 29] = rank / np.maximum(sizes[qid] - 1, 1)
+    X[:, 31] = (rank == 0).astype(np.float32)
     return X
 
 

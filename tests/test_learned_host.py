@@ -74,3 +74,65 @@ def test_candidate_orders_and_features():
     assert np.all(X[4:10, 11] == 0) and np.all(X[4:10, 15] == 1)
     # wildcard and element one-hot columns
     assert X[14, 16] == 1 and X[0, 17 + learned._ELEMENTS.index(8)] == 1
+
+
+def _features_by_node(infos, domsum, qpairs):
+    """Straightforward per-node statement of the 32 features (the vectorised version must match)."""
+    total = sum(qi.n for qi in infos)
+    X = np.zeros((total, learned.NUM_FEATURES), np.float32)
+    labels = np.concatenate([qi.labels for qi in infos])
+    freq = np.bincount(labels, minlength=256) / max(len(labels), 1)
+    r = 0
+    for q, qi in enumerate(infos):
+        n = qi.n
+        pairs = float(qpairs[q])
+        dom = domsum[r:r + n].astype(np.float64) / pairs if pairs > 0 else np.zeros(n)
+        m = int(qi.deg.sum()) // 2
+        reach = np.where(qi.dist < 99, qi.dist, 0)
+        ecc = reach.max(axis=1)
+        meandom = dom.mean()
+        for u in range(n):
+            f = X[r + u]
+            nb = sorted(qi.adj[u])
+            lab = int(qi.labels[u])
+            f[:6] = [n / 16, m / 16, (m - n + 1) / 4, qi.deg.max() / 4, ecc.max() / 8,
+                     len(set(qi.labels.tolist())) / 8]
+            f[6] = np.log2(1 + pairs) / 16
+            f[7] = np.log2(1 + meandom) / 6
+            f[8], f[9] = qi.deg[u] / 4, ecc[u] / 8
+            f[10] = reach[u].sum() / max(n - 1, 1) / 4
+            f[11] = np.log2(1 + dom[u]) / 6
+            f[12] = np.log2((1 + dom[u]) / (1 + meandom)) / 4
+            f[13] = sum(qi.deg[v] == 1 for v in nb) / 4
+            f[14] = sum(qi.deg[v] for v in nb) / 16
+            f[15] = float(qi.on_cycle[u])
+            f[16] = float(lab == 255)
+            if lab in learned._ELEMENTS:
+                f[17 + learned._ELEMENTS.index(lab)] = 1
+            elif lab != 255:
+                f[24] = 1
+            f[25] = u / max(n - 1, 1)
+            f[26] = np.log2(1 + min(dom[v] for v in nb)) / 6 if nb else 0
+            f[27] = -np.log2(max(freq[lab], 1e-9)) / 8
+            f[28] = sorted(range(n), key=lambda v: (-qi.deg[v], v)).index(u) / max(n - 1, 1)
+            f[29] = sorted(range(n), key=lambda v: (dom[v], v)).index(u) / max(n - 1, 1)
+            f[30] = float(u == int(np.argmax(qi.deg)))
+            f[31] = float(u == sorted(range(n), key=lambda v: (dom[v], v))[0])
+        r += n
+    return X
+
+
+def test_vectorised_features_match_per_node_statement():
+    from bms2_amd import graphs
+    _, queries = graphs.generate("uneven-1m", seed=2, num_data_graphs=50, num_query_graphs=60)
+    infos = learned.query_infos(queries)
+    rng = np.random.default_rng(0)
+    domsum = rng.integers(0, 500, size=queries.total_nodes).astype(np.uint64)
+    domsum[:20] = 7  # ties
+    qpairs = rng.integers(0, 30, size=queries.num_graphs).astype(np.uint64)
+    qpairs[::7] = 0
+    got = learned.features(infos, domsum, qpairs)
+    want = _features_by_node(infos, domsum, qpairs)
+    assert np.allclose(got, want, rtol=1e-6, atol=1e-6), np.argwhere(~np.isclose(got, want))[:5]
+    # cached static parts give the same result on a second call
+    assert np.array_equal(learned.features(infos, domsum, qpairs), got)
