@@ -16,7 +16,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import sys
 import time
 from collections import deque
@@ -132,7 +131,10 @@ def report(requests: list[Request], loop: dict, rate: float, order: str,
 
 class PipelineExecutor:
     """Answers batches on the GPU: binds the batch's queries to the resident pipeline, runs it,
-    and reads per-query totals."""
+    and reads per-query totals.  With a network, a batch holding queries never planned before is
+    planned (trace of plan_graphs resident graphs, bf16 inference) and the orders are cached per
+    pool query: a query's plan depends on the query and the resident data only, so later batches
+    reuse it without tracing."""
 
     def __init__(self, q, pipe, pool, num_data_graphs: int, max_queries: int, net=None,
                  plan_graphs: int = 4096):
@@ -144,6 +146,8 @@ class PipelineExecutor:
         self.valid = DeviceBuffer(q, 8 * self.D * ((max_queries + 63) // 64), zero=False)
         self.stats = DeviceBuffer(q, 64, zero=True)
         self.infos = None
+        self.orders: dict[int, np.ndarray] = {}  # pool index -> planned order row
+        self.planned_batches = 0
         if net is not None:
             from . import learned
             self.infos = learned.query_infos(pool)
@@ -159,8 +163,14 @@ class PipelineExecutor:
         self.pipe.set_queries(dq)
         self.keep = [dq]
         if self.net is not None:
-            learned.plan(self.pipe, self.net, qb, sample_graphs=self.plan_graphs,
-                         infos=[self.infos[int(i)] for i in idx])
+            if all(int(i) in self.orders for i in idx):
+                self.pipe.set_order(np.stack([self.orders[int(i)] for i in idx]))
+            else:
+                pl = learned.plan(self.pipe, self.net, qb, sample_graphs=self.plan_graphs,
+                                  infos=[self.infos[int(i)] for i in idx])
+                self.planned_batches += 1
+                for k, i in enumerate(idx):
+                    self.orders.setdefault(int(i), pl.order[k].copy())
         self.pipe.run(self.counts, self.valid, self.stats)
         matched, emb = self.pipe.query_totals(self.counts)
         o = 0
@@ -210,6 +220,8 @@ def _main(argv=None) -> int:
                 r.done = -1.0
             reqs, loop = serve(reqs, ex, data.num_graphs, budget)
             rep = report(reqs, loop, rate, order, data.num_graphs)
+            rep.extra = dict(planned_batches=ex.planned_batches, cached_orders=len(ex.orders))
+            ex.planned_batches = 0
             log(f"[serve] {order} @ {rate:g} req/s: p50 {rep.p50_ms} ms, p99 {rep.p99_ms} ms, "
                 f"{rep.queries_per_s} queries/s, {rep.batches} batches")
             out.append(rep.__dict__)

@@ -57,3 +57,20 @@ def test_serving_executor_answers(queue):
         assert np.array_equal(r.matched, (rc[:, r.queries] > 0).sum(axis=0).astype(np.uint64))
         assert np.array_equal(r.embeddings, rc[:, r.queries].sum(axis=0).astype(np.uint64))
     assert len(loop["batches"]) >= 2
+
+
+def test_serving_executor_learned_orders_keep_answers(queue):
+    from bms2_amd import learned
+    data, pool = graphs.generate("molecules-2m", seed=0, num_data_graphs=2000, num_query_graphs=120)
+    dd = api.DeviceBatchedCSRGraph(queue, data)
+    pipe = api.Pipeline(queue, api.DeviceBatchedCSRGraph(queue, _subset(pool, range(16))), dd,
+                        refinement_steps=6)
+    net = learned.OrderNet.load(queue)
+    ex = serve.PipelineExecutor(queue, pipe, pool, data.num_graphs, max_queries=512, net=net,
+                                plan_graphs=1000)
+    reqs = serve.make_requests(10, 500.0, pool.num_graphs, seed=5, max_queries=60)
+    out, loop = serve.serve(reqs, ex, data.num_graphs, serve.batch_bytes(data.num_graphs, 100))
+    rc, _, _ = oracle.pipeline(pool, data, 6, False)
+    for r in out:
+        assert np.array_equal(r.embeddings, rc[:, r.queries].sum(axis=0).astype(np.uint64))
+    assert ex.planned_batches >= 1 and len(ex.orders) > 0
