@@ -51,6 +51,7 @@ struct ScreenParams {
   const uint32_t* qhead;    // [Q][4]: first record, nq, valid (nq > 1, gmcr.hpp:64), 0
   const uint32_t* qpre;     // [Q][kPreDwords] prefilter records
   const uint32_t* qrec;     // [Qn][kRecDwords] records in screening order per query
+  uint32_t hist_map;        // labels of the compact shell histograms (screen_kernel<K, 2>)
   const uint8_t* jorder;    // [Q][32] local query node matched at each DFS depth
   const uint64_t* qadj8;    // [Q] byte d: earlier depths adjacent to depth d (nq <= kSmallNq)
   uint64_t* adj_out;        // [Dn] adjacency rows over pack lanes
@@ -101,7 +102,22 @@ struct QueryPrepParams {
   const uint32_t* pack_gbegin;
   const uint32_t* rec_pos;  // [Qn] record slot of each query node
   uint32_t* qrec;
+  uint32_t hist_map;        // 8 labels (4 bits each) of the compact shell histograms
+  uint32_t compact;         // 1: shell levels as 2 words over the hist_map labels, 0: 4 words
 };
+
+// Shell histogram restricted to the 8 labels of `map` (nibble i = count of label map[i]).  A
+// label that no query node carries has query count 0 at every level and passes every data node,
+// so dropping it keeps the dominance test exact.
+__device__ inline uint32_t compact_hist(uint64_t h, uint32_t map) {
+  uint32_t out = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const uint32_t lab = (map >> (4 * i)) & 15u;
+    out |= uint32_t((h >> (4 * lab)) & 15u) << (4 * i);
+  }
+  return out;
+}
 
 __device__ inline uint32_t cload(const uint32_t* p, uint32_t i) {
   return ((const BMS2_CONST uint32_t*)(p))[i];
@@ -138,15 +154,25 @@ __global__ __launch_bounds__(64) void query_prep_kernel(QueryPrepParams p) {
   r[9] = uint32_t(t.cycle);
   r[10] = uint32_t(t.cycle >> 32);
   r[11] = 0;
+  if (p.compact) {
 #pragma unroll
-  for (int k = 0; k < kMaxLevels; ++k) {
-    const uint32_t lo = uint32_t(t.hist[k]), hi = uint32_t(t.hist[k] >> 32);
-    r[12 + 4 * k + 0] = (lo & 0x0f0f0f0fu) + 0x0f0f0f0fu;
-    r[12 + 4 * k + 1] = ((lo >> 4) & 0x0f0f0f0fu) + 0x0f0f0f0fu;
-    r[12 + 4 * k + 2] = (hi & 0x0f0f0f0fu) + 0x0f0f0f0fu;
-    r[12 + 4 * k + 3] = ((hi >> 4) & 0x0f0f0f0fu) + 0x0f0f0f0fu;
+    for (int k = 0; k < kMaxLevels; ++k) {
+      const uint32_t c = compact_hist(t.hist[k], p.hist_map);
+      r[12 + 2 * k + 0] = (c & 0x0f0f0f0fu) + 0x0f0f0f0fu;
+      r[12 + 2 * k + 1] = ((c >> 4) & 0x0f0f0f0fu) + 0x0f0f0f0fu;
+    }
+    for (int w = 12 + 2 * kMaxLevels; w < kRecDwords; ++w) r[w] = 0;
+  } else {
+#pragma unroll
+    for (int k = 0; k < kMaxLevels; ++k) {
+      const uint32_t lo = uint32_t(t.hist[k]), hi = uint32_t(t.hist[k] >> 32);
+      r[12 + 4 * k + 0] = (lo & 0x0f0f0f0fu) + 0x0f0f0f0fu;
+      r[12 + 4 * k + 1] = ((lo >> 4) & 0x0f0f0f0fu) + 0x0f0f0f0fu;
+      r[12 + 4 * k + 2] = (hi & 0x0f0f0f0fu) + 0x0f0f0f0fu;
+      r[12 + 4 * k + 3] = ((hi >> 4) & 0x0f0f0f0fu) + 0x0f0f0f0fu;
+    }
+    for (int w = 12 + 4 * kMaxLevels; w < kRecDwords; ++w) r[w] = 0;
   }
-  for (int w = 12 + 4 * kMaxLevels; w < kRecDwords; ++w) r[w] = 0;
 }
 
 // Per-query prefilter record (kPreDwords): [0..3] presence bits every node needs (label < 16 at
@@ -179,7 +205,7 @@ __global__ void query_pre_kernel(const uint32_t* __restrict__ qhead, const uint3
 
 // ------------------------------------------------------------------ screening
 
-template <int K>
+template <int K, int HW>  // HW: u32 words per shell level (2 compact, 4 full)
 __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
   __shared__ uint64_t adj[64];
   __shared__ uint8_t lbl[64];
@@ -206,14 +232,20 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
   // ring counts with the top bit of each u16 set: (d | 0x8000) - q has bit 15 set iff d >= q
   const uint32_t d_cyc0 = uint32_t(t.cycle) | 0x80008000u;
   const uint32_t d_cyc1 = uint32_t(t.cycle >> 32) | 0x80008000u;
-  uint32_t d_hist[K > 0 ? K : 1][4];
+  uint32_t d_hist[K > 0 ? K : 1][HW];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const uint32_t lo = uint32_t(t.hist[k]), hi = uint32_t(t.hist[k] >> 32);
-    d_hist[k][0] = lo & 0x0f0f0f0fu;
-    d_hist[k][1] = (lo >> 4) & 0x0f0f0f0fu;
-    d_hist[k][2] = hi & 0x0f0f0f0fu;
-    d_hist[k][3] = (hi >> 4) & 0x0f0f0f0fu;
+    if (HW == 2) {
+      const uint32_t c = compact_hist(t.hist[k], p.hist_map);
+      d_hist[k][0] = c & 0x0f0f0f0fu;
+      d_hist[k][HW - 1] = (c >> 4) & 0x0f0f0f0fu;
+    } else {
+      const uint32_t lo = uint32_t(t.hist[k]), hi = uint32_t(t.hist[k] >> 32);
+      d_hist[k][0] = lo & 0x0f0f0f0fu;
+      d_hist[k][1 % HW] = (lo >> 4) & 0x0f0f0f0fu;
+      d_hist[k][2 % HW] = hi & 0x0f0f0f0fu;
+      d_hist[k][3 % HW] = (hi >> 4) & 0x0f0f0f0fu;
+    }
   }
 
   // Graph segments: node lanes of a graph are contiguous.  mytop = last lane of the node's graph
@@ -349,12 +381,12 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
       const uint32_t* r = p.qrec + size_t(rb) * kRecDwords;
       for (uint32_t ti = 0; ti < nq; ++ti, r += kRecDwords) {
         // the whole record in one batch of scalar loads, then a branch-free test
-        uint32_t w[12 + 4 * K];
+        uint32_t w[12 + HW * K];
 #pragma unroll
-        for (int x = 0; x < 12 + 4 * K; ++x) w[x] = cload(r, x);
+        for (int x = 0; x < 12 + HW * K; ++x) w[x] = cload(r, x);
         // keep the record's loads in one batch ahead of the uniform branches below
 #pragma unroll
-        for (int x = 0; x < 12 + 4 * K; ++x) asm volatile("" ::"s"(w[x]));
+        for (int x = 0; x < 12 + HW * K; ++x) asm volatile("" ::"s"(w[x]));
         const uint32_t ql = w[0] & 255u, qd = (w[0] >> 8) & 255u, local = (w[0] >> 16) & 255u;
         const uint32_t lm = w[0] >> 24;  // shell levels with a non-empty query histogram
         uint64_t m = ballot((d_label == ql || ql == kWildcard) && d_deg >= qd) & lanes;
@@ -366,9 +398,12 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
           uint32_t fail = ((c0 & c1) & 0x80008000u) ^ 0x80008000u;  // a ring field below the query's
 #pragma unroll
           for (int k = 0; k < K; ++k)
-            if ((lm >> k) & 1)  // an empty query shell passes every data node
-              fail |= ((w[12 + 4 * k] - d_hist[k][0]) | (w[13 + 4 * k] - d_hist[k][1]) |
-                       (w[14 + 4 * k] - d_hist[k][2]) | (w[15 + 4 * k] - d_hist[k][3])) & 0x10101010u;
+            if ((lm >> k) & 1) {  // an empty query shell passes every data node
+              uint32_t x = 0;
+#pragma unroll
+              for (int h = 0; h < HW; ++h) x |= w[12 + HW * k + h] - d_hist[k][h];
+              fail |= x & 0x10101010u;
+            }
           m &= ballot((acc | fail) == 0);
         }
         if (lane == local) {
@@ -830,6 +865,7 @@ struct bms2_pipeline_s {
   bms2_pipeline_options opt;
   uint32_t num_packs = 0, num_qpacks = 0, nq_cap = 0, rec_words_l = 0;
   uint32_t max_nq = 0;
+  uint32_t hist_map = 0, hist_compact = 0;  // labels of compact shell histograms
   uint64_t cap_s = 0, cap_seg = 0, cap_l = 0;
   uint64_t alloc_s = 0, alloc_seg = 0, alloc_l_words = 0;  // allocated record buffer sizes
   // measured records / segments per pack of each class, sizes the chunks
@@ -1135,6 +1171,17 @@ int bind_queries(bms2_pipeline_s* p, const bms2_device_csr* query) {
     return set_error(BMS2_ERR_HIP, "copy of the query batch failed");
   std::vector<uint32_t> label_freq(256, 0);
   for (uint8_t l : qlab) label_freq[l]++;
+  // shell histograms over the labels < 16 the queries carry, when there are at most 8 of them
+  {
+    std::vector<uint32_t> used;
+    for (uint32_t l = 0; l < 16; ++l)
+      if (label_freq[l]) used.push_back(l);
+    p->hist_compact = used.size() <= 8 ? 1 : 0;
+    p->hist_map = 0;
+    if (p->hist_compact)
+      for (uint32_t i = 0; i < 8; ++i)
+        p->hist_map |= (i < used.size() ? used[i] : (used.empty() ? 0 : used[0])) << (4 * i);
+  }
   std::vector<uint32_t> qhead(size_t(Q) * 4, 0), qpos(std::max<uint64_t>(Qn, 1), 0);
   for (uint32_t g = 0; g < Q; ++g) {
     const uint32_t b = qgo[g], n = qgo[g + 1] - qgo[g];
@@ -1346,7 +1393,9 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
   js.qnodes = tr.qnodes;
   js.pair_nodes = tr.pair_nodes;
   QueryPrepParams qp{p->query.graph_offsets, p->query.row_offsets, p->query.column_indices,
-                     p->query.node_labels, p->d_qpacks, p->d_qpos, p->d_qrec};
+                     p->query.node_labels, p->d_qpacks, p->d_qpos, p->d_qrec, p->hist_map,
+                     p->hist_compact};
+  sp.hist_map = p->hist_map;
   const size_t stage_bytes = std::max<size_t>(
       (size_t(kStageSmall) * kSmallWords + size_t(kStageLarge) * p->rec_words_l) * 8,
       64 * 8 * sizeof(uint32_t));
@@ -1362,7 +1411,10 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
     auto launch_chunk = [&](uint32_t p0, uint32_t p1) -> int {
       hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_ctl, uint32_t(kCtlWords));
       sp.pack_begin = p0;
-      hipLaunchKernelGGL(screen_kernel<K>, dim3(p1 - p0), dim3(64), stage_bytes, s, sp);
+      if (p->hist_compact)
+        hipLaunchKernelGGL((screen_kernel<K, 2>), dim3(p1 - p0), dim3(64), stage_bytes, s, sp);
+      else
+        hipLaunchKernelGGL((screen_kernel<K, 4>), dim3(p1 - p0), dim3(64), stage_bytes, s, sp);
       BMS2_CHECK_LAUNCH("screen_kernel");
       const dim3 gs(uint32_t(p->cus) * p->small_waves), gl(uint32_t(p->cus) * p->large_waves);
       if (p->slots == 4)

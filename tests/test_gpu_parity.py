@@ -205,3 +205,46 @@ def test_autotune_keeps_results(queue, uneven_2k):
     c, v = pipe.results(counts, valid)
     assert np.array_equal(c, rc) and np.array_equal(v, rv)
     assert np.array_equal(stats.to_numpy(np.uint64, 8)[:4], rs)
+
+
+def _random_labelled(rng, n_graphs, nmin, nmax, labels):
+    """Random connected graphs (spanning tree plus a few extra edges), labels uniform in labels."""
+    out = []
+    for _ in range(n_graphs):
+        n = int(rng.integers(nmin, nmax + 1))
+        edges = {(int(rng.integers(0, v)), v) for v in range(1, n)}
+        for _ in range(int(rng.integers(0, 4))):
+            a, b = sorted(int(x) for x in rng.choice(n, 2, replace=False))
+            edges.add((a, b))
+        out.append(([int(x) for x in rng.choice(labels, n)], sorted(edges)))
+    return out
+
+
+@pytest.mark.parametrize("qlabels", [8, 13])
+def test_shell_histogram_label_sets(queue, qlabels):
+    """Queries over at most 8 labels use 2-word shell levels restricted to those labels, more
+    labels the full 4-word levels; data graphs carry 13 labels either way."""
+    rng = np.random.default_rng(qlabels)
+    dgs = _random_labelled(rng, 300, 8, 40, np.arange(13))
+    data = graphs.GraphBatch.from_graphs(dgs)
+    # queries: connected pieces of data graphs relabelled into the first qlabels labels
+    qgs = []
+    for labs, edges in _random_labelled(rng, 60, 3, 7, np.arange(qlabels)):
+        qgs.append((labs, edges))
+    for g in range(0, 300, 10):  # induced pieces of data graphs keep matches
+        labs, edges = dgs[g]
+        keep = {0}
+        for a, b in edges:
+            if len(keep) < 5 and (a in keep) != (b in keep):
+                keep |= {a, b}
+        ids = sorted(keep)
+        pos = {v: i for i, v in enumerate(ids)}
+        ql = [min(labs[v], qlabels - 1) for v in ids]
+        qgs.append((ql, [(pos[a], pos[b]) for a, b in edges if a in pos and b in pos]))
+    queries = graphs.GraphBatch.from_graphs(qgs)
+    assert len(set(np.asarray(queries.node_labels).tolist())) == qlabels
+    for K in (2, 6):
+        counts, valid, stats = api.run_pipeline(queue, queries, data, K, False)
+        rc, rv, rs = oracle.pipeline(queries, data, K, False)
+        assert np.array_equal(valid, rv) and np.array_equal(counts, rc)
+        assert np.array_equal(stats[:4], rs)
