@@ -618,10 +618,12 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
   constexpr int NQ = kSmallNq;
   constexpr uint32_t kRing = 256;  // >= S * (largest segment)
   constexpr uint32_t kBatch = 16;  // idle lanes that trigger a handout
+  constexpr uint32_t kSegClaim = 8;  // segments claimed per atomic on the global cursor
   static_assert(S * kSegRecs <= int(kRing), "ring must hold every slot's segment");
   __shared__ uint64_t R[NQ][64];   // remaining candidates per depth
   __shared__ uint64_t NF[NQ][64];  // neighbour row of the node matched per depth
   __shared__ uint64_t D[NQ][64];   // domain per depth (matching order)
+  __shared__ uint64_t PC[NQ][64];  // per depth d: constraint on depth d + 1 from depths < d
   __shared__ uint64_t ADJ[S][64];  // adjacency rows of each slot's pack
   __shared__ uint32_t RING[kRing];  // queued record index | slot << 29
   __shared__ uint32_t USE[S];
@@ -632,6 +634,7 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
   if (lane < uint32_t(S)) USE[lane] = 0;
   __syncthreads();
   uint32_t qh = 0, qt = 0;  // ring head and tail (uniform)
+  uint64_t lseg = 0, lseg_end = 0;  // segments claimed by this wave, not yet in a slot
   bool exhausted = false, busy = false;
   uint32_t slot = 0, q = 0, g = 0, nq = 0;
   int depth = 0;
@@ -647,11 +650,17 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
 #pragma unroll
         for (int s = 0; s < S; ++s) {
           if (!exhausted && qt - qh < nidle + 64 && uniform(USE[s]) == 0) {
-            uint64_t si = 0;
-            if (lane == 0)
-              si = atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + kCtlSegCursor), 1ull);
-            si = uniform64(readlane64(si, 0));
-            if (si >= nseg) {
+            if (lseg >= lseg_end) {
+              uint64_t base = 0;
+              if (lane == 0)
+                base = atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + kCtlSegCursor),
+                                 (unsigned long long)kSegClaim);
+              base = uniform64(readlane64(base, 0));
+              lseg = base;
+              lseg_end = base >= nseg ? base : min(base + kSegClaim, nseg);
+            }
+            const uint64_t si = lseg++;
+            if (si >= lseg_end) {
               exhausted = true;
             } else {
               const uint4 sd = p.segs[si];
@@ -684,6 +693,7 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
 #pragma unroll
         for (int d = 0; d < NQ; ++d) D[d][lane] = w[2 + d];
         R[0][lane] = w[2];
+        PC[0][lane] = ~uint64_t(0);
         depth = 0;
         used = 0;
         cnt = 0;
@@ -697,34 +707,39 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
     }
     bool finished = false;
     if (busy) {
-      const uint64_t Rd = R[depth][lane];
+      // pop exhausted frames, then take the next candidate of the current frame
+      uint64_t Rd = R[depth][lane];
+      while (Rd == 0 && depth > 0) {
+        --depth;
+        used &= ~(uint64_t(1) << F[depth][lane]);
+        Rd = R[depth][lane];
+      }
       if (Rd == 0) {
-        if (depth == 0) {
-          finished = true;
-        } else {
-          --depth;
-          used &= ~(uint64_t(1) << F[depth][lane]);
-        }
+        finished = true;
       } else {
         const int v = ctz64(Rd);
         R[depth][lane] = Rd & (Rd - 1);
-        F[depth][lane] = uint8_t(v);
         const uint64_t nv = ADJ[slot][v];
-        NF[depth][lane] = nv;
         ++nodes;
         const int nd = depth + 1;
         const uint64_t vbit = uint64_t(1) << v;
-        uint64_t Rn = D[nd][lane] & ~(used | vbit);
         const uint32_t am = uint32_t(qa >> (8 * nd)) & 0xffu;
-        Rn &= ((am >> depth) & 1) ? nv : ~nv;
-        for (int i = 0; i < depth; ++i) {
-          const uint64_t a = NF[i][lane];
-          Rn &= ((am >> i) & 1) ? a : ~a;
-        }
+        const uint64_t Rn = D[nd][lane] & ~(used | vbit) & PC[depth][lane] &
+                            (((am >> depth) & 1) ? nv : ~nv);
         if (uint32_t(nd) == nq - 1) {
           cnt += uint64_t(popc64(Rn));
           if (p.find_first && cnt) finished = true;
         } else {
+          // descend: record the match, and the constraint on depth nd + 1 from depths <= depth
+          F[depth][lane] = uint8_t(v);
+          NF[depth][lane] = nv;
+          const uint32_t an = uint32_t(qa >> (8 * (nd + 1))) & 0xffu;
+          uint64_t c = ((an >> depth) & 1) ? nv : ~nv;
+          for (int i = 0; i < depth; ++i) {
+            const uint64_t x = NF[i][lane];
+            c &= ((an >> i) & 1) ? x : ~x;
+          }
+          PC[nd][lane] = c;
           used |= vbit;
           depth = nd;
           R[nd][lane] = Rn;
@@ -1299,8 +1314,8 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
   p->cus = cus;
-  // defaults: as many waves as LDS allows (small join ~18 KB per wave)
-  p->small_waves = opt->block_waves ? opt->block_waves : 8;
+  // defaults: as many waves as LDS allows (small join ~22 KB per wave: 7 per CU)
+  p->small_waves = opt->block_waves ? opt->block_waves : 7;
   p->slots = 8;
   if ((rc = bind_queries(p, query))) return fail(rc);
   *out = p;
@@ -1561,9 +1576,9 @@ extern "C" int bms2_pipeline_autotune(bms2_pipeline p, uint64_t* d_counts, uint6
   };
   std::vector<Cand> cands;
   const uint32_t lw0 = p->nq_cap <= 16 ? 12 : 6;
-  for (uint32_t sw : {4u, 6u, 8u})
+  for (uint32_t sw : {5u, 6u, 7u})
     for (uint32_t sl : {4u, 8u}) cands.push_back({sw, lw0, sl});
-  for (uint32_t lw : {lw0 / 2, lw0 * 3 / 4}) cands.push_back({8, lw, 8});
+  for (uint32_t lw : {lw0 / 2, lw0 * 3 / 4}) cands.push_back({7, lw, 8});
   const Cand keep{p->small_waves, p->large_waves, p->slots};
   float best = 0.f;
   Cand best_c = keep;
