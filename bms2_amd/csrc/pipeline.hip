@@ -36,10 +36,12 @@ constexpr int kSumDwords = 16;   // prefilter summary of one data graph
 constexpr int kSmallNq = 8;      // queries of up to 8 nodes use the small records and join
 constexpr int kSmallWords = 2 + kSmallNq;
 
-// Screening record layout (u32 words): [0] label | deg << 8 | local id << 16 | level mask << 24,
-// [1..8] path Bloom, [9..10] ring counts, [12 + 4k .. 15 + 4k] shell level k.  Shell words hold
-// the even / odd nibbles of the low and high histogram words, one nibble per byte, plus 15 per
-// byte: byte = q + 15, so (byte - d) has bit 4 set exactly when q > d.
+// Screening record layout (u32 words): [0] deg | label << 8 | local id << 16 | level mask << 24,
+// [1..8] path Bloom, [9..10] ring counts, [12 + HW k .. 11 + HW (k + 1)] shell level k (HW = 2
+// compact or 4 full words).  Shell words hold the even / odd nibbles of the histogram, one count
+// per byte; the screen keeps the data node's counts as 15 - d per byte, so one 64-bit add over
+// eight labels gives q + 15 - d per byte (at most 30: no carry between bytes), with bit 4 set
+// exactly when q > d.
 struct ScreenParams {
   const uint32_t* graph_offsets;
   const uint32_t* row_offsets;
@@ -148,7 +150,7 @@ __global__ __launch_bounds__(64) void query_prep_kernel(QueryPrepParams p) {
   for (int k = 0; k < kMaxLevels; ++k)
     if (t.hist[k]) lm |= 1u << k;
   uint32_t* r = p.qrec + size_t(p.rec_pos[node]) * kRecDwords;
-  r[0] = (t.label & 255u) | (min(t.deg, 255u) << 8) | (local << 16) | (lm << 24);
+  r[0] = min(t.deg, 255u) | ((t.label & 255u) << 8) | (local << 16) | (lm << 24);
 #pragma unroll
   for (int w = 0; w < 8; ++w) r[1 + w] = t.path[w];
   r[9] = uint32_t(t.cycle);
@@ -158,18 +160,18 @@ __global__ __launch_bounds__(64) void query_prep_kernel(QueryPrepParams p) {
 #pragma unroll
     for (int k = 0; k < kMaxLevels; ++k) {
       const uint32_t c = compact_hist(t.hist[k], p.hist_map);
-      r[12 + 2 * k + 0] = (c & 0x0f0f0f0fu) + 0x0f0f0f0fu;
-      r[12 + 2 * k + 1] = ((c >> 4) & 0x0f0f0f0fu) + 0x0f0f0f0fu;
+      r[12 + 2 * k + 0] = c & 0x0f0f0f0fu;
+      r[12 + 2 * k + 1] = (c >> 4) & 0x0f0f0f0fu;
     }
     for (int w = 12 + 2 * kMaxLevels; w < kRecDwords; ++w) r[w] = 0;
   } else {
 #pragma unroll
     for (int k = 0; k < kMaxLevels; ++k) {
       const uint32_t lo = uint32_t(t.hist[k]), hi = uint32_t(t.hist[k] >> 32);
-      r[12 + 4 * k + 0] = (lo & 0x0f0f0f0fu) + 0x0f0f0f0fu;
-      r[12 + 4 * k + 1] = ((lo >> 4) & 0x0f0f0f0fu) + 0x0f0f0f0fu;
-      r[12 + 4 * k + 2] = (hi & 0x0f0f0f0fu) + 0x0f0f0f0fu;
-      r[12 + 4 * k + 3] = ((hi >> 4) & 0x0f0f0f0fu) + 0x0f0f0f0fu;
+      r[12 + 4 * k + 0] = lo & 0x0f0f0f0fu;
+      r[12 + 4 * k + 1] = (lo >> 4) & 0x0f0f0f0fu;
+      r[12 + 4 * k + 2] = hi & 0x0f0f0f0fu;
+      r[12 + 4 * k + 3] = (hi >> 4) & 0x0f0f0f0fu;
     }
     for (int w = 12 + 4 * kMaxLevels; w < kRecDwords; ++w) r[w] = 0;
   }
@@ -186,7 +188,7 @@ __global__ void query_pre_kernel(const uint32_t* __restrict__ qhead, const uint3
   uint32_t out[kPreDwords] = {};
   for (uint32_t i = 0; i < nq; ++i) {
     const uint32_t* r = qrec + size_t(rb + i) * kRecDwords;
-    const uint32_t l = r[0] & 255u, d = (r[0] >> 8) & 255u;
+    const uint32_t d = r[0] & 255u, l = (r[0] >> 8) & 255u;
     if (l < 16) {
       const uint32_t bit = l * 8 + min(d, 7u);
       out[bit >> 5] |= 1u << (bit & 31);
@@ -210,6 +212,7 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
   __shared__ uint64_t adj[64];
   __shared__ uint8_t lbl[64];
   __shared__ __align__(16) uint32_t gsum_lds[kPackGraphs * kSumDwords];
+  __shared__ uint64_t qdom[BMS2_MAX_QUERY_NODES];  // domain of each query node (local id)
   extern __shared__ __align__(16) uint64_t stage[];  // staged records; Bloom scratch first
   const uint32_t lane = lane_id();
   const uint32_t pack = p.pack_begin + blockIdx.x;
@@ -224,28 +227,34 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
                     PackScratch{adj, lbl, reinterpret_cast<uint32_t*>(stage)}, t);
   if (lane < nn) p.adj_out[n0 + lane] = adj[lane];
 
-  const uint32_t d_label = lane < nn ? t.label : 0x100u;  // never equal to a u8 query label
-  const uint32_t d_deg = t.deg;
+  // label << 8 | degree (capped at 255); lanes past the pack get a key above every query's range
+  const uint32_t d_deg = min(t.deg, 255u);
+  const uint32_t d_key = lane < nn ? ((t.label & 255u) << 8) | d_deg : 0x10000u;
   uint32_t d_pnot[8];
 #pragma unroll
   for (int w = 0; w < 8; ++w) d_pnot[w] = ~t.path[w];
   // ring counts with the top bit of each u16 set: (d | 0x8000) - q has bit 15 set iff d >= q
   const uint32_t d_cyc0 = uint32_t(t.cycle) | 0x80008000u;
   const uint32_t d_cyc1 = uint32_t(t.cycle >> 32) | 0x80008000u;
-  uint32_t d_hist[K > 0 ? K : 1][HW];
+  // shell levels as 15 - d per byte, in the byte order of the query records' shell words
+  constexpr int HL = HW / 2;  // u64 words per shell level
+  constexpr uint64_t kFifteens = 0x0f0f0f0f0f0f0f0full;
+  uint64_t d_hist[K > 0 ? K : 1][HL];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if (HW == 2) {
       const uint32_t c = compact_hist(t.hist[k], p.hist_map);
-      d_hist[k][0] = c & 0x0f0f0f0fu;
-      d_hist[k][HW - 1] = (c >> 4) & 0x0f0f0f0fu;
+      d_hist[k][0] = kFifteens - ((c & 0x0f0f0f0fu) | (uint64_t((c >> 4) & 0x0f0f0f0fu) << 32));
     } else {
       const uint32_t lo = uint32_t(t.hist[k]), hi = uint32_t(t.hist[k] >> 32);
-      d_hist[k][0] = lo & 0x0f0f0f0fu;
-      d_hist[k][1 % HW] = (lo >> 4) & 0x0f0f0f0fu;
-      d_hist[k][2 % HW] = hi & 0x0f0f0f0fu;
-      d_hist[k][3 % HW] = (hi >> 4) & 0x0f0f0f0fu;
+      d_hist[k][0] = kFifteens - ((lo & 0x0f0f0f0fu) | (uint64_t((lo >> 4) & 0x0f0f0f0fu) << 32));
+      d_hist[k][HL - 1] =
+          kFifteens - ((hi & 0x0f0f0f0fu) | (uint64_t((hi >> 4) & 0x0f0f0f0fu) << 32));
     }
+    // opaque from here on: the test adds the query word to this value as it stands (left
+    // visible, the compiler rebuilds q + 15 - d from d with a borrow chain, twice the work)
+#pragma unroll
+    for (int j = 0; j < HL; ++j) asm volatile("" : "+v"(d_hist[k][j]));
   }
 
   // Graph segments: node lanes of a graph are contiguous.  mytop = last lane of the node's graph
@@ -340,7 +349,7 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
     atomicMax(gs + 15, uint32_t(t.cycle >> 48));
   }
   __syncthreads();
-  const bool root_lane = lane < nn && lane == mytop;
+  const uint32_t myslotbit = lane < nn ? 1u << myslot : 0u;  // this lane's graph in a gpass mask
 
   for (uint32_t qb = 0; qb < p.Q; qb += 64) {
     // 64 queries at once, one per lane, against every graph of the pack: every node's (label,
@@ -373,11 +382,13 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
       const uint32_t gm = __builtin_amdgcn_readlane(gpass, li);
       const uint32_t rb = __builtin_amdgcn_readlane(q_rb, li);
       const uint32_t nq = __builtin_amdgcn_readlane(q_nq, li);
-      const bool in_graph = lane < nn && ((gm >> myslot) & 1);
-      uint64_t lanes = ballot(in_graph);
-      uint64_t tops = ballot(in_graph && root_lane);
-      // domains by local query node id, lane u holding node u's (VGPRs)
-      uint32_t dom_lo = 0, dom_hi = 0;
+      // matching order and query adjacency, loaded ahead of the node loop
+      const uint32_t jl = lane < nq ? uint32_t(p.jorder[size_t(qu) * 32 + lane]) : 0u;
+      const uint64_t qa8 =
+          nq <= uint32_t(kSmallNq) ? ((const BMS2_CONST uint64_t*)(p.qadj8))[qu] : 0;
+      uint64_t lanes = ballot((gm & myslotbit) != 0);
+      uint64_t tops = lanes & T;
+      // domains by local query node id go to qdom (written by lane 0)
       const uint32_t* r = p.qrec + size_t(rb) * kRecDwords;
       for (uint32_t ti = 0; ti < nq; ++ti, r += kRecDwords) {
         // the whole record in one batch of scalar loads, then a branch-free test
@@ -387,29 +398,29 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
         // keep the record's loads in one batch ahead of the uniform branches below
 #pragma unroll
         for (int x = 0; x < 12 + HW * K; ++x) asm volatile("" ::"s"(w[x]));
-        const uint32_t ql = w[0] & 255u, qd = (w[0] >> 8) & 255u, local = (w[0] >> 16) & 255u;
-        const uint32_t lm = w[0] >> 24;  // shell levels with a non-empty query histogram
-        uint64_t m = ballot((d_label == ql || ql == kWildcard) && d_deg >= qd) & lanes;
+        const uint32_t qkey = w[0] & 0xffffu, qd = w[0] & 255u, local = (w[0] >> 16) & 255u;
+        // label equal (any label for the wildcard) and degree at least the query node's, each a
+        // single compare: key - (ql << 8 | qd) <= 255 - qd exactly when label == ql, deg >= qd
+        uint64_t m = ((qkey >> 8) == kWildcard ? ballot(d_deg >= qd)
+                                               : ballot(d_key - qkey <= 255u - qd)) & lanes;
         if (m) {
           uint32_t acc = 0;
 #pragma unroll
           for (int x = 0; x < 8; ++x) acc |= w[1 + x] & d_pnot[x];
           const uint32_t c0 = d_cyc0 - w[9], c1 = d_cyc1 - w[10];
           uint32_t fail = ((c0 & c1) & 0x80008000u) ^ 0x80008000u;  // a ring field below the query's
+          // every shell level at once (an empty query level adds 0 and passes every data node)
+          uint64_t sh = 0;
 #pragma unroll
           for (int k = 0; k < K; ++k)
-            if ((lm >> k) & 1) {  // an empty query shell passes every data node
-              uint32_t x = 0;
 #pragma unroll
-              for (int h = 0; h < HW; ++h) x |= w[12 + HW * k + h] - d_hist[k][h];
-              fail |= x & 0x10101010u;
-            }
+            for (int j = 0; j < HL; ++j)
+              sh |= (uint64_t(w[12 + HW * k + 2 * j]) | (uint64_t(w[13 + HW * k + 2 * j]) << 32)) +
+                    d_hist[k][j];
+          fail |= uint32_t(sh | (sh >> 32)) & 0x10101010u;
           m &= ballot((acc | fail) == 0);
         }
-        if (lane == local) {
-          dom_lo = uint32_t(m);
-          dom_hi = uint32_t(m >> 32);
-        }
+        if (lane == 0) qdom[local] = m;
         // Graphs left with a candidate: per segment, (m | top) - first has its top bit set
         // unless every non-top bit of m is clear, in which case m's own top bit decides.
         const uint64_t u = (m | T) - S;
@@ -422,12 +433,10 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
       }
       if (!tops) continue;
       if (lane < ng && slot_top < 64 && ((tops >> slot_top) & 1)) vbits |= uint64_t(1) << (qu & 63);
+      __syncthreads();  // lane 0's qdom writes visible to every lane (one wave: no waiting)
       // domains in matching order: lane d takes the domain of the node matched at depth d
-      const uint32_t jl = lane < nq ? uint32_t(p.jorder[size_t(qu) * 32 + lane]) : 0u;
-      const uint64_t jdom = (uint64_t(uint32_t(__shfl(int(dom_hi), int(jl)))) << 32) |
-                            uint32_t(__shfl(int(dom_lo), int(jl)));
-      const uint64_t own = (uint64_t(dom_hi) << 32) | dom_lo;  // lane = local node id
-      const uint64_t qa8 = nq <= uint32_t(kSmallNq) ? p.qadj8[qu] : 0;
+      const uint64_t jdom = lane < nq ? qdom[jl] : 0;
+      const uint64_t own = lane < nq ? qdom[lane] : 0;  // lane = local node id
       // GMCR pairs -> staged records {q | g << 32, n0 | nq << 32, domains in matching order}
       uint64_t at = tops;
       while (at) {
