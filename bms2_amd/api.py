@@ -26,6 +26,7 @@ from .graphs import GraphBatch
 
 SCOPE_DATA = _lib.SCOPE_DATA
 SCOPE_QUERY = _lib.SCOPE_QUERY
+PACK_WINDOW = 1024  # BMS2_PACK_WINDOW: data packs are planned per window of graphs
 
 
 def _L():
@@ -447,7 +448,8 @@ class Pipeline(_Handle):
         """Join traces over the leading data graphs (at least max_graphs of them): summed
         domain size per query node, GMCR pairs and DFS nodes per query, optionally DFS nodes per
         (graph, query) pair, plus the counts of the traced graphs."""
-        G = min(int(max_graphs) + 64, self.D)  # whole packs: at most 8 graphs past max_graphs
+        W = PACK_WINDOW  # whole pack windows: max_graphs rounded up to the window
+        G = min((int(max_graphs) + W - 1) // W * W, self.D)
         counts = DeviceBuffer(self.q, 8 * max(G * self.Q, 1), zero=False)
         valid = DeviceBuffer(self.q, 8 * max(G * self.VW, 1), zero=False)
         domsum = DeviceBuffer(self.q, 8 * max(query_nodes, 1), zero=True)

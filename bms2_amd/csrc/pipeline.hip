@@ -47,7 +47,8 @@ struct ScreenParams {
   const uint32_t* row_offsets;
   const uint32_t* column_indices;
   const uint8_t* labels;
-  const uint32_t* pack_gbegin;
+  const uint32_t* pack_gbegin;  // [P + 1] first entry of each pack in pack_gids
+  const uint32_t* pack_gids;    // graph ids, pack by pack in lane order
   uint32_t pack_begin;
   uint32_t Q;
   const uint32_t* qhead;    // [Q][4]: first record, nq, valid (nq > 1, gmcr.hpp:64), 0
@@ -62,7 +63,7 @@ struct ScreenParams {
   uint32_t VW;
   uint64_t* recs_s;         // small-query records (nq <= kSmallNq), kSmallWords u64 each
   uint64_t cap_s;
-  uint4* segs;              // one descriptor per flush of small records: {first, count, n0, nn}
+  uint4* segs;              // one descriptor per flush of small records: {first, count, rows, nn}
   uint64_t cap_seg;
   uint64_t* recs_l;         // large-query records, rec_words_l u64 each
   uint64_t cap_l;
@@ -218,14 +219,33 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
   const uint32_t pack = p.pack_begin + blockIdx.x;
   const uint32_t gb = p.pack_gbegin[pack], ge = p.pack_gbegin[pack + 1];
   const uint32_t ng = ge - gb;
-  const uint32_t n0 = p.graph_offsets[gb], nn = p.graph_offsets[ge] - n0;
+  // Graph slot j (graph pack_gids[gb + j]) occupies the next run of lanes.  mytop = last lane of
+  // the node's graph (64 for lanes past the pack), base = its first node - its first lane;
+  // graph-slot lanes j < ng hold the graph id and its top lane.
+  uint32_t nn = 0, mytop = 64, mygraph = 0, myslot = 0, slot_top = 64, slot_gid = 0, base = 0;
+  for (uint32_t j = 0; j < ng; ++j) {
+    const uint32_t g = p.pack_gids[gb + j];
+    const uint32_t g0 = p.graph_offsets[g], e = nn + (p.graph_offsets[g + 1] - g0);
+    if (lane >= nn && lane < e) {
+      mytop = e - 1;
+      mygraph = g;
+      myslot = j;
+      base = g0 - nn;
+    }
+    if (lane == j) {
+      slot_gid = g;
+      slot_top = e > nn ? e - 1 : 64;
+    }
+    nn = e;
+  }
   if (nn > 64) return;  // a graph above 64 nodes: the staged path serves it
   const uint64_t node_mask = low_mask(nn);
+  const uint32_t rows = pack * 64;  // the pack's adjacency rows, one per lane
 
   Tuple t;
-  compute_tuples<K>(n0, nn, p.row_offsets, p.column_indices, p.labels,
+  compute_tuples<K>(base, nn, p.row_offsets, p.column_indices, p.labels,
                     PackScratch{adj, lbl, reinterpret_cast<uint32_t*>(stage)}, t);
-  if (lane < nn) p.adj_out[n0 + lane] = adj[lane];
+  if (lane < nn) p.adj_out[rows + lane] = adj[lane];
 
   // label << 8 | degree (capped at 255); lanes past the pack get a key above every query's range
   const uint32_t d_deg = min(t.deg, 255u);
@@ -257,21 +277,6 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
     for (int j = 0; j < HL; ++j) asm volatile("" : "+v"(d_hist[k][j]));
   }
 
-  // Graph segments: node lanes of a graph are contiguous.  mytop = last lane of the node's graph
-  // (64 for lanes past the pack); graph-slot lanes j < ng hold the graph id and its top lane.
-  uint32_t mytop = 64, mygraph = 0, myslot = 0, slot_top = 64, slot_gid = 0;
-  for (uint32_t j = 0; j < ng; ++j) {
-    const uint32_t s = p.graph_offsets[gb + j] - n0, e = p.graph_offsets[gb + j + 1] - n0;
-    if (lane >= s && lane < e) {
-      mytop = e - 1;
-      mygraph = gb + j;
-      myslot = j;
-    }
-    if (lane == j) {
-      slot_gid = gb + j;
-      slot_top = e > s ? e - 1 : 64;
-    }
-  }
   const uint64_t T = ballot(lane < nn && lane == mytop);  // top lane of every non-empty graph
   const uint64_t S = ((T << 1) | 1) & node_mask;           // first lane of every non-empty graph
 
@@ -303,7 +308,7 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
         for (uint32_t w = lane; w < ns * kSmallWords; w += 64) dst[w] = stage_s[w];
         if (lane < nsg)
           p.segs[si + lane] = make_uint4(uint32_t(base) + lane * kSegRecs,
-                                         min(uint32_t(kSegRecs), ns - lane * kSegRecs), n0, nn);
+                                         min(uint32_t(kSegRecs), ns - lane * kSegRecs), rows, nn);
       }
     }
     if (nl) {
@@ -437,7 +442,7 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
       // domains in matching order: lane d takes the domain of the node matched at depth d
       const uint64_t jdom = lane < nq ? qdom[jl] : 0;
       const uint64_t own = lane < nq ? qdom[lane] : 0;  // lane = local node id
-      // GMCR pairs -> staged records {q | g << 32, n0 | nq << 32, domains in matching order}
+      // GMCR pairs -> staged records {q | g << 32, rows | nq << 32, domains in matching order}
       uint64_t at = tops;
       while (at) {
         const int tb = ctz64(at);
@@ -449,8 +454,8 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
         if (lane == 0) {
           rec[0] = uint64_t(qu) | (uint64_t(g) << 32);
           // small records carry the query adjacency by depth (its byte 0 is always 0) and nq;
-          // large records the pack's first node and nq
-          rec[1] = small ? (qa8 | nq) : (uint64_t(n0) | (uint64_t(nq) << 32));
+          // large records the pack's first adjacency row and nq
+          rec[1] = small ? (qa8 | nq) : (uint64_t(rows) | (uint64_t(nq) << 32));
         }
         if (lane < nq) rec[2 + lane] = jdom & seg;
         if (p.domsum) {
@@ -602,7 +607,7 @@ __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
 
 struct JoinSmallParams {
   const uint64_t* recs;     // small-query records, kSmallWords u64 each
-  const uint4* segs;        // {first record, count, n0, nn} per flush
+  const uint4* segs;        // {first record, count, first adjacency row, nn} per flush
   uint64_t cap_seg;
   uint32_t Q;
   const uint64_t* adj;      // [Dn] adjacency rows over pack lanes
@@ -810,6 +815,78 @@ std::vector<uint32_t> build_packs(const std::vector<uint32_t>& graph_offsets) {
   return begins;
 }
 
+// Pack plan of a resident data batch: packs of whole graphs with at most 64 nodes and
+// kPackGraphs graphs, listed as graph ids (pack_gids[begins[p] ..]) in lane order.
+struct DataPacks {
+  std::vector<uint32_t> begins, gids;
+  // window w covers graphs [win_graph[w], win_graph[w + 1]) in packs [win_pack[w], win_pack[w + 1])
+  std::vector<uint32_t> win_pack, win_graph;
+};
+
+// Best-fit decreasing inside windows of kPackWindow consecutive graphs: a window's graphs, largest
+// first, each go to the open pack with the least room that still holds them (a new pack when none
+// does).  Filling packs to ~98% of their lanes instead of the ~78% of packing consecutive
+// molecules cuts the screen's waves and query-node tests; the windows keep every pack's graphs
+// close in id (locality of the count rows) and give the plan prefixes that cover exactly the
+// graphs below a window edge.  A graph above 64 nodes gets a pack of its own, which the screen
+// skips.
+constexpr uint32_t kPackWindow = BMS2_PACK_WINDOW;
+DataPacks build_data_packs(const std::vector<uint32_t>& go) {
+  const uint32_t G = uint32_t(go.size() - 1);
+  DataPacks d;
+  d.gids.reserve(G);
+  std::vector<uint32_t> ids;
+  struct Bin {
+    uint32_t nodes;
+    std::vector<uint32_t> g;
+  };
+  std::vector<Bin> bins;
+  std::vector<std::vector<uint32_t>> by_room(65);  // open bins by free lanes
+  for (uint32_t w0 = 0; w0 < G; w0 += kPackWindow) {
+    const uint32_t w1 = std::min(G, w0 + kPackWindow);
+    d.win_pack.push_back(uint32_t(d.begins.size()));
+    d.win_graph.push_back(w0);
+    ids.clear();
+    for (uint32_t g = w0; g < w1; ++g) ids.push_back(g);
+    std::stable_sort(ids.begin(), ids.end(), [&](uint32_t a, uint32_t b) {
+      return go[a + 1] - go[a] > go[b + 1] - go[b];
+    });
+    bins.clear();
+    for (auto& v : by_room) v.clear();
+    for (uint32_t g : ids) {
+      const uint32_t n = go[g + 1] - go[g];
+      if (n > 64) {  // alone, emitted in place
+        d.begins.push_back(uint32_t(d.gids.size()));
+        d.gids.push_back(g);
+        continue;
+      }
+      uint32_t b = UINT32_MAX;
+      for (uint32_t r = n; r <= 64 && b == UINT32_MAX; ++r)
+        if (!by_room[r].empty()) {
+          b = by_room[r].back();
+          by_room[r].pop_back();
+        }
+      if (b == UINT32_MAX) {
+        b = uint32_t(bins.size());
+        bins.push_back(Bin{0, {}});
+      }
+      Bin& bin = bins[b];
+      bin.nodes += n;
+      bin.g.push_back(g);
+      if (bin.g.size() < size_t(kPackGraphs) && bin.nodes < 64) by_room[64 - bin.nodes].push_back(b);
+    }
+    for (Bin& bin : bins) {
+      std::sort(bin.g.begin(), bin.g.end());
+      d.begins.push_back(uint32_t(d.gids.size()));
+      d.gids.insert(d.gids.end(), bin.g.begin(), bin.g.end());
+    }
+  }
+  d.win_pack.push_back(uint32_t(d.begins.size()));
+  d.win_graph.push_back(G);
+  d.begins.push_back(uint32_t(d.gids.size()));
+  return d;
+}
+
 // Data graphs above 64 nodes (staged path): zero their count and GMCR rows.
 __global__ void big_zero_kernel(const uint32_t* __restrict__ big_gid, uint32_t nbig, uint32_t Q,
                                 uint32_t VW, uint32_t g_limit, uint64_t* __restrict__ counts,
@@ -897,7 +974,9 @@ struct bms2_pipeline_s {
   // launch shape (autotuned): waves per CU of each join, LDS slots per small-join wave
   int cus = 256;
   uint32_t small_waves = 8, large_waves = 12, slots = 8;
-  uint32_t* d_packs = nullptr;
+  uint32_t* d_packs = nullptr;      // pack p: graphs d_pack_gids[d_packs[p] .. d_packs[p + 1])
+  uint32_t* d_pack_gids = nullptr;
+  std::vector<uint32_t> win_pack, win_graph;  // pack windows (see build_data_packs)
   uint32_t* d_qpacks = nullptr;
   uint32_t* d_qhead = nullptr;
   uint32_t* d_qpre = nullptr;
@@ -938,6 +1017,7 @@ namespace {
 void pipeline_free(bms2_pipeline p) {
   if (!p) return;
   (void)hipFree(p->d_packs);
+  (void)hipFree(p->d_pack_gids);
   (void)hipFree(p->d_qpacks);
   (void)hipFree(p->d_qhead);
   (void)hipFree(p->d_qpre);
@@ -1310,14 +1390,22 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
   if (hipMemcpy(go.data(), data->graph_offsets, 4ull * go.size(), hipMemcpyDeviceToHost) !=
       hipSuccess)
     return fail(set_error(BMS2_ERR_HIP, "copy of data graph offsets failed"));
-  std::vector<uint32_t> packs = build_packs(go);
-  p->num_packs = uint32_t(packs.size() - 1);
+  DataPacks dp = build_data_packs(go);
+  p->num_packs = uint32_t(dp.begins.size() - 1);
+  p->win_pack = std::move(dp.win_pack);
+  p->win_graph = std::move(dp.win_graph);
   if ((rc = build_big_batch(p, go))) return fail(rc);
-  if ((rc = device_alloc((void**)&p->d_packs, 4ull * packs.size())) ||
-      (rc = device_alloc((void**)&p->d_adj, 8ull * std::max<uint64_t>(data->total_nodes, 1))) ||
+  if ((rc = device_alloc((void**)&p->d_packs, 4ull * dp.begins.size())) ||
+      (rc = device_alloc((void**)&p->d_pack_gids, 4ull * std::max<size_t>(dp.gids.size(), 1))) ||
+      (rc = device_alloc((void**)&p->d_adj, 8ull * 64 * std::max<uint32_t>(p->num_packs, 1))) ||
       (rc = device_alloc((void**)&p->d_ctl, 8 * kCtlWords)))
     return fail(rc);
-  BMS2_HIP(hipMemcpyAsync(p->d_packs, packs.data(), 4ull * packs.size(), hipMemcpyHostToDevice, s));
+  BMS2_HIP(hipMemcpyAsync(p->d_packs, dp.begins.data(), 4ull * dp.begins.size(),
+                          hipMemcpyHostToDevice, s));
+  if (!dp.gids.empty())
+    BMS2_HIP(hipMemcpyAsync(p->d_pack_gids, dp.gids.data(), 4ull * dp.gids.size(),
+                            hipMemcpyHostToDevice, s));
+  BMS2_HIP(hipStreamSynchronize(s));  // the plan's host vectors end with this function
   int dev = 0, cus = 0;
   (void)hipGetDevice(&dev);
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
@@ -1373,6 +1461,7 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
   sp.column_indices = p->data.column_indices;
   sp.labels = p->data.node_labels;
   sp.pack_gbegin = p->d_packs;
+  sp.pack_gids = p->d_pack_gids;
   sp.Q = Q;
   sp.qhead = p->d_qhead;
   sp.qpre = p->d_qpre;
@@ -1560,16 +1649,15 @@ extern "C" int bms2_pipeline_trace(bms2_pipeline p, uint32_t max_graphs, uint64_
   if (int rc = use_device(p->q)) return rc;
   if (!d_counts || !d_valid || !d_domsum || !d_qpairs || !d_qnodes)
     return set_error(BMS2_ERR_INVALID, "null trace buffer");
-  // leading packs covering max_graphs graphs
-  std::vector<uint32_t> packs(p->num_packs + 1);
-  BMS2_HIP(hipMemcpy(packs.data(), p->d_packs, 4ull * packs.size(), hipMemcpyDeviceToHost));
-  uint32_t pe = 0;
-  while (pe < p->num_packs && packs[pe] < max_graphs) ++pe;
-  if (graphs_out) *graphs_out = packs[pe];
+  // leading pack windows covering at least max_graphs graphs (all of them when max_graphs is)
+  size_t w = 0;
+  while (w + 1 < p->win_graph.size() && p->win_graph[w] < max_graphs) ++w;
+  const uint32_t pe = p->win_pack[w], ge = p->win_graph[w];
+  if (graphs_out) *graphs_out = ge;
   Trace tr{d_domsum, d_qpairs, d_qnodes, d_pair_nodes};
   EventScope ks{p->q, nullptr};
   uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (int rc = run_packs(p, pe, packs[pe], d_counts, d_valid, tr, ks, stats)) return rc;
+  if (int rc = run_packs(p, pe, ge, d_counts, d_valid, tr, ks, stats)) return rc;
   BMS2_HIP(hipStreamSynchronize(p->q->stream));
   return BMS2_OK;
 }

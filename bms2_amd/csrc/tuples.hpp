@@ -24,10 +24,12 @@ struct PackScratch {
   uint32_t* bloom;  // [64 * 8] per-lane Bloom words
 };
 
-// Computes the tuple of lane `lane` (< nn) of a pack starting at global node n0.  Must be called
-// by the whole wave (contains barriers and ballots).  Levels >= K are left zero.
+// Computes the tuple of lane `lane` (< nn) of a pack.  `base` is per lane: the lane's node is
+// global node base + lane, and every neighbour v (same graph) sits in lane v - base, so a graph
+// may occupy any run of lanes (base = its first node - its first lane).  Must be called by the
+// whole wave (contains barriers and ballots).  Levels >= K are left zero.
 template <int K>
-__device__ inline void compute_tuples(uint32_t n0, uint32_t nn, const uint32_t* __restrict__ ro,
+__device__ inline void compute_tuples(uint32_t base, uint32_t nn, const uint32_t* __restrict__ ro,
                                       const uint32_t* __restrict__ ci,
                                       const uint8_t* __restrict__ labels, PackScratch s, Tuple& t) {
   const uint32_t lane = lane_id();
@@ -37,11 +39,11 @@ __device__ inline void compute_tuples(uint32_t n0, uint32_t nn, const uint32_t* 
   t.deg = 0;
   bool multi = false;
   if (active) {
-    const uint32_t u = n0 + lane;
+    const uint32_t u = base + lane;
     t.label = labels[u];
     const uint32_t b = ro[u], e = ro[u + 1];
     t.deg = e - b;
-    for (uint32_t k = b; k < e; ++k) row |= uint64_t(1) << (ci[k] - n0);
+    for (uint32_t k = b; k < e; ++k) row |= uint64_t(1) << (ci[k] - base);
     multi = uint32_t(popc64(row)) != t.deg;  // parallel edges
   }
   s.adj[lane] = row;
@@ -136,37 +138,37 @@ __device__ inline void compute_tuples(uint32_t n0, uint32_t nn, const uint32_t* 
       }
     } else {
       // Parallel edges: enumerate CSR entries so every walk counts with its multiplicity.
-      const uint32_t sg = n0 + lane;
+      const uint32_t sg = base + lane;
       for (uint32_t k1 = ro[sg]; k1 < ro[sg + 1]; ++k1) {
-        const uint32_t v1 = ci[k1] - n0;
+        const uint32_t v1 = ci[k1] - base;
         if (v1 == lane) continue;
         const bool s1 = s.lbl[v1] == ls;
         const uint64_t m1 = m0 | (uint64_t(1) << v1);
-        for (uint32_t k2 = ro[n0 + v1]; k2 < ro[n0 + v1 + 1]; ++k2) {
-          const uint32_t v2 = ci[k2] - n0;
+        for (uint32_t k2 = ro[base + v1]; k2 < ro[base + v1 + 1]; ++k2) {
+          const uint32_t v2 = ci[k2] - base;
           if ((m1 >> v2) & 1) continue;
           const bool s2 = s1 && s.lbl[v2] == ls;
           const uint64_t m2 = m1 | (uint64_t(1) << v2);
-          for (uint32_t k3 = ro[n0 + v2]; k3 < ro[n0 + v2 + 1]; ++k3) {
-            const uint32_t v3 = ci[k3] - n0;
+          for (uint32_t k3 = ro[base + v2]; k3 < ro[base + v2 + 1]; ++k3) {
+            const uint32_t v3 = ci[k3] - base;
             if ((m2 >> v3) & 1) continue;
             const bool s3 = s2 && s.lbl[v3] == ls;
             const uint64_t m3 = m2 | (uint64_t(1) << v3);
-            for (uint32_t k4 = ro[n0 + v3]; k4 < ro[n0 + v3 + 1]; ++k4) {
-              const uint32_t v4 = ci[k4] - n0;
+            for (uint32_t k4 = ro[base + v3]; k4 < ro[base + v3 + 1]; ++k4) {
+              const uint32_t v4 = ci[k4] - base;
               if ((m3 >> v4) & 1) continue;
               const bool s4 = s3 && s.lbl[v4] == ls;
               const uint64_t m4 = m3 | (uint64_t(1) << v4);
-              for (uint32_t k5 = ro[n0 + v4]; k5 < ro[n0 + v4 + 1]; ++k5) {
-                const uint32_t v5 = ci[k5] - n0;
+              for (uint32_t k5 = ro[base + v4]; k5 < ro[base + v4 + 1]; ++k5) {
+                const uint32_t v5 = ci[k5] - base;
                 if (v5 == lane) {
                   if (s4) ++c5s; else ++c5d;
                   continue;
                 }
                 if ((m4 >> v5) & 1) continue;
                 const bool s5 = s4 && s.lbl[v5] == ls;
-                for (uint32_t k6 = ro[n0 + v5]; k6 < ro[n0 + v5 + 1]; ++k6)
-                  if (ci[k6] - n0 == lane) {
+                for (uint32_t k6 = ro[base + v5]; k6 < ro[base + v5 + 1]; ++k6)
+                  if (ci[k6] - base == lane) {
                     if (s5) ++c6s; else ++c6d;
                   }
               }

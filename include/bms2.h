@@ -298,8 +298,12 @@ int bms2_pipeline_autotune(bms2_pipeline p, uint64_t* d_counts, uint64_t* d_vali
  * unused alternative. */
 int bms2_pipeline_set_order(bms2_pipeline p, const uint8_t* order);
 
-/* Join traces over the data graphs of the leading packs (at least max_graphs graphs, fewer when
- * the batch is smaller): the pipeline runs on graphs [0, *graphs_out) and accumulates, per query
+/* Data packs are planned in windows of BMS2_PACK_WINDOW consecutive data graphs. */
+#define BMS2_PACK_WINDOW 1024u
+
+/* Join traces over the data graphs of the leading pack windows: the pipeline runs on graphs
+ * [0, *graphs_out), where *graphs_out is max_graphs rounded up to a multiple of BMS2_PACK_WINDOW
+ * (all graphs when the batch is smaller), and accumulates, per query
  * node (global query node id), the summed domain size over its GMCR pairs (d_domsum u64[Qn]),
  * per query the number of GMCR pairs (d_qpairs u64[Q]) and DFS nodes expanded (d_qnodes u64[Q]),
  * and per pair the DFS nodes (d_pair_nodes u32[graphs][Q], saturating, nullable).  The caller

@@ -96,11 +96,14 @@ def test_trace_prefix(queue, molecules_2k):
     pipe = api.Pipeline(queue, api.DeviceBatchedCSRGraph(queue, queries),
                         api.DeviceBatchedCSRGraph(queue, data), refinement_steps=2)
     tr = pipe.trace(500, queries.total_nodes)
-    assert 500 <= tr["graphs"] < 500 + 8
+    assert tr["graphs"] == api.PACK_WINDOW  # whole pack windows
     rc, _, rs = oracle.pipeline(queries, data, 2, False, g_end=tr["graphs"])
     assert np.array_equal(tr["counts"], rc)
     assert int(tr["qpairs"].sum()) == int(rs[0])
     assert int(tr["qnodes"].sum()) == int(rs[3])
+    # past the first window: the rest of the 2,000-graph batch
+    tr = pipe.trace(api.PACK_WINDOW + 1, queries.total_nodes)
+    assert tr["graphs"] == data.num_graphs
 
 
 def _rand_batch(rows, rng):
