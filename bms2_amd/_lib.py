@@ -11,6 +11,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libbms2.so"
 
 OK = 0
+ERR_CAPACITY = 7  # BMS2_ERR_CAPACITY: caller buffer too small
 SCOPE_DATA = 0
 SCOPE_QUERY = 1
 WILDCARD_NODE = 255
@@ -153,6 +154,10 @@ PROTOTYPES: dict[str, tuple] = {
     "bms2_gmcr_device_ptrs": (i32, [vp, vpp, vpp]),
     "bms2_gmcr_copy_to_host": (i32, [vp, vp, vp]),
     "bms2_join_candidates": (i32, [vp, DCP, DCP, vp, vp, vp, i32, vp, vpp]),
+    "bms2_join_partial_candidates": (i32, [vp, DCP, DCP, vp, u32, vp, u64, C.POINTER(C.c_uint64),
+                                           vpp]),
+    "bms2_join_with_partial_matches": (i32, [vp, DCP, DCP, vp, vp, u32, vp, u64, vp, i32, vp,
+                                             vpp]),
     "bms2_pipeline_create": (i32, [vp, DCP, DCP, C.POINTER(PipelineOptions), vpp]),
     "bms2_pipeline_destroy": (i32, [vp]),
     "bms2_pipeline_run": (i32, [vp, vp, vp, vp, vpp, vpp]),

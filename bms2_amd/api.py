@@ -59,9 +59,16 @@ def _ev(ptr):
 
 
 class Queue:
-    """A HIP stream on one device (sycl::queue)."""
+    """A HIP stream on one device (sycl::queue).
+
+    Device objects created on a queue (buffers, graph batches, handles) count themselves against
+    it while alive: close() on a queue that still has some only marks it, and the last of them to
+    be freed destroys it, so no object is ever freed through a destroyed queue, whatever order the
+    garbage collector finalises them in."""
 
     def __init__(self, device: int = 0, stream: int | None = None):
+        self._deps = 0
+        self._closing = False
         h = C.c_void_p()
         if stream is None:
             check(_L().bms2_queue_create(device, C.byref(h)))
@@ -69,6 +76,19 @@ class Queue:
             check(_L().bms2_queue_wrap(device, C.c_void_p(stream), C.byref(h)))
         self.h = h
         self.device = device
+
+    def _acquire(self) -> None:
+        self._deps += 1
+
+    def _release(self) -> None:
+        self._deps -= 1
+        if self._closing and self._deps <= 0:
+            self._destroy()
+
+    def _destroy(self) -> None:
+        if self.h:
+            h, self.h = self.h, None
+            check(_L().bms2_queue_destroy(h))
 
     def wait(self) -> None:
         check(_L().bms2_queue_wait(self.h))
@@ -78,9 +98,9 @@ class Queue:
         return int(_L().bms2_queue_stream(self.h) or 0)
 
     def close(self) -> None:
-        if self.h:
-            check(_L().bms2_queue_destroy(self.h))
-            self.h = None
+        self._closing = True
+        if self._deps <= 0:
+            self._destroy()
 
     def __del__(self):
         try:
@@ -98,6 +118,7 @@ class DeviceBuffer:
         p = C.c_void_p()
         check(_L().bms2_device_malloc(q.h, max(self.nbytes, 8), C.byref(p)))
         self.ptr = p.value
+        q._acquire()
         if zero:
             check(_L().bms2_memset_zero(q.h, C.c_void_p(self.ptr), max(self.nbytes, 8)))
 
@@ -123,8 +144,11 @@ class DeviceBuffer:
 
     def free(self) -> None:
         if self.ptr:
-            check(_L().bms2_device_free(self.q.h, C.c_void_p(self.ptr)))
-            self.ptr = None
+            ptr, self.ptr = self.ptr, None
+            try:
+                check(_L().bms2_device_free(self.q.h, C.c_void_p(ptr)))
+            finally:
+                self.q._release()
 
     def __del__(self):
         try:
@@ -141,6 +165,7 @@ class DeviceBatchedCSRGraph:
         self.c = _lib.DeviceCsr()
         h = host.to_c()
         check(_L().bms2_device_csr_create(q.h, C.byref(h), C.byref(self.c)))
+        q._acquire()
         self.host = host
 
     @property
@@ -160,7 +185,11 @@ class DeviceBatchedCSRGraph:
 
     def destroy(self) -> None:
         if self.c.graph_offsets:
-            check(_L().bms2_device_csr_destroy(self.q.h, C.byref(self.c)))
+            try:
+                check(_L().bms2_device_csr_destroy(self.q.h, C.byref(self.c)))
+            finally:
+                self.c.graph_offsets = None
+                self.q._release()
 
     def __del__(self):
         try:
@@ -174,16 +203,23 @@ def create_device_csr_graph(q: Queue, host: GraphBatch) -> DeviceBatchedCSRGraph
 
 
 class _Handle:
+    """An opaque library handle on queue self.q, destroyed with the Python object."""
+
     _destroy = ""
+
+    def _bind(self, h) -> None:
+        self.h = h
+        self.q._acquire()
 
     def __del__(self):
         h = getattr(self, "h", None)
         if h:
+            self.h = None
             try:
                 getattr(_L(), self._destroy)(h)
             except Exception:
                 pass
-            self.h = None
+            self.q._release()
 
 
 class Signature(_Handle):
@@ -195,7 +231,7 @@ class Signature(_Handle):
         self.q, self.n = q, (int(data_nodes), int(query_nodes))
         h = C.c_void_p()
         check(_L().bms2_signature_create(q.h, data_nodes, query_nodes, C.byref(h)))
-        self.h = h
+        self._bind(h)
 
     def _gen(self, g, scope, event):
         e = _ev_out(event)
@@ -234,7 +270,7 @@ class PathSignature(_Handle):
         self.q, self.n = q, (int(data_nodes), int(query_nodes))
         h = C.c_void_p()
         check(_L().bms2_path_signature_create(q.h, data_nodes, query_nodes, C.byref(h)))
-        self.h = h
+        self._bind(h)
 
     def _gen(self, g, scope, event):
         e = _ev_out(event)
@@ -261,7 +297,7 @@ class CycleSignature(_Handle):
         self.q, self.n = q, (int(data_nodes), int(query_nodes))
         h = C.c_void_p()
         check(_L().bms2_cycle_signature_create(q.h, data_nodes, query_nodes, C.byref(h)))
-        self.h = h
+        self._bind(h)
 
     def _gen(self, g, scope, event):
         e = _ev_out(event)
@@ -291,7 +327,7 @@ class Candidates(_Handle):
         self.rows = int(source_nodes)
         h = C.c_void_p()
         check(_L().bms2_candidates_create(q.h, source_nodes, target_nodes, C.byref(h)))
-        self.h = h
+        self._bind(h)
         self.W = int(_L().bms2_candidates_words_per_row(h))
 
     def host(self) -> np.ndarray:
@@ -327,7 +363,7 @@ class GMCR(_Handle):
         self.q = q
         h = C.c_void_p()
         check(_L().bms2_gmcr_create(q.h, C.byref(h)))
-        self.h = h
+        self._bind(h)
         self.num_data = 0
 
     def generate_gmcr(self, query, data, cand: Candidates, event=False):
@@ -360,6 +396,43 @@ def join_candidates(q: Queue, query, data, cand: Candidates, gmcr: GMCR, num_mat
     return _ev(e)
 
 
+def join_partial_candidates(q: Queue, query, data, cand: Candidates, end: int,
+                            capacity: int | None = None, event=False):
+    """joinPartialCandidates2 (the --use-cs step 1): every embedding of query 0's first `end`
+    nodes, as device records of end + 1 int32 {mapped global data node ids, data graph id}.
+    Returns (records DeviceBuffer, count[, Event]).  With capacity None the count is taken first
+    and the buffer sized to it."""
+    n = C.c_uint64(0)
+    if capacity is None:
+        rc = _L().bms2_join_partial_candidates(q.h, C.byref(query.c), C.byref(data.c), cand.h,
+                                               int(end), None, 0, C.byref(n), None)
+        if rc not in (_lib.OK, _lib.ERR_CAPACITY):
+            check(rc)
+        capacity = int(n.value)
+    buf = DeviceBuffer(q, 4 * (int(end) + 1) * max(int(capacity), 1), zero=False)
+    e = _ev_out(event)
+    check(_L().bms2_join_partial_candidates(q.h, C.byref(query.c), C.byref(data.c), cand.h,
+                                            int(end), C.c_void_p(buf.ptr), int(capacity),
+                                            C.byref(n), C.byref(e) if e is not None else None))
+    if event:
+        return buf, int(n.value), _ev(e)
+    return buf, int(n.value)
+
+
+def join_with_partial_matches(q: Queue, query, data, cand: Candidates, gmcr: GMCR, end: int,
+                              partial: DeviceBuffer, num_partial: int, num_matches: DeviceBuffer,
+                              pair_counts: DeviceBuffer, find_first: bool = True, event=False):
+    """joinWithPartialMatches (the --use-cs step 2): adds the total into the device u64
+    `num_matches`; `pair_counts` (u64 per GMCR entry) receives the counts (0/1 flags when
+    find_first)."""
+    e = _ev_out(event)
+    check(_L().bms2_join_with_partial_matches(
+        q.h, C.byref(query.c), C.byref(data.c), cand.h, gmcr.h, int(end), C.c_void_p(partial.ptr),
+        int(num_partial), C.c_void_p(num_matches.ptr), int(find_first),
+        C.c_void_p(pair_counts.ptr), C.byref(e) if e is not None else None))
+    return _ev(e)
+
+
 class Pipeline(_Handle):
     """The fused stage sequence over a resident data batch."""
 
@@ -379,7 +452,7 @@ class Pipeline(_Handle):
         h = C.c_void_p()
         check(_L().bms2_pipeline_create(q.h, C.byref(query.c), C.byref(data.c), C.byref(o),
                                         C.byref(h)))
-        self.h = h
+        self._bind(h)
         self.VW = (self.Q + 63) // 64
 
     def alloc_outputs(self) -> tuple[DeviceBuffer, DeviceBuffer, DeviceBuffer]:
@@ -473,11 +546,13 @@ class Pipeline(_Handle):
 
 
 def run_staged(q: Queue, query: GraphBatch, data: GraphBatch, K: int, find_first: bool = False,
-               keep_domains: bool = True):
+               keep_domains: bool = True, use_cs: bool = False):
     """The stage sequence of BMS2.cpp:131-234 through the staged entry points.
 
     Returns dict(domains=[iteration 0..K], gmcr=(offsets, indices), counts=per GMCR entry,
-    total=match total, signatures=...)."""
+    total=match total, signatures=...).  use_cs joins as BMS2 --use-cs does (BMS2.cpp:217-300):
+    partial matches of query 0 at depth = its node count, then their extensions; the result also
+    holds the partial-match records (`partial`)."""
     dq = DeviceBatchedCSRGraph(q, query)
     dd = DeviceBatchedCSRGraph(q, data)
     sig = Signature(q, dd.total_nodes, dq.total_nodes)
@@ -512,7 +587,14 @@ def run_staged(q: Queue, query: GraphBatch, data: GraphBatch, K: int, find_first
     out["gmcr"] = gm.host()
     total = DeviceBuffer(q, 8, zero=True)
     per_pair = DeviceBuffer(q, 8 * max(gm.total, 1), zero=True)
-    join_candidates(q, dq, dd, cand, gm, total, find_first, per_pair)
+    if use_cs and query.num_graphs:
+        end = int(query.graph_offsets[1] - query.graph_offsets[0])
+        part, n = join_partial_candidates(q, dq, dd, cand, end)
+        join_with_partial_matches(q, dq, dd, cand, gm, end, part, n, total, per_pair, find_first)
+        q.wait()
+        out["partial"] = part.to_numpy(np.int32, n * (end + 1)).reshape(n, end + 1)
+    else:
+        join_candidates(q, dq, dd, cand, gm, total, find_first, per_pair)
     q.wait()
     out["total"] = int(total.to_numpy(np.uint64)[0])
     out["counts"] = per_pair.to_numpy(np.uint64, gm.total)

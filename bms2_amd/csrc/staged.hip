@@ -462,6 +462,212 @@ __global__ __launch_bounds__(256) void join_kernel(
   if (lane_id() == 0 && c) atomicAdd(reinterpret_cast<unsigned long long*>(total), (unsigned long long)c);
 }
 
+// ------------------------------------------------------------------ CS (cluster) join
+
+// Depth-first enumeration shared by the cluster-mode kernels: maps of depths [d0, n) of one query
+// into one data graph (global nodes [s, e), NW words of local ids), standard backtracking.  The
+// caller fills dom[i] (domain of depth i, local bits) for i in [d0, n), qadj[i] (bit j < i: depths
+// i and j adjacent in the query) for i < n, and for a fixed prefix i < d0 its local node f[i], its
+// neighbour row NR[i] and its bit in `used`.  Every complete map is passed to leaf(v) with f[n-1]
+// = v, in ascending DFS order (candidates of each depth by ascending node id); leaf returns true to
+// stop.  count_only counts the last depth by popcount instead.  Returns the number of maps.
+template <int NW>
+struct CsFrame {
+  uint64_t dom[BMS2_MAX_QUERY_NODES][NW];
+  uint64_t NR[BMS2_MAX_QUERY_NODES][NW];
+  uint64_t R[BMS2_MAX_QUERY_NODES][NW];
+  uint32_t qadj[BMS2_MAX_QUERY_NODES];
+  uint32_t f[BMS2_MAX_QUERY_NODES];
+  uint64_t used[NW];
+};
+
+template <int NW>
+__device__ inline void cs_nrow(const uint32_t* __restrict__ dro, const uint32_t* __restrict__ dci,
+                               uint32_t s, uint32_t x, uint64_t* out) {
+  for (int k = 0; k < NW; ++k) out[k] = 0;
+  for (uint32_t k = dro[s + x]; k < dro[s + x + 1]; ++k) {
+    const uint32_t y = dci[k] - s;
+    out[y >> 6] |= uint64_t(1) << (y & 63);
+  }
+}
+
+template <int NW>
+__device__ inline void cs_candidates(CsFrame<NW>& F, int d) {
+  for (int k = 0; k < NW; ++k) F.R[d][k] = F.dom[d][k] & ~F.used[k];
+  for (int i = 0; i < d; ++i) {
+    const bool adj = (F.qadj[d] >> i) & 1;
+    for (int k = 0; k < NW; ++k) F.R[d][k] &= adj ? F.NR[i][k] : ~F.NR[i][k];
+  }
+}
+
+template <int NW, typename Leaf>
+__device__ uint64_t cs_dfs(CsFrame<NW>& F, int d0, int n, const uint32_t* __restrict__ dro,
+                           const uint32_t* __restrict__ dci, uint32_t s, bool count_only, Leaf leaf) {
+  uint64_t c = 0;
+  int d = d0;
+  cs_candidates(F, d);
+  while (true) {
+    if (count_only && d == n - 1) {
+      for (int k = 0; k < NW; ++k) {
+        c += uint64_t(popc64(F.R[d][k]));
+        F.R[d][k] = 0;
+      }
+    }
+    int k0 = -1;
+    for (int k = 0; k < NW; ++k)
+      if (F.R[d][k]) {
+        k0 = k;
+        break;
+      }
+    if (k0 < 0) {
+      if (d == d0) break;
+      --d;
+      F.used[F.f[d] >> 6] &= ~(uint64_t(1) << (F.f[d] & 63));
+      continue;
+    }
+    const uint32_t v = uint32_t(k0 * 64 + ctz64(F.R[d][k0]));
+    F.R[d][k0] &= F.R[d][k0] - 1;
+    F.f[d] = v;
+    if (d == n - 1) {
+      ++c;
+      if (leaf(v)) break;
+      continue;
+    }
+    cs_nrow<NW>(dro, dci, s, v, F.NR[d]);
+    F.used[v >> 6] |= uint64_t(1) << (v & 63);
+    ++d;
+    cs_candidates(F, d);
+  }
+  return c;
+}
+
+// Query 0's domains (depths 0..end-1) and adjacency inside data graph g.
+template <int NW>
+__device__ inline void cs_prototype(CsFrame<NW>& F, const uint64_t* __restrict__ cand, uint64_t W,
+                                    const uint32_t* __restrict__ qro, const uint32_t* __restrict__ qci,
+                                    uint32_t qb, uint32_t end, uint32_t s, uint32_t e) {
+  for (uint32_t i = 0; i < end; ++i) {
+    const uint64_t* row = cand + uint64_t(qb + i) * W;
+    for (int k = 0; k < NW; ++k) F.dom[i][k] = row_bits(row, s, e, k);
+    uint32_t a = 0;
+    for (uint32_t k = qro[qb + i]; k < qro[qb + i + 1]; ++k) {
+      const uint32_t j = qci[k] - qb;
+      if (j < i) a |= 1u << j;
+    }
+    F.qadj[i] = a;
+  }
+  for (int k = 0; k < NW; ++k) F.used[k] = 0;
+}
+
+// joinPartialCandidates2 (isomorphism.hpp:1021-1178): one lane per data graph; pass 0 counts the
+// embeddings of query 0's first `end` nodes (cnt[g], total), pass 1 writes them at off[g] as
+// records {end global data node ids, data graph id} (records at or past cap are dropped).
+template <int NW>
+__global__ __launch_bounds__(64) void cs_partial_kernel(
+    int pass, const uint64_t* __restrict__ cand, uint64_t W, const uint32_t* __restrict__ qgo,
+    const uint32_t* __restrict__ qro, const uint32_t* __restrict__ qci,
+    const uint32_t* __restrict__ dgo, uint32_t D, const uint32_t* __restrict__ dro,
+    const uint32_t* __restrict__ dci, uint32_t end, uint32_t* __restrict__ cnt,
+    const uint32_t* __restrict__ off, uint64_t* __restrict__ total, int32_t* __restrict__ out,
+    uint64_t cap) {
+  const uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  uint64_t c = 0;
+  if (g < D) {
+    const uint32_t s = dgo[g], e = dgo[g + 1];
+    CsFrame<NW> F;
+    cs_prototype<NW>(F, cand, W, qro, qci, qgo[0], end, s, e);
+    if (pass == 0) {
+      c = cs_dfs<NW>(F, 0, int(end), dro, dci, s, true, [](uint32_t) { return false; });
+      cnt[g] = uint32_t(c);
+    } else {
+      uint64_t pos = off[g];
+      cs_dfs<NW>(F, 0, int(end), dro, dci, s, false, [&](uint32_t) {
+        if (pos < cap) {
+          int32_t* r = out + pos * (end + 1);
+          for (uint32_t i = 0; i < end; ++i) r[i] = int32_t(s + F.f[i]);
+          r[end] = int32_t(g);
+        }
+        ++pos;
+        return false;
+      });
+    }
+  }
+  if (pass == 0) {
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (lane_id() == 0 && c) atomicAdd(reinterpret_cast<unsigned long long*>(total), (unsigned long long)c);
+  }
+}
+
+// joinWithPartialMatches (isomorphism.hpp:756-977): one lane per partial match; over the GMCR
+// queries of its data graph, n_q == end counts the match, n_q > end counts its extensions to
+// depths end..n_q-1 (q's domains, q's induced adjacency against every earlier depth, prefix
+// included).  Per GMCR entry: counts added (count-all) or a 0 -> 1 flag (find_first, the
+// pair_done flags of :836-866); the total is added to *total.
+template <int NW>
+__global__ __launch_bounds__(64) void cs_extend_kernel(
+    const uint64_t* __restrict__ cand, uint64_t W, const uint32_t* __restrict__ qgo,
+    const uint32_t* __restrict__ qro, const uint32_t* __restrict__ qci,
+    const uint32_t* __restrict__ dgo, const uint32_t* __restrict__ dro,
+    const uint32_t* __restrict__ dci, const uint32_t* __restrict__ goff,
+    const uint32_t* __restrict__ gidx, uint32_t D, uint32_t end,
+    const int32_t* __restrict__ partial, uint64_t num_partial, int find_first,
+    uint64_t* __restrict__ per_pair, uint64_t* __restrict__ total) {
+  const uint64_t m = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  uint64_t sum = 0;
+  // a record naming a graph or node outside the batch (not one step 1 wrote) is skipped
+  bool ok = m < num_partial;
+  const int32_t* rec = partial + (ok ? m : 0) * (end + 1);
+  const uint32_t g = ok ? uint32_t(rec[end]) : 0u;
+  ok = ok && g < D;
+  const uint32_t s = ok ? dgo[g] : 0u, e = ok ? dgo[g + 1] : 0u;
+  for (uint32_t i = 0; i < end && ok; ++i) ok = uint32_t(rec[i]) >= s && uint32_t(rec[i]) < e;
+  if (ok) {
+    CsFrame<NW> F;
+    for (int k = 0; k < NW; ++k) F.used[k] = 0;
+    for (uint32_t i = 0; i < end; ++i) {
+      const uint32_t x = uint32_t(rec[i]) - s;
+      F.f[i] = x;
+      F.used[x >> 6] |= uint64_t(1) << (x & 63);
+      cs_nrow<NW>(dro, dci, s, x, F.NR[i]);
+    }
+    uint64_t used0[NW];
+    for (int k = 0; k < NW; ++k) used0[k] = F.used[k];
+    for (uint32_t t = goff[g]; t < goff[g + 1]; ++t) {
+      if (find_first && __atomic_load_n(per_pair + t, __ATOMIC_RELAXED)) continue;
+      const uint32_t q = gidx[t];
+      const uint32_t qb = qgo[q], nq = qgo[q + 1] - qb;
+      if (nq < end) continue;
+      uint64_t c = 1;
+      if (nq > end) {
+        for (uint32_t i = 0; i < nq; ++i) {
+          uint32_t a = 0;
+          for (uint32_t k = qro[qb + i]; k < qro[qb + i + 1]; ++k) {
+            const uint32_t j = qci[k] - qb;
+            if (j < i) a |= 1u << j;
+          }
+          F.qadj[i] = a;
+          if (i >= end) {
+            const uint64_t* row = cand + uint64_t(qb + i) * W;
+            for (int k = 0; k < NW; ++k) F.dom[i][k] = row_bits(row, s, e, k);
+          }
+        }
+        for (int k = 0; k < NW; ++k) F.used[k] = used0[k];
+        c = cs_dfs<NW>(F, int(end), int(nq), dro, dci, s, !find_first,
+                       [&](uint32_t) { return find_first != 0; });
+      }
+      if (!c) continue;
+      if (find_first) {
+        if (atomicCAS(reinterpret_cast<unsigned long long*>(per_pair + t), 0ull, 1ull) == 0ull) ++sum;
+      } else {
+        atomicAdd(reinterpret_cast<unsigned long long*>(per_pair + t), (unsigned long long)c);
+        sum += c;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+  if (lane_id() == 0 && sum) atomicAdd(reinterpret_cast<unsigned long long*>(total), (unsigned long long)sum);
+}
+
 inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 1u << 20) {
   uint64_t g = (n + block - 1) / block;
   return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(g, cap)));
@@ -997,6 +1203,124 @@ int bms2_join_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_
   if (int rc = staged_join(q, query, data, cand, gmcr, find_first, nullptr, d_num_matches,
                            d_pair_counts, nullptr))
     return rc;
+  return es.end();
+}
+
+// ---- CS (cluster) join
+namespace {
+int check_cs(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
+             bms2_candidates cand, uint32_t end) {
+  if (int rc = use_device(q)) return rc;
+  if (!query || !data || !cand) return set_error(BMS2_ERR_INVALID, "null argument");
+  if (cand->rows != query->total_nodes || cand->targets != data->total_nodes)
+    return set_error(BMS2_ERR_INVALID, "candidates of another pair of batches");
+  if (end == 0 || end > BMS2_MAX_QUERY_NODES)
+    return set_error(BMS2_ERR_INVALID, "partial match depth %u outside 1..%u", end,
+                     BMS2_MAX_QUERY_NODES);
+  if (query->max_graph_nodes > BMS2_MAX_QUERY_NODES)
+    return set_error(BMS2_ERR_UNSUPPORTED, "query with %u nodes (limit %u, isomorphism.hpp:248)",
+                     query->max_graph_nodes, BMS2_MAX_QUERY_NODES);
+  if (data->max_graph_nodes > BMS2_MAX_GRAPH_NODES)
+    return set_error(BMS2_ERR_UNSUPPORTED, "data graph with %u nodes (limit %u)",
+                     data->max_graph_nodes, BMS2_MAX_GRAPH_NODES);
+  return BMS2_OK;
+}
+}  // namespace
+
+int bms2_join_partial_candidates(bms2_queue q, const bms2_device_csr* query,
+                                 const bms2_device_csr* data, bms2_candidates cand, uint32_t end,
+                                 int32_t* d_partial, uint64_t capacity, uint64_t* num_partial,
+                                 bms2_event* ev) {
+  if (int rc = check_cs(q, query, data, cand, end)) return rc;
+  if (!num_partial) return set_error(BMS2_ERR_INVALID, "null argument");
+  if (capacity && !d_partial) return set_error(BMS2_ERR_INVALID, "null partial-match buffer");
+  *num_partial = 0;
+  uint32_t qgo[2] = {0, 0};
+  if (query->num_graphs) {
+    BMS2_HIP(hipMemcpy(qgo, query->graph_offsets, sizeof(qgo), hipMemcpyDeviceToHost));
+  }
+  const uint32_t D = data->num_graphs;
+  EventScope es{q, ev};
+  if (int rc = es.begin()) return rc;
+  // query 0 with fewer than `end` nodes yields nothing (isomorphism.hpp:1087)
+  if (query->num_graphs && D && qgo[1] - qgo[0] >= end) {
+    hipStream_t st = q->stream;
+    uint32_t* cnt = nullptr;
+    uint32_t* off = nullptr;
+    uint64_t* tot = nullptr;
+    int rc = BMS2_OK;
+    if ((rc = device_alloc((void**)&cnt, 4ull * D)) ||
+        (rc = device_alloc((void**)&off, 4ull * (D + 1))) ||
+        (rc = device_alloc((void**)&tot, 8))) {
+      (void)hipFree(cnt);
+      (void)hipFree(off);
+      return rc;
+    }
+    auto launch = [&](auto kern, int pass) {
+      hipLaunchKernelGGL(kern, dim3((D + 63) / 64), dim3(64), 0, st, pass, cand->bits, cand->W,
+                         query->graph_offsets, query->row_offsets, query->column_indices,
+                         data->graph_offsets, D, data->row_offsets, data->column_indices, end, cnt,
+                         off, tot, d_partial, capacity);
+    };
+    const bool small = data->max_graph_nodes <= 64;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(64), 0, st, tot, uint64_t(1));
+    if (small) launch(cs_partial_kernel<1>, 0); else launch(cs_partial_kernel<4>, 0);
+    uint64_t total = 0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&total, tot, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      rc = set_error(BMS2_ERR_HIP, "partial-match count failed");
+    if (rc == BMS2_OK && total > UINT32_MAX)
+      rc = set_error(BMS2_ERR_UNSUPPORTED, "%llu partial matches (limit %u)",
+                     (unsigned long long)total, UINT32_MAX);
+    if (rc == BMS2_OK) {
+      *num_partial = total;
+      if (total > capacity)
+        rc = set_error(BMS2_ERR_CAPACITY, "%llu partial matches, buffer holds %llu",
+                       (unsigned long long)total, (unsigned long long)capacity);
+    }
+    if (rc == BMS2_OK && total) {
+      hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, cnt, uint64_t(D), off);
+      if (small) launch(cs_partial_kernel<1>, 1); else launch(cs_partial_kernel<4>, 1);
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
+        rc = set_error(BMS2_ERR_HIP, "partial-match fill failed");
+    }
+    (void)hipFree(cnt);
+    (void)hipFree(off);
+    (void)hipFree(tot);
+    if (rc) return rc;
+  }
+  return es.end();
+}
+
+int bms2_join_with_partial_matches(bms2_queue q, const bms2_device_csr* query,
+                                   const bms2_device_csr* data, bms2_candidates cand,
+                                   bms2_gmcr gmcr, uint32_t end, const int32_t* d_partial,
+                                   uint64_t num_partial, uint64_t* d_num_matches, int find_first,
+                                   uint64_t* d_pair_counts, bms2_event* ev) {
+  if (int rc = check_cs(q, query, data, cand, end)) return rc;
+  if (!gmcr || !d_num_matches || !d_pair_counts || (num_partial && !d_partial))
+    return set_error(BMS2_ERR_INVALID, "null argument");
+  if (!gmcr->offsets) return set_error(BMS2_ERR_INVALID, "gmcr not generated");
+  if (gmcr->num_data != data->num_graphs) return set_error(BMS2_ERR_INVALID, "gmcr of another batch");
+  EventScope es{q, ev};
+  if (int rc = es.begin()) return rc;
+  hipStream_t st = q->stream;
+  const uint64_t T = gmcr->total;
+  if (T)
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(T, 256, 4096)), dim3(256), 0, st,
+                       d_pair_counts, T);
+  if (num_partial && T) {
+    auto launch = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(grid_for(num_partial, 64, 1u << 30)), dim3(64), 0, st,
+                         cand->bits, cand->W, query->graph_offsets, query->row_offsets,
+                         query->column_indices, data->graph_offsets, data->row_offsets,
+                         data->column_indices, gmcr->offsets, gmcr->indices, data->num_graphs,
+                         end, d_partial, num_partial, find_first, d_pair_counts, d_num_matches);
+    };
+    if (data->max_graph_nodes <= 64) launch(cs_extend_kernel<1>); else launch(cs_extend_kernel<4>);
+  }
+  BMS2_CHECK_LAUNCH("cs_extend_kernel");
   return es.end();
 }
 

@@ -193,6 +193,100 @@ def generate(workload: str | dict = "molecules-2k", seed: int = 0, threads: int 
     return data, queries
 
 
+def _graph_adj(b: GraphBatch, g: int) -> tuple[list[int], list[dict[int, int]]]:
+    """Labels and adjacency (local neighbour -> edge label) of graph g of a batch."""
+    n0, n1 = int(b.graph_offsets[g]), int(b.graph_offsets[g + 1])
+    adj: list[dict[int, int]] = [dict() for _ in range(n1 - n0)]
+    for u in range(n0, n1):
+        for k in range(int(b.row_offsets[u]), int(b.row_offsets[u + 1])):
+            adj[u - n0][int(b.column_indices[k]) - n0] = int(b.edge_labels[k])
+    return [int(x) for x in b.node_labels[n0:n1]], adj
+
+
+def _induced(labels, adj, nodes) -> tuple[list[int], list[tuple[int, int, int]]]:
+    """Induced subgraph on `nodes`, numbered in the given order."""
+    pos = {v: i for i, v in enumerate(nodes)}
+    edges = [(pos[u], pos[v], lab) for u in nodes for v, lab in sorted(adj[u].items())
+             if v in pos and pos[u] < pos[v]]
+    return [labels[v] for v in nodes], edges
+
+
+def _grow(adj, start: list[int], size: int, rng) -> list[int]:
+    """Connected growth of a node set: random frontier nodes appended until `size` nodes."""
+    chosen, inset = list(start), set(start)
+    while len(chosen) < size:
+        frontier = sorted({v for u in chosen for v in adj[u] if v not in inset})
+        if not frontier:
+            break
+        v = frontier[int(rng.integers(len(frontier)))]
+        chosen.append(v)
+        inset.add(v)
+    return chosen
+
+
+def _induced_embedding(plabels, pedges, labels, adj) -> list[int] | None:
+    """One label-preserving induced embedding of a small pattern (backtracking), or None."""
+    n = len(plabels)
+    padj = [set() for _ in range(n)]
+    for u, v, _ in pedges:
+        padj[u].add(v)
+        padj[v].add(u)
+    img: list[int] = []
+
+    def extend(i: int) -> bool:
+        if i == n:
+            return True
+        for c in range(len(labels)):
+            if labels[c] != plabels[i] or c in img:
+                continue
+            if all((j in padj[i]) == (img[j] in adj[c]) for j in range(i)):
+                img.append(c)
+                if extend(i + 1):
+                    return True
+                img.pop()
+        return False
+
+    return list(img) if extend(0) else None
+
+
+def cluster_queries(data: GraphBatch, num_members: int, seed: int = 0,
+                    proto_nodes: tuple[int, int] = (4, 7),
+                    member_nodes: tuple[int, int] = (6, 16)) -> GraphBatch:
+    """A query batch in the cluster convention of the reference's CS mode (generate_cs.py:338-354,
+    556-569): query 0 is the prototype, every member starts with an induced copy of it whose nodes
+    come first in the prototype's order, followed by the member's own nodes.  The prototype is a
+    connected induced subgraph of a data graph; members are connected induced subgraphs grown
+    around an induced image of the prototype in data graphs.  Deterministic in seed."""
+    rng = np.random.default_rng(seed)
+    G = data.num_graphs
+    sizes = np.diff(np.asarray(data.graph_offsets, np.int64))
+    p = int(rng.integers(proto_nodes[0], proto_nodes[1] + 1))
+    big = np.flatnonzero(sizes >= p)
+    if len(big) == 0:
+        raise ValueError("no data graph holds a prototype of that size")
+    while True:
+        g0 = int(big[rng.integers(len(big))])
+        labels0, adj0 = _graph_adj(data, g0)
+        S = _grow(adj0, [int(rng.integers(len(labels0)))], p, rng)
+        if len(S) == p:
+            break
+    proto = _induced(labels0, adj0, S)
+    out = [proto]
+    for _ in range(num_members):
+        img, labels, adj = None, None, None
+        for _ in range(32):  # another data graph holding the prototype, else its source
+            g = int(rng.integers(G))
+            labels, adj = _graph_adj(data, g)
+            img = _induced_embedding(proto[0], proto[1], labels, adj)
+            if img is not None:
+                break
+        if img is None:
+            labels, adj, img = labels0, adj0, list(S)
+        size = int(rng.integers(member_nodes[0], member_nodes[1] + 1))
+        out.append(_induced(labels, adj, _grow(adj, img, max(size, p), rng)))
+    return GraphBatch.from_graphs(out)
+
+
 def load_dat(path: str, max_graphs: int = 0, skip_malformed: bool = False) -> GraphBatch:
     """io::loadCSRGraphsFromFile (io.hpp:43-52)."""
     h = _lib.HostCsr()

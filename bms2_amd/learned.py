@@ -52,7 +52,7 @@ class OrderNet(_Handle):
         self.q = q
         h = C.c_void_p()
         check(_L().bms2_mlp_create(q.h, int(seed), C.byref(h)))
-        self.h = h
+        self._bind(h)
         self.num_params = int(_L().bms2_mlp_num_params())
 
     def get_params(self) -> np.ndarray:

@@ -232,6 +232,33 @@ int bms2_join_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_
                          bms2_candidates cand, bms2_gmcr gmcr, uint64_t* d_num_matches,
                          int find_first, uint64_t* d_pair_counts, bms2_event* ev);
 
+/* CS (cluster) join, step 1: joinPartialCandidates2 (isomorphism.hpp:1021-1178), the --use-cs
+ * path of BMS2.cpp:217-259.  Every embedding of query 0's first `end` nodes (the cluster
+ * prototype; BMS2 --use-cs takes end = its node count) in every data graph, each node inside its
+ * domain, injective, induced adjacency at every depth, standard backtracking.  Written to d_partial
+ * as records of end + 1 int32 {mapped global data node ids in depth order, data graph id}, in data
+ * graph order then ascending DFS order (the reference's order depends on atomics).  *num_partial
+ * (host) receives the number found; when it exceeds `capacity` (records) nothing is written and
+ * the call fails with BMS2_ERR_CAPACITY (the reference writes past its 10^8-record buffer).
+ * Synchronises the queue. */
+int bms2_join_partial_candidates(bms2_queue q, const bms2_device_csr* query,
+                                 const bms2_device_csr* data, bms2_candidates cand, uint32_t end,
+                                 int32_t* d_partial, uint64_t capacity, uint64_t* num_partial,
+                                 bms2_event* ev);
+
+/* CS (cluster) join, step 2: joinWithPartialMatches (isomorphism.hpp:756-977).  For every partial
+ * match of step 1 and every GMCR query q of its data graph: n_q == end counts the match itself,
+ * n_q > end counts its extensions to depths end..n_q-1 (q's domains, q's induced adjacency against
+ * every earlier depth including the prototype part, injective), n_q < end nothing.
+ * d_pair_counts (required, one u64 per GMCR entry, zeroed by the call) receives the counts, or in
+ * find_first mode a 0/1 flag per entry (the reference's pair_done flags); the total is
+ * atomically added to the device u64 *d_num_matches (the caller zeroes it). */
+int bms2_join_with_partial_matches(bms2_queue q, const bms2_device_csr* query,
+                                   const bms2_device_csr* data, bms2_candidates cand,
+                                   bms2_gmcr gmcr, uint32_t end, const int32_t* d_partial,
+                                   uint64_t num_partial, uint64_t* d_num_matches, int find_first,
+                                   uint64_t* d_pair_counts, bms2_event* ev);
+
 /* ---------------------------------------------------------------- fused pipeline
  * The stage sequence of BMS2.cpp:131-234 (signatures -> filter -> refine x K -> GMCR -> join) as
  * fused launches that keep domains on chip.  Results equal the staged calls above. */

@@ -59,6 +59,11 @@ def lib() -> C.CDLL:
         h.orc_gmcr.argtypes = [P, P, vp, C.c_uint64, vp, vp, C.c_uint64]
         h.orc_join.restype = C.c_uint64
         h.orc_join.argtypes = [P, P, vp, C.c_uint64, vp, vp, C.c_int, vp, vp, vp]
+        h.orc_join_partial.restype = C.c_uint64
+        h.orc_join_partial.argtypes = [P, P, vp, C.c_uint64, C.c_uint32, vp, C.c_uint64]
+        h.orc_join_with_partial.restype = C.c_uint64
+        h.orc_join_with_partial.argtypes = [P, P, vp, C.c_uint64, vp, vp, C.c_uint32, vp,
+                                            C.c_uint64, C.c_int, vp]
         h.orc_pipeline.restype = C.c_int
         h.orc_pipeline.argtypes = [P, P, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, vp, vp, vp,
                                    C.c_int]
@@ -169,6 +174,30 @@ def join(q, d, cand, offsets, indices, find_first=False, order=None):
     total = lib().orc_join(C.byref(cq), C.byref(cd), _ptr(cand), cand.shape[1], _ptr(offsets),
                            _ptr(idx), int(find_first), _ptr(ordp), _ptr(counts), _ptr(nodes))
     return int(total), counts[:T], nodes[:T]
+
+
+def join_partial(q, d, cand, end: int) -> np.ndarray:
+    """joinPartialCandidates2: every embedding of query 0's first `end` nodes, as rows of
+    end + 1 ints {mapped global data node ids, data graph id} (data graph order, DFS order)."""
+    cq, cd = csr(q), csr(d)
+    n = lib().orc_join_partial(C.byref(cq), C.byref(cd), _ptr(cand), cand.shape[1], end, None, 0)
+    out = np.zeros((max(n, 1), end + 1), np.int32)
+    lib().orc_join_partial(C.byref(cq), C.byref(cd), _ptr(cand), cand.shape[1], end, _ptr(out), n)
+    return out[:n]
+
+
+def join_with_partial(q, d, cand, offsets, indices, end: int, partial: np.ndarray,
+                      find_first: bool = False) -> tuple[int, np.ndarray]:
+    """joinWithPartialMatches: (total, per-GMCR-entry counts)."""
+    cq, cd = csr(q), csr(d)
+    T = len(indices)
+    counts = np.zeros(max(T, 1), np.uint64)
+    part = np.ascontiguousarray(partial, np.int32)
+    idx = np.ascontiguousarray(indices, np.uint32)
+    total = lib().orc_join_with_partial(C.byref(cq), C.byref(cd), _ptr(cand), cand.shape[1],
+                                        _ptr(offsets), _ptr(idx), end, _ptr(part), len(part),
+                                        int(find_first), _ptr(counts))
+    return int(total), counts[:T]
 
 
 class StagedResult:

@@ -385,6 +385,127 @@ uint64_t orc_join(const orc_csr* q, const orc_csr* d, const uint64_t* cand, uint
   return total;
 }
 
+uint64_t orc_join_partial(const orc_csr* q, const orc_csr* d, const uint64_t* cand, uint64_t W,
+                          uint32_t end, int32_t* out, uint64_t cap) {
+  // joinPartialCandidates2 (isomorphism.hpp:1021-1178): in every data graph, every injective map
+  // of query 0's nodes 0..end-1 into the graph with each node inside its domain and the induced
+  // adjacency test at every depth (isValidMapping_fast_bitwise), standard backtracking.  Record =
+  // the end mapped global data node ids, then the data graph id (:983-988).  Query 0 with fewer
+  // than end nodes contributes nothing (:1087).  Records in data graph order, then DFS order.
+  if (q->num_graphs == 0 || end == 0) return 0;
+  const GraphRange qr = graph_range(q, 0);
+  if (end > qr.end - qr.begin) return 0;
+  uint64_t n = 0;
+  std::vector<uint32_t> mapping(end);
+  for (uint32_t g = 0; g < d->num_graphs; ++g) {
+    const GraphRange r = graph_range(d, g);
+    std::vector<std::vector<uint32_t>> lists(end);
+    for (uint32_t i = 0; i < end; ++i)
+      for (uint32_t dn = r.begin; dn < r.end; ++dn)
+        if ((cand[uint64_t(qr.begin + i) * W + dn / 64] >> (dn % 64)) & 1) lists[i].push_back(dn);
+    std::vector<char> visited(r.end - r.begin, 0);
+    std::vector<uint32_t> cursor(end, 0);
+    int depth = 0;
+    while (depth >= 0) {
+      if (cursor[depth] >= lists[depth].size()) {
+        --depth;
+        if (depth >= 0) visited[mapping[depth] - r.begin] = 0;
+        continue;
+      }
+      const uint32_t c = lists[depth][cursor[depth]++];
+      if (visited[c - r.begin]) continue;
+      bool ok = true;
+      for (int i = 0; i < depth && ok; ++i)
+        ok = is_neighbor(q, qr.begin + i, qr.begin + depth) == is_neighbor(d, mapping[i], c);
+      if (!ok) continue;
+      mapping[depth] = c;
+      if (depth == int(end) - 1) {
+        if (n < cap && out)
+          for (uint32_t i = 0; i <= end; ++i)
+            out[n * (end + 1) + i] = int32_t(i < end ? mapping[i] : g);
+        ++n;
+        continue;
+      }
+      visited[c - r.begin] = 1;
+      ++depth;
+      cursor[depth] = 0;
+    }
+  }
+  return n;
+}
+
+uint64_t orc_join_with_partial(const orc_csr* q, const orc_csr* d, const uint64_t* cand,
+                               uint64_t W, const uint32_t* offsets, const uint32_t* indices,
+                               uint32_t end, const int32_t* partial, uint64_t num_partial,
+                               int find_first, uint64_t* counts) {
+  // joinWithPartialMatches (isomorphism.hpp:756-977): for every partial match (mapping of depths
+  // 0..end-1 in data graph g) and every GMCR query q of g: n_q == end counts the partial match
+  // itself (:857-868), n_q > end counts its extensions to depths end..n_q-1 with q's domains, q's
+  // induced adjacency against every earlier depth (the prototype part included) and
+  // injectivity (:871-948), n_q < end nothing.  Standard backtracking.  counts[t] per GMCR entry
+  // t; find_first keeps 1 per entry with any match (the pair_done flags, :836-866, :952-960).
+  const uint64_t T = offsets[d->num_graphs];
+  for (uint64_t t = 0; t < T; ++t) counts[t] = 0;
+  uint64_t total = 0;
+  for (uint64_t m = 0; m < num_partial; ++m) {
+    const int32_t* rec = partial + m * (end + 1);
+    const uint32_t g = uint32_t(rec[end]);
+    const GraphRange r = graph_range(d, g);
+    for (uint32_t t = offsets[g]; t < offsets[g + 1]; ++t) {
+      if (find_first && counts[t]) continue;
+      const uint32_t qi = indices[t];
+      const GraphRange qr = graph_range(q, qi);
+      const uint32_t nq = qr.end - qr.begin;
+      if (nq < end) continue;
+      uint64_t c = 0;
+      if (nq == end) {
+        c = 1;
+      } else {
+        std::vector<uint32_t> mapping(nq);
+        std::vector<char> visited(r.end - r.begin, 0);
+        for (uint32_t i = 0; i < end; ++i) {
+          mapping[i] = uint32_t(rec[i]);
+          visited[mapping[i] - r.begin] = 1;
+        }
+        std::vector<std::vector<uint32_t>> lists(nq);
+        for (uint32_t i = end; i < nq; ++i)
+          for (uint32_t dn = r.begin; dn < r.end; ++dn)
+            if ((cand[uint64_t(qr.begin + i) * W + dn / 64] >> (dn % 64)) & 1)
+              lists[i].push_back(dn);
+        std::vector<uint32_t> cursor(nq, 0);
+        int depth = int(end);
+        bool stop = false;
+        while (depth >= int(end) && !stop) {
+          if (cursor[depth] >= lists[depth].size()) {
+            --depth;
+            if (depth >= int(end)) visited[mapping[depth] - r.begin] = 0;
+            continue;
+          }
+          const uint32_t cn = lists[depth][cursor[depth]++];
+          if (visited[cn - r.begin]) continue;
+          bool ok = true;
+          for (int i = 0; i < depth && ok; ++i)
+            ok = is_neighbor(q, qr.begin + i, qr.begin + depth) == is_neighbor(d, mapping[i], cn);
+          if (!ok) continue;
+          if (depth == int(nq) - 1) {
+            ++c;
+            if (find_first) stop = true;
+            continue;
+          }
+          mapping[depth] = cn;
+          visited[cn - r.begin] = 1;
+          ++depth;
+          cursor[depth] = 0;
+        }
+      }
+      if (find_first) c = c ? 1 : 0;
+      counts[t] += c;
+      total += c;
+    }
+  }
+  return total;
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------------------------------------

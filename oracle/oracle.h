@@ -70,6 +70,19 @@ uint64_t orc_gmcr(const orc_csr* q, const orc_csr* d, const uint64_t* cand, uint
 uint64_t orc_join(const orc_csr* q, const orc_csr* d, const uint64_t* cand, uint64_t W,
                   const uint32_t* offsets, const uint32_t* indices, int find_first,
                   const uint32_t* order, uint64_t* counts, uint64_t* dfs_nodes);
+/* joinPartialCandidates2 (isomorphism.hpp:1021-1178), standard backtracking: every embedding of
+ * query 0's first `end` nodes in every data graph, as records of end + 1 ints {mapped global data
+ * node ids, data graph id} in data graph order then DFS order; writes at most cap records (out
+ * nullable) and returns how many there are. */
+uint64_t orc_join_partial(const orc_csr* q, const orc_csr* d, const uint64_t* cand, uint64_t W,
+                          uint32_t end, int32_t* out, uint64_t cap);
+/* joinWithPartialMatches (isomorphism.hpp:756-977), standard backtracking: counts[t] (per GMCR
+ * entry, overwritten) = over the partial matches of entry t's data graph, 1 each when n_q == end,
+ * their extensions to all of q when n_q > end; find_first keeps [count > 0].  Returns the total. */
+uint64_t orc_join_with_partial(const orc_csr* q, const orc_csr* d, const uint64_t* cand,
+                               uint64_t W, const uint32_t* offsets, const uint32_t* indices,
+                               uint32_t end, const int32_t* partial, uint64_t num_partial,
+                               int find_first, uint64_t* counts);
 
 /* Fused per-data-graph restatement of the stage sequence BMS2.cpp:131-234 with K refinement
  * steps, over data graphs [g_begin, g_end), parallel over data graphs (OpenMP, `threads`, 0 =
