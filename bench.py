@@ -54,6 +54,102 @@ def load_traffic() -> dict | None:
     return None
 
 
+def cs_bench(args) -> int:
+    """BMS2 --use-cs on one cluster batch (a 5-node prototype + 63 members, cluster_queries seed
+    2: a prototype common in the molecules, ~5 partial matches per graph) over
+    the first `--data-graphs` (default 200,000) molecules of the workload, K refinement steps,
+    count-all: the two CS steps (joinPartialCandidates2 + joinWithPartialMatches) timed against the
+    regular staged join on the same domains and GMCR, with HIP events.  Parity: per-GMCR-entry
+    counts of the first --cpu-sample-graphs (default 20,000) data graphs against the oracle, whose
+    CS join on that prefix (one thread) is the CPU baseline."""
+    import oracle
+    from bms2_amd import api, graphs
+
+    D0 = args.data_graphs or 200_000
+    data, _ = graphs.generate(args.workload, seed=0, num_data_graphs=D0)
+    queries = graphs.cluster_queries(data, 63, seed=2, proto_nodes=(5, 5))
+    K, Q, D = args.refinement_steps, queries.num_graphs, data.num_graphs
+    end = int(queries.graph_offsets[1] - queries.graph_offsets[0])
+    log = lambda m: print(f"[cs] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    log(f"{D} molecules x cluster of {Q} queries (prototype of {end} nodes), K={K}")
+    q = api.Queue(0)
+    dq = api.DeviceBatchedCSRGraph(q, queries)
+    dd = api.DeviceBatchedCSRGraph(q, data)
+    sig = api.Signature(q, dd.total_nodes, dq.total_nodes)
+    path = api.PathSignature(q, dd.total_nodes, dq.total_nodes)
+    cyc = api.CycleSignature(q, dd.total_nodes, dq.total_nodes)
+    cand = api.Candidates(q, dq.total_nodes, dd.total_nodes)
+    for g, fn in ((dd, "data"), (dq, "query")):
+        getattr(sig, f"generate_{fn}_signatures")(g)
+        getattr(path, f"generate_{fn}_path_signatures")(g)
+        getattr(cyc, f"generate_{fn}_cycle_signatures")(g)
+    api.filter_candidates(q, dq, dd, sig, path, cyc, cand)
+    for k in range(1, K + 1):
+        sig.refine_data_signatures(dd, k)
+        sig.refine_query_signatures(dq, k)
+        api.refine_candidates(q, dq, dd, sig, cand)
+    gm = api.GMCR(q)
+    gm.generate_gmcr(dq, dd, cand)
+    q.wait()
+    T = gm.total
+    total = api.DeviceBuffer(q, 8, zero=True)
+    pairs = api.DeviceBuffer(q, 8 * max(T, 1), zero=True)
+    reg_ms, cs1_ms, cs2_ms = [], [], []
+    part, n = api.join_partial_candidates(q, dq, dd, cand, end)  # sizes the buffer
+    for it in range(args.warmup + args.steps):
+        total.zero()
+        e = api.join_candidates(q, dq, dd, cand, gm, total, False, pairs, event=True)
+        q.wait()
+        reg_total = int(total.to_numpy(np.uint64)[0])
+        if it >= args.warmup:
+            reg_ms.append(e.elapsed_ms())
+        total.zero()
+        part, n, e1 = api.join_partial_candidates(q, dq, dd, cand, end, capacity=n, event=True)
+        e2 = api.join_with_partial_matches(q, dq, dd, cand, gm, end, part, n, total, pairs, False,
+                                           event=True)
+        q.wait()
+        cs_total = int(total.to_numpy(np.uint64)[0])
+        if it >= args.warmup:
+            cs1_ms.append(e1.elapsed_ms())
+            cs2_ms.append(e2.elapsed_ms())
+    gpu_counts = pairs.to_numpy(np.uint64, T)
+    off, _ = gm.host()
+    log(f"regular join {np.mean(reg_ms):.2f} ms, CS partial {np.mean(cs1_ms):.2f} ms + extend "
+        f"{np.mean(cs2_ms):.2f} ms; {n} partial matches, {T} GMCR pairs, totals {reg_total} / "
+        f"{cs_total}")
+    # parity and CPU baseline on a prefix
+    S = min(args.cpu_sample_graphs or 20_000, D)
+    sub = data.subset(0, S)
+    ref = oracle.staged(queries, sub, K, keep_domains=False)
+    t0 = time.perf_counter()
+    rp = oracle.join_partial(queries, sub, ref.final_domains, end)
+    rt, rc = oracle.join_with_partial(queries, sub, ref.final_domains, ref.gmcr_offsets,
+                                      ref.gmcr_indices, end, rp)
+    cpu_s = time.perf_counter() - t0
+    ok = bool(np.array_equal(gpu_counts[:int(off[S])], rc))
+    log(f"oracle CS join on the first {S} graphs: {cpu_s:.2f}s, counts equal: {ok}")
+    cs_ms = float(np.mean(cs1_ms) + np.mean(cs2_ms))
+    out = {
+        "metric": "CS join (query,graph) pairs/s, exact counts, cluster batch, 1 GPU",
+        "value": round(Q * D / (cs_ms / 1e3), 1), "unit": "pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(cs_ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (project generator, seed 0; cluster_queries seed 2, 5-node prototype)",
+        "config": {"workload": f"{args.workload}: first {D} molecules x cluster of {Q} queries "
+                               f"(prototype {end} nodes), K={K}, count-all, BMS2 --use-cs",
+                   "data_graphs": D, "queries": Q, "refinement_steps": K},
+        "partial_matches": n, "gmcr_pairs": T, "total": cs_total,
+        "partial_ms": round(float(np.mean(cs1_ms)), 3), "extend_ms": round(float(np.mean(cs2_ms)), 3),
+        "regular_join_ms": round(float(np.mean(reg_ms)), 3), "regular_total": reg_total,
+        "cpu_baseline": {"value": round(Q * S / cpu_s, 1), "unit": "pairs/s", "cores": 1,
+                         "kind": "port", "sample": f"oracle CS join (both steps) on the first {S} "
+                                                   f"graphs, domains and GMCR from the oracle"},
+        "parity_on_cpu_sample": ok,
+    }
+    print(json.dumps(out))
+    return 0 if ok else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +169,8 @@ def main() -> int:
                     help="time the learned leg only (profiling runs)")
     ap.add_argument("--plan-graphs", type=int, default=16384,
                     help="data graphs traced to plan the learned order")
+    ap.add_argument("--cs", action="store_true",
+                    help="time the CS (cluster) join instead (BMS2 --use-cs), see cs_bench")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -80,6 +178,9 @@ def main() -> int:
               "would shard data graphs across GPUs with no cross-GPU traffic during compute, and "
               "this build runs and reports one GPU's share", file=sys.stderr)
         return 2
+
+    if args.cs:
+        return cs_bench(args)
 
     from bms2_amd import api, graphs
 

@@ -2,7 +2,9 @@
 // sigmo:: facade in include/sigmo.hpp.  Same stage order (BMS2.cpp:131-234) and the same result
 // lines; flags follow src/utils.hpp:41-59 for the subset implemented here:
 //   -Q <query.dat> -D <data.dat> [-i K] [--find-all] [--max-data-graphs N]
-//   [--max-query-graphs N] [--skip-join] [-p]
+//   [--max-query-graphs N] [--skip-join] [-p] [--use-cs]
+// --use-cs joins as BMS2.cpp:217-300 does: partial matches of the first query (a cluster file's
+// prototype) at depth = its node count, then their extensions to every GMCR query.
 #include <sigmo.hpp>
 
 #include <chrono>
@@ -18,13 +20,14 @@ namespace {
 struct Args {
   std::string query_file, data_file;
   size_t refinement_steps = 0;
-  bool find_all = false, skip_join = false, print_candidates = false;
+  bool find_all = false, skip_join = false, print_candidates = false, use_cs = false;
   size_t max_data_graphs = 1000000, max_query_graphs = 1000;
 };
 
 [[noreturn]] void usage(const char* msg) {
   std::cerr << "bms2_main: " << msg << "\nusage: bms2_main -Q query.dat -D data.dat [-i K] "
-            << "[--find-all] [--max-data-graphs N] [--max-query-graphs N] [--skip-join] [-p]\n";
+            << "[--find-all] [--max-data-graphs N] [--max-query-graphs N] [--skip-join] [-p] "
+            << "[--use-cs]\n";
   std::exit(2);
 }
 
@@ -42,6 +45,7 @@ Args parse(int argc, char** argv) {
     else if (s == "--find-all") a.find_all = true;
     else if (s == "--skip-join") a.skip_join = true;
     else if (s == "-p" || s == "--print-candidates") a.print_candidates = true;
+    else if (s == "--use-cs") a.use_cs = true;
     else if (s == "--max-data-graphs") a.max_data_graphs = std::stoul(next());
     else if (s == "--max-query-graphs") a.max_query_graphs = std::stoul(next());
     else usage(("unknown flag " + s).c_str());
@@ -97,9 +101,33 @@ int main(int argc, char** argv) {
     if (!args.skip_join) {
       sigmo::isomorphism::mapping::GMCR gmcr{queue};
       join_time += gmcr.generateGMCR(dq, dd, candidates).getProfilingInfo();
-      join_time += sigmo::isomorphism::join::joinCandidates(queue, dq, dd, candidates, gmcr,
-                                                            &total_matches, !args.find_all)
-                       .getProfilingInfo();
+      // BMS2.cpp:217-227: --use-cs takes the first query's node count as partial-match depth
+      const size_t depth = args.use_cs && !query_graphs.empty() ? query_graphs[0].getNumNodes() : 0;
+      if (depth == 0) {
+        join_time += sigmo::isomorphism::join::joinCandidates(queue, dq, dd, candidates, gmcr,
+                                                              &total_matches, !args.find_all)
+                         .getProfilingInfo();
+      } else {
+        std::cout << "[Config] --use-cs enabled. partial_match_depth set to " << depth << "\n";
+        // count first, then a buffer of exactly that many records
+        size_t found = 0;
+        try {
+          sigmo::isomorphism::join::joinPartialCandidates2(queue, dq, dd, candidates, depth, nullptr,
+                                                           &found, 0);
+        } catch (const std::exception&) {
+          // capacity 0: the count is returned with the capacity error
+        }
+        int* partial = sigmo::malloc_device<int>(found * (depth + 1), queue);
+        join_time += sigmo::isomorphism::join::joinPartialCandidates2(
+                         queue, dq, dd, candidates, depth, partial, &found, found)
+                         .getProfilingInfo();
+        std::cout << "  > Found " << found << " partial matches.\n";
+        join_time += sigmo::isomorphism::join::joinWithPartialMatches(
+                         queue, dq, dd, candidates, gmcr, depth, partial, found, &total_matches,
+                         !args.find_all)
+                         .getProfilingInfo();
+        sigmo::free(partial, queue);
+      }
     }
     std::cout << "------------- Overall GPU Stats -------------\n"
               << "Signature time: " << ms(sig_time) << " ms\n"

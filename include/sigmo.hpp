@@ -373,6 +373,79 @@ inline utils::BatchedEvent joinCandidates(Queue& queue, DeviceBatchedCSRGraph& q
   be.add(e);
   return be;
 }
+
+// joinPartialCandidates2 (isomorphism.hpp:1021-1178), the first step of BMS2 --use-cs: every
+// embedding of query 0's first `end` nodes, written to device memory `partial_matches` (records
+// of end + 1 ints {data node ids, data graph id}; `capacity` records, e.g. from
+// sigmo::malloc_device<int>(capacity * (end + 1), queue)).  *num_partial_matches receives the
+// number found.  Throws when it exceeds capacity (the reference writes past its buffer).
+inline utils::BatchedEvent joinPartialCandidates2(Queue& queue, DeviceBatchedCSRGraph& query_graphs,
+                                                  DeviceBatchedCSRGraph& data_graphs,
+                                                  candidates::Candidates& candidates, size_t end,
+                                                  int* partial_matches, size_t* num_partial_matches,
+                                                  size_t capacity) {
+  uint64_t n = 0;
+  bms2_event e = nullptr;
+  const int rc = bms2_join_partial_candidates(queue.handle(), &query_graphs, &data_graphs,
+                                              candidates.handle(), uint32_t(end),
+                                              reinterpret_cast<int32_t*>(partial_matches),
+                                              capacity, &n, &e);
+  if (num_partial_matches) *num_partial_matches = size_t(n);
+  check(rc);
+  utils::BatchedEvent be;
+  be.add(e);
+  return be;
+}
+
+// joinWithPartialMatches (isomorphism.hpp:756-977), the second step of BMS2 --use-cs: extends
+// the partial matches to every GMCR query of their data graph; the total is added to
+// *total_full_matches when the call returns.  The reference's pair_done flags ([Q * D]) are kept
+// per GMCR entry inside the call, so pair_done is not read; per_pair (optional) receives one
+// count (find_first: 0/1) per GMCR entry.
+inline utils::BatchedEvent joinWithPartialMatches(Queue& queue, DeviceBatchedCSRGraph& query_graphs,
+                                                  DeviceBatchedCSRGraph& data_graphs,
+                                                  candidates::Candidates& candidates,
+                                                  mapping::GMCR& gmcr, size_t end,
+                                                  const int* partial_matches,
+                                                  size_t num_partial_matches,
+                                                  size_t* total_full_matches, bool find_first,
+                                                  uint32_t* pair_done = nullptr,
+                                                  std::vector<uint64_t>* per_pair = nullptr) {
+  (void)pair_done;
+  void* d_total = nullptr;
+  void* d_pairs = nullptr;
+  const size_t T = gmcr.getTotalQueryIndices();
+  check(bms2_device_malloc(queue.handle(), 8, &d_total));
+  check(bms2_memset_zero(queue.handle(), d_total, 8));
+  check(bms2_device_malloc(queue.handle(), 8 * (T ? T : 1), &d_pairs));
+  bms2_event e = nullptr;
+  int rc = bms2_join_with_partial_matches(
+      queue.handle(), &query_graphs, &data_graphs, candidates.handle(), gmcr.handle(),
+      uint32_t(end), reinterpret_cast<const int32_t*>(partial_matches), num_partial_matches,
+      static_cast<uint64_t*>(d_total), find_first ? 1 : 0, static_cast<uint64_t*>(d_pairs), &e);
+  uint64_t total = 0;
+  if (rc == BMS2_OK) rc = bms2_memcpy_d2h(queue.handle(), &total, d_total, 8);
+  if (rc == BMS2_OK && per_pair) {
+    per_pair->assign(T, 0);
+    if (T) rc = bms2_memcpy_d2h(queue.handle(), per_pair->data(), d_pairs, 8 * T);
+  }
+  bms2_device_free(queue.handle(), d_total);
+  bms2_device_free(queue.handle(), d_pairs);
+  check(rc);
+  if (total_full_matches) *total_full_matches += size_t(total);
+  utils::BatchedEvent be;
+  be.add(e);
+  return be;
+}
 }  // namespace join
 }  // namespace isomorphism
+
+// sycl::malloc_device / sycl::free for the buffers the drivers pass between calls.
+template <typename T>
+T* malloc_device(size_t count, Queue& queue) {
+  void* p = nullptr;
+  check(bms2_device_malloc(queue.handle(), sizeof(T) * (count ? count : 1), &p));
+  return static_cast<T*>(p);
+}
+inline void free(void* p, Queue& queue) { bms2_device_free(queue.handle(), p); }
 }  // namespace sigmo
