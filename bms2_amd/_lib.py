@@ -11,6 +11,8 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libbms2.so"
 
 OK = 0
+ERR_INVALID = 1  # BMS2_ERR_INVALID
+ERR_UNSUPPORTED = 2  # BMS2_ERR_UNSUPPORTED
 ERR_CAPACITY = 7  # BMS2_ERR_CAPACITY: caller buffer too small
 SCOPE_DATA = 0
 SCOPE_QUERY = 1
@@ -124,6 +126,7 @@ PROTOTYPES: dict[str, tuple] = {
     "bms2_memcpy_d2h": (i32, [vp, vp, vp, u64]),
     "bms2_memset_zero": (i32, [vp, vp, u64]),
     "bms2_device_csr_create": (i32, [vp, HCP, DCP]),
+    "bms2_device_csr_pack": (i32, [vp, u32, vp, vp, vp, vp, vp, u64, u64, DCP]),
     "bms2_device_csr_destroy": (i32, [vp, DCP]),
     "bms2_device_csr_alloc_size": (u64, [DCP]),
     "bms2_signature_create": (i32, [vp, u64, u64, vpp]),

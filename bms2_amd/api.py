@@ -168,6 +168,42 @@ class DeviceBatchedCSRGraph:
         q._acquire()
         self.host = host
 
+    @staticmethod
+    def pack(q: Queue, host: GraphBatch) -> "DeviceBatchedCSRGraph":
+        """The same batch packed on the GPU (bms2_device_csr_pack): the graphs' local arrays go
+        up end to end and the device adds the offsets and validates."""
+        self = DeviceBatchedCSRGraph.__new__(DeviceBatchedCSRGraph)
+        self.q = q
+        self.c = _lib.DeviceCsr()
+        nodes, ro_local, ci_local = host.local_arrays()
+        el = np.ascontiguousarray(host.edge_labels, np.uint8)
+        nl = np.ascontiguousarray(host.node_labels, np.uint8)
+        p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        check(_L().bms2_device_csr_pack(q.h, host.num_graphs, p(nodes), p(ro_local), p(ci_local),
+                                        p(nl), p(el), host.total_nodes, len(ci_local),
+                                        C.byref(self.c)))
+        q._acquire()
+        self.host = host
+        return self
+
+    def to_host(self) -> GraphBatch:
+        """The batch's device arrays copied back."""
+        q = self.q
+
+        def get(ptr, n, dt):
+            out = np.empty(n, dt)
+            if n:
+                check(_L().bms2_memcpy_d2h(q.h, out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr),
+                                           out.nbytes))
+            return out
+
+        c = self.c
+        return GraphBatch(get(c.graph_offsets, c.num_graphs + 1, np.uint32),
+                          get(c.row_offsets, c.total_nodes + 1, np.uint32),
+                          get(c.column_indices, c.total_edges, np.uint32),
+                          get(c.node_labels, c.total_nodes, np.uint8),
+                          get(c.edge_labels, c.total_edges, np.uint8))
+
     @property
     def num_graphs(self) -> int:
         return int(self.c.num_graphs)

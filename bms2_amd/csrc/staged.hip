@@ -462,6 +462,75 @@ __global__ __launch_bounds__(256) void join_kernel(
   if (lane_id() == 0 && c) atomicAdd(reinterpret_cast<unsigned long long*>(total), (unsigned long long)c);
 }
 
+// ------------------------------------------------------------------ GPU-side CSR packing
+
+// Entry count of each graph: the local row end of its last node (0 for an empty graph).
+__global__ void pack_entries_kernel(const uint32_t* __restrict__ nodes,
+                                    const uint32_t* __restrict__ goff, uint32_t G,
+                                    const uint32_t* __restrict__ ro_local, uint32_t* __restrict__ ent) {
+  const uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (g < G) ent[g] = nodes[g] ? ro_local[goff[g] + nodes[g] - 1] : 0u;
+}
+
+// Error bits of the packed batch, set by pack_csr_kernel.
+constexpr uint32_t kPackRange = 1, kPackLoop = 2, kPackAsym = 4, kPackRows = 8, kPackSize = 16;
+
+// createDeviceCSRGraph (graph.hpp:251-310) on the GPU: one wave per graph, lanes over its nodes.
+// Global row end = local row end + the graph's first entry, global column id = local id + the
+// graph's first node; every entry is checked (endpoint inside the graph, no self-loop, v lists u
+// as often as u lists v) before anything reads through it.  stat[0] error bits, stat[1] largest
+// graph, stat[2] largest row.
+__global__ __launch_bounds__(64) void pack_csr_kernel(
+    const uint32_t* __restrict__ goff, const uint32_t* __restrict__ eoff, uint32_t G,
+    uint32_t max_nodes, const uint32_t* __restrict__ ro_local, const uint32_t* __restrict__ ci_local,
+    uint32_t* __restrict__ ro, uint32_t* __restrict__ ci, uint32_t* __restrict__ stat) {
+  const uint32_t g = blockIdx.x;
+  if (g >= G) return;
+  const uint32_t lane = lane_id();
+  const uint32_t s = goff[g], n = goff[g + 1] - s, e0 = eoff[g], m = eoff[g + 1] - e0;
+  if (lane == 0) {
+    if (g == 0) ro[0] = 0;
+    atomicMax(stat + 1, n);
+    if (n > max_nodes) atomicOr(stat, kPackSize);
+  }
+  // local row range of node i of this graph, or false when it is not a valid range
+  auto range = [&](uint32_t i, uint32_t* b, uint32_t* e) {
+    *b = i ? ro_local[s + i - 1] : 0u;
+    *e = ro_local[s + i];
+    return *b <= *e && *e <= m;
+  };
+  for (uint32_t i = lane; i < n; i += 64) {
+    uint32_t b, e;
+    const bool ok = range(i, &b, &e);
+    ro[s + i + 1] = ro_local[s + i] + e0;
+    if (!ok) {
+      atomicOr(stat, kPackRows);
+      continue;
+    }
+    atomicMax(stat + 2, e - b);
+    uint32_t err = 0;
+    for (uint32_t k = b; k < e; ++k) {
+      const uint32_t v = ci_local[e0 + k];
+      ci[e0 + k] = v + s;
+      if (v >= n) {
+        err |= kPackRange;
+        continue;
+      }
+      if (v == i) {
+        err |= kPackLoop;
+        continue;
+      }
+      uint32_t vb, ve;
+      if (!range(v, &vb, &ve)) continue;  // flagged by v's own lane
+      uint32_t fwd = 0, back = 0;
+      for (uint32_t j = b; j < e; ++j) fwd += ci_local[e0 + j] == v;
+      for (uint32_t j = vb; j < ve; ++j) back += ci_local[e0 + j] == i;
+      if (fwd != back) err |= kPackAsym;
+    }
+    if (err) atomicOr(stat, err);
+  }
+}
+
 // ------------------------------------------------------------------ CS (cluster) join
 
 // Depth-first enumeration shared by the cluster-mode kernels: maps of depths [d0, n) of one query
@@ -1204,6 +1273,120 @@ int bms2_join_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_
                            d_pair_counts, nullptr))
     return rc;
   return es.end();
+}
+
+// ---- GPU-side CSR packing
+int bms2_device_csr_pack(bms2_queue q, uint32_t num_graphs, const uint32_t* nodes,
+                         const uint32_t* ro_local, const uint32_t* ci_local,
+                         const uint8_t* node_labels, const uint8_t* edge_labels,
+                         uint64_t total_nodes, uint64_t total_edges, bms2_device_csr* out) {
+  if (int rc = use_device(q)) return rc;
+  if (!out || (num_graphs && !nodes) || (total_nodes && (!ro_local || !node_labels)) ||
+      (total_edges && !ci_local))
+    return set_error(BMS2_ERR_INVALID, "null argument");
+  if (total_nodes >= (1ull << 32) || total_edges >= (1ull << 32))
+    return set_error(BMS2_ERR_UNSUPPORTED, "batch exceeds 2^32-1 nodes or CSR entries (u32 ids)");
+  uint64_t sum = 0;
+  for (uint32_t g = 0; g < num_graphs; ++g) sum += nodes[g];
+  if (sum != total_nodes)
+    return set_error(BMS2_ERR_INVALID, "node counts sum to %llu, total_nodes is %llu",
+                     (unsigned long long)sum, (unsigned long long)total_nodes);
+  bms2_device_csr d{};
+  d.num_graphs = num_graphs;
+  d.total_nodes = total_nodes;
+  d.total_edges = total_edges;
+  uint32_t* t_nodes = nullptr;
+  uint32_t* t_ro = nullptr;
+  uint32_t* t_ci = nullptr;
+  uint32_t* t_ent = nullptr;
+  uint32_t* t_eoff = nullptr;
+  uint32_t* t_stat = nullptr;
+  hipStream_t st = q->stream;
+  int rc = BMS2_OK;
+  auto cleanup = [&]() {
+    for (void* p : {(void*)t_nodes, (void*)t_ro, (void*)t_ci, (void*)t_ent, (void*)t_eoff,
+                    (void*)t_stat})
+      (void)hipFree(p);
+  };
+  if ((rc = device_alloc((void**)&d.graph_offsets, 4ull * (num_graphs + 1))) ||
+      (rc = device_alloc((void**)&d.row_offsets, 4ull * (total_nodes + 1))) ||
+      (rc = device_alloc((void**)&d.column_indices, 4ull * total_edges)) ||
+      (rc = device_alloc((void**)&d.node_labels, total_nodes)) ||
+      (rc = device_alloc((void**)&d.edge_labels, total_edges)) ||
+      (rc = device_alloc((void**)&t_nodes, 4ull * num_graphs)) ||
+      (rc = device_alloc((void**)&t_ro, 4ull * total_nodes)) ||
+      (rc = device_alloc((void**)&t_ci, 4ull * total_edges)) ||
+      (rc = device_alloc((void**)&t_ent, 4ull * num_graphs)) ||
+      (rc = device_alloc((void**)&t_eoff, 4ull * (num_graphs + 1))) ||
+      (rc = device_alloc((void**)&t_stat, 16))) {
+    cleanup();
+    bms2_device_csr_destroy(q, &d);
+    return rc;
+  }
+  auto fail = [&](int code) {
+    (void)hipStreamSynchronize(st);
+    cleanup();
+    bms2_device_csr_destroy(q, &d);
+    return code;
+  };
+#define PACK_HIP(x) \
+  if ((x) != hipSuccess) return fail(set_error(BMS2_ERR_HIP, "%s failed", #x))
+  if (num_graphs) {
+    PACK_HIP(hipMemcpyAsync(t_nodes, nodes, 4ull * num_graphs, hipMemcpyHostToDevice, st));
+  }
+  if (total_nodes) {
+    PACK_HIP(hipMemcpyAsync(t_ro, ro_local, 4ull * total_nodes, hipMemcpyHostToDevice, st));
+    PACK_HIP(hipMemcpyAsync(d.node_labels, node_labels, total_nodes, hipMemcpyHostToDevice, st));
+  }
+  if (total_edges) {
+    PACK_HIP(hipMemcpyAsync(t_ci, ci_local, 4ull * total_edges, hipMemcpyHostToDevice, st));
+    if (edge_labels) {
+      PACK_HIP(hipMemcpyAsync(d.edge_labels, edge_labels, total_edges, hipMemcpyHostToDevice, st));
+    } else {
+      PACK_HIP(hipMemsetAsync(d.edge_labels, 0, total_edges, st));
+    }
+  }
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<uint64_t*>(t_stat),
+                     uint64_t(2));
+  uint32_t E = 0;
+  if (num_graphs) {
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, t_nodes, uint64_t(num_graphs),
+                       d.graph_offsets);
+    hipLaunchKernelGGL(pack_entries_kernel, dim3((num_graphs + 255) / 256), dim3(256), 0, st,
+                       t_nodes, d.graph_offsets, num_graphs, t_ro, t_ent);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, t_ent, uint64_t(num_graphs), t_eoff);
+    PACK_HIP(hipMemcpyAsync(&E, t_eoff + num_graphs, 4, hipMemcpyDeviceToHost, st));
+  } else {
+    PACK_HIP(hipMemsetAsync(d.graph_offsets, 0, 4, st));
+  }
+  PACK_HIP(hipStreamSynchronize(st));
+  if (uint64_t(E) != total_edges)
+    return fail(set_error(BMS2_ERR_INVALID, "row ends give %u CSR entries, total_edges is %llu", E,
+                          (unsigned long long)total_edges));
+  if (num_graphs) {
+    hipLaunchKernelGGL(pack_csr_kernel, dim3(num_graphs), dim3(64), 0, st, d.graph_offsets, t_eoff,
+                       num_graphs, BMS2_MAX_GRAPH_NODES, t_ro, t_ci, d.row_offsets,
+                       d.column_indices, t_stat);
+  } else {
+    PACK_HIP(hipMemsetAsync(d.row_offsets, 0, 4, st));
+  }
+  uint32_t stat[4] = {0, 0, 0, 0};
+  PACK_HIP(hipMemcpyAsync(stat, t_stat, 16, hipMemcpyDeviceToHost, st));
+  PACK_HIP(hipStreamSynchronize(st));
+#undef PACK_HIP
+  if (stat[0] & kPackSize)
+    return fail(set_error(BMS2_ERR_UNSUPPORTED, "a graph has %u nodes; the limit is %u", stat[1],
+                          BMS2_MAX_GRAPH_NODES));
+  if (stat[0] & kPackRows) return fail(set_error(BMS2_ERR_INVALID, "local row ends not monotone"));
+  if (stat[0] & kPackRange)
+    return fail(set_error(BMS2_ERR_INVALID, "a CSR entry points outside its graph"));
+  if (stat[0] & kPackLoop) return fail(set_error(BMS2_ERR_UNSUPPORTED, "self-loop edge"));
+  if (stat[0] & kPackAsym) return fail(set_error(BMS2_ERR_INVALID, "CSR is not symmetric"));
+  cleanup();
+  d.max_graph_nodes = stat[1];
+  d.max_degree = stat[2];
+  *out = d;
+  return BMS2_OK;
 }
 
 // ---- CS (cluster) join

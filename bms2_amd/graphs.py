@@ -95,6 +95,18 @@ class GraphBatch:
             edge_labels=self.edge_labels[e0:e1].copy(),
         )
 
+    def local_arrays(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(node count per graph, row end of every node counted from its graph's first CSR
+        entry, neighbour ids local to their graph): the input of bms2_device_csr_pack."""
+        go = np.asarray(self.graph_offsets, np.int64)
+        ro = np.asarray(self.row_offsets, np.int64)
+        sizes = np.diff(go)
+        node_graph = np.repeat(np.arange(self.num_graphs), sizes)
+        ro_local = ro[1:] - ro[go[:-1]][node_graph]
+        ent_graph = np.repeat(node_graph, np.diff(ro))
+        ci_local = np.asarray(self.column_indices, np.int64) - go[:-1][ent_graph]
+        return (sizes.astype(np.uint32), ro_local.astype(np.uint32), ci_local.astype(np.uint32))
+
     def graph(self, g: int) -> tuple[np.ndarray, list[tuple[int, int]]]:
         """(labels, undirected edge list with local ids, u < v) of graph g."""
         n0, n1 = int(self.graph_offsets[g]), int(self.graph_offsets[g + 1])

@@ -139,6 +139,18 @@ typedef struct bms2_device_csr {
 
 /* createDeviceCSRGraph (graph.hpp:251-310): uploads a packed host batch. */
 int bms2_device_csr_create(bms2_queue q, const bms2_host_csr* host, bms2_device_csr* out);
+/* createDeviceCSRGraph with the packing done on the GPU (the reference packs on the host in a
+ * serial loop marked "TODO offload on GPU", graph.hpp:249-310).  The graphs arrive as their own
+ * local arrays laid end to end, with no offsets added: nodes[g] node counts; ro_local[total_nodes]
+ * for each node the end of its neighbour list counted from its graph's first CSR entry (a
+ * CSRGraph's row_offsets[1..n]); ci_local[total_edges] neighbour ids local to the graph;
+ * node_labels; edge_labels (nullable: all 0).  The device turns them into the global batch and
+ * checks what bms2_device_csr_create checks on the host (endpoints inside their graph, no
+ * self-loops, symmetric lists, graphs within BMS2_MAX_GRAPH_NODES). */
+int bms2_device_csr_pack(bms2_queue q, uint32_t num_graphs, const uint32_t* nodes,
+                         const uint32_t* ro_local, const uint32_t* ci_local,
+                         const uint8_t* node_labels, const uint8_t* edge_labels,
+                         uint64_t total_nodes, uint64_t total_edges, bms2_device_csr* out);
 /* destroyDeviceCSRGraph (graph.hpp:312-318). */
 int bms2_device_csr_destroy(bms2_queue q, bms2_device_csr* g);
 /* getDeviceCSRGraphAllocSize (graph.hpp:320-324). */

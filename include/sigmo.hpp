@@ -11,6 +11,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <cstring>
 #include <vector>
 
 #include "bms2.h"
@@ -102,34 +103,36 @@ struct DeviceBatchedCSRGraph : bms2_device_csr {
   DeviceBatchedCSRGraph() : bms2_device_csr{} {}
 };
 
-// createDeviceCSRGraph (graph.hpp:251-310)
+// createDeviceCSRGraph (graph.hpp:251-310) with the packing on the GPU (graph.hpp:249 "TODO
+// offload on GPU"): the host only lays the graphs' own arrays end to end; the device adds the
+// node and entry offsets and validates (bms2_device_csr_pack).
 inline DeviceBatchedCSRGraph createDeviceCSRGraph(Queue& queue, std::vector<CSRGraph>& graphs) {
-  bms2_host_csr h{};
-  std::vector<uint32_t> go(graphs.size() + 1, 0), ro(1, 0), ci;
-  std::vector<uint8_t> nl, el;
+  size_t N = 0, E = 0;
+  for (const CSRGraph& gr : graphs) {
+    N += gr.getNumNodes();
+    E += gr.getRowOffsets()[gr.getNumNodes()];
+  }
+  std::vector<uint32_t> nodes(graphs.size()), ro(N), ci(E);
+  std::vector<uint8_t> nl(N), el(E);
+  size_t n0 = 0, e0 = 0;
   for (size_t g = 0; g < graphs.size(); ++g) {
     const CSRGraph& gr = graphs[g];
-    const uint32_t base = go[g];
-    const size_t n = gr.getNumNodes();
-    const uint32_t ebase = uint32_t(ci.size());
-    for (size_t i = 0; i < n; ++i) ro.push_back(gr.getRowOffsets()[i + 1] + ebase);
-    for (uint32_t k = 0; k < gr.getRowOffsets()[n]; ++k) {
-      ci.push_back(gr.getColumnIndices()[k] + base);
-      el.push_back(gr.getEdgeLabels()[k]);
+    const size_t n = gr.getNumNodes(), m = gr.getRowOffsets()[n];
+    nodes[g] = uint32_t(n);
+    if (n) {
+      std::memcpy(ro.data() + n0, gr.getRowOffsets() + 1, 4 * n);
+      std::memcpy(nl.data() + n0, gr.getNodeLabels(), n);
     }
-    for (size_t i = 0; i < n; ++i) nl.push_back(gr.getNodeLabels()[i]);
-    go[g + 1] = base + uint32_t(n);
+    if (m) {
+      std::memcpy(ci.data() + e0, gr.getColumnIndices(), 4 * m);
+      std::memcpy(el.data() + e0, gr.getEdgeLabels(), m);
+    }
+    n0 += n;
+    e0 += m;
   }
-  h.num_graphs = uint32_t(graphs.size());
-  h.total_nodes = nl.size();
-  h.total_edges = ci.size();
-  h.graph_offsets = go.data();
-  h.row_offsets = ro.data();
-  h.column_indices = ci.data();
-  h.node_labels = nl.data();
-  h.edge_labels = el.data();
   DeviceBatchedCSRGraph d;
-  check(bms2_device_csr_create(queue.handle(), &h, &d));
+  check(bms2_device_csr_pack(queue.handle(), uint32_t(graphs.size()), nodes.data(), ro.data(),
+                             ci.data(), nl.data(), el.data(), N, E, &d));
   return d;
 }
 
