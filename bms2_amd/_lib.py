@@ -111,6 +111,7 @@ PROTOTYPES: dict[str, tuple] = {
     "bms2_version": (i32, []),
     "bms2_host_csr_free": (None, [HCP]),
     "bms2_load_dat": (i32, [C.c_char_p, u64, i32, HCP]),
+    "bms2_parse_dat_lines": (i32, [C.POINTER(C.c_char_p), u64, i32, HCP]),
     "bms2_write_dat": (i32, [C.c_char_p, HCP]),
     "bms2_generate_workload": (i32, [C.POINTER(WorkloadSpec), HCP, HCP]),
     "bms2_queue_create": (i32, [i32, vpp]),

@@ -66,12 +66,13 @@ def build_library(jobs: int | None = None) -> Path:
 
 
 def build_driver() -> Path:
-    """bms2_main: the reference's BMS2 driver written against include/sigmo.hpp."""
-    exe = PKG / "bms2_main"
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", str(ROOT / "include"),
-          str(PKG / "examples" / "bms2_main.cpp"), "-o", str(exe), "-L", str(PKG), "-lbms2",
-          "-Wl,-rpath,$ORIGIN"])
-    return exe
+    """bms2_main and bms2_batch: the reference's BMS2 and BMS2_batch drivers written against
+    include/sigmo.hpp."""
+    for name in ("bms2_main", "bms2_batch"):
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-pthread", "-I", str(ROOT / "include"),
+              str(PKG / "examples" / f"{name}.cpp"), "-o", str(PKG / name), "-L", str(PKG),
+              "-lbms2", "-Wl,-rpath,$ORIGIN"])
+    return PKG / "bms2_main"
 
 
 def build_oracle() -> Path:

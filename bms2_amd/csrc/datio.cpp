@@ -126,6 +126,34 @@ extern "C" int bms2_load_dat(const char* path, uint64_t max_graphs, int skip_mal
   return pack_graphs(graphs, out, 1);
 }
 
+extern "C" int bms2_parse_dat_lines(const char* const* lines, uint64_t num_lines,
+                                    int skip_malformed, bms2_host_csr* out) {
+  using namespace bms2;
+  if (!out || (num_lines && !lines)) return set_error(BMS2_ERR_INVALID, "null argument");
+  std::vector<EdgeListGraph> graphs;
+  graphs.reserve(num_lines);
+  for (uint64_t i = 0; i < num_lines; ++i) {
+    const std::string line = lines[i] ? lines[i] : "";
+    EdgeListGraph g;
+    std::string why;
+    if (line.find_first_not_of(" \t\r\n") == std::string::npos) why = "empty line";
+    if (why.empty()) why = parse_line(line, &g);
+    if (why.empty())
+      for (const auto& e : g.edges)
+        if (e.u == e.v) {
+          why = "self-loop edge (not supported)";
+          break;
+        }
+    if (!why.empty()) {
+      if (skip_malformed) continue;
+      return set_error(BMS2_ERR_FORMAT, "line %llu of the batch: %s", (unsigned long long)i,
+                       why.c_str());
+    }
+    graphs.push_back(std::move(g));
+  }
+  return pack_graphs(graphs, out, 1);
+}
+
 extern "C" int bms2_write_dat(const char* path, const bms2_host_csr* g) {
   using namespace bms2;
   if (!path || !g) return set_error(BMS2_ERR_INVALID, "null argument");

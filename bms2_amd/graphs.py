@@ -307,6 +307,14 @@ def load_dat(path: str, max_graphs: int = 0, skip_malformed: bool = False) -> Gr
     return GraphBatch.from_c(h)
 
 
+def parse_dat_lines(lines: list[str], skip_malformed: bool = False) -> GraphBatch:
+    """io::loadCSRGraphsFromLines (io.hpp:34-41): one graph per line, in memory."""
+    arr = (C.c_char_p * max(len(lines), 1))(*[ln.encode() for ln in lines])
+    h = _lib.HostCsr()
+    _lib.check(_lib.lib().bms2_parse_dat_lines(arr, len(lines), int(skip_malformed), C.byref(h)))
+    return GraphBatch.from_c(h)
+
+
 def write_dat(path: str, g: GraphBatch) -> None:
     h = g.to_c()
     _lib.check(_lib.lib().bms2_write_dat(str(path).encode(), C.byref(h)))

@@ -70,6 +70,12 @@ void bms2_host_csr_free(bms2_host_csr* g);
 int bms2_load_dat(const char* path, uint64_t max_graphs, int skip_malformed, bms2_host_csr* out);
 /* Writes the batch in the same format (the writer of generate_data.py:76-92). */
 int bms2_write_dat(const char* path, const bms2_host_csr* g);
+/* io::loadCSRGraphsFromLines (io.hpp:34-41): the same reader over num_lines lines in memory, one
+ * graph per line (the batch driver's ingestion, BMS2_batch.cpp:104-124).  skip_malformed drops
+ * empty and malformed lines, as the batch driver's try/catch does; otherwise the first one fails
+ * with BMS2_ERR_FORMAT naming its index. */
+int bms2_parse_dat_lines(const char* const* lines, uint64_t num_lines, int skip_malformed,
+                         bms2_host_csr* out);
 
 /* Deterministic synthetic molecule workloads (no reference counterpart: the reference reads
  * RDKit-converted SMARTS files, generate_data.py).  Graph i of the data batch depends only on

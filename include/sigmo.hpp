@@ -145,10 +145,9 @@ inline size_t getDeviceGraphAllocSize(const DeviceBatchedCSRGraph& g) {
 }
 
 namespace io {
-// loadCSRGraphsFromFile (io.hpp:43-52) over the library's .dat reader.
-inline std::vector<CSRGraph> loadCSRGraphsFromFile(const std::string& filename) {
-  bms2_host_csr h{};
-  check(bms2_load_dat(filename.c_str(), 0, 0, &h));
+namespace detail {
+// Splits a packed host batch into per-graph CSRGraphs (local ids, row offsets from 0).
+inline std::vector<CSRGraph> split(bms2_host_csr& h) {
   std::vector<CSRGraph> out;
   out.reserve(h.num_graphs);
   for (uint32_t g = 0; g < h.num_graphs; ++g) {
@@ -164,6 +163,25 @@ inline std::vector<CSRGraph> loadCSRGraphsFromFile(const std::string& filename) 
   }
   bms2_host_csr_free(&h);
   return out;
+}
+}  // namespace detail
+
+// loadCSRGraphsFromFile (io.hpp:43-52) over the library's .dat reader.
+inline std::vector<CSRGraph> loadCSRGraphsFromFile(const std::string& filename) {
+  bms2_host_csr h{};
+  check(bms2_load_dat(filename.c_str(), 0, 0, &h));
+  return detail::split(h);
+}
+
+// loadCSRGraphsFromLines (io.hpp:34-41); skip_malformed drops lines the reader rejects (the
+// batch driver's try/catch around each line, BMS2_batch.cpp:119-122).
+inline std::vector<CSRGraph> loadCSRGraphsFromLines(const std::vector<std::string>& lines,
+                                                    bool skip_malformed = false) {
+  std::vector<const char*> ptrs(lines.size());
+  for (size_t i = 0; i < lines.size(); ++i) ptrs[i] = lines[i].c_str();
+  bms2_host_csr h{};
+  check(bms2_parse_dat_lines(ptrs.data(), ptrs.size(), skip_malformed ? 1 : 0, &h));
+  return detail::split(h);
 }
 }  // namespace io
 

@@ -128,6 +128,25 @@ def test_dat_formats_and_errors(tmp_path):
     assert e.value.code == 4
 
 
+def test_parse_dat_lines_equals_file_reader(tmp_path, molecules_2k):
+    """loadCSRGraphsFromLines (io.hpp:34-41) over the batch driver's lines."""
+    data, _ = molecules_2k
+    sub = data.subset(0, 40)
+    p = tmp_path / "g.dat"
+    graphs.write_dat(str(p), sub)
+    lines = p.read_text().splitlines()
+    a, b = graphs.parse_dat_lines(lines), graphs.load_dat(str(p))
+    for f in ("graph_offsets", "row_offsets", "column_indices", "node_labels", "edge_labels"):
+        assert np.array_equal(getattr(a, f), getattr(b, f))
+    # malformed and empty lines: skipped on request, else the first one is named
+    mixed = [lines[0], "", "n#2 l#11 0 8 1 8 e#2 0 1", lines[1]]
+    assert graphs.parse_dat_lines(mixed, skip_malformed=True).num_graphs == 2
+    with pytest.raises(_lib.Bms2Error) as e:
+        graphs.parse_dat_lines(mixed)
+    assert e.value.code == 5 and "line 1" in str(e.value)
+    assert graphs.parse_dat_lines([]).num_graphs == 0
+
+
 def test_generator_rejects_bad_spec():
     with pytest.raises(_lib.Bms2Error) as e:
         graphs.generate({"num_data_graphs": 10, "num_query_graphs": 4, "data_min_nodes": 5,
