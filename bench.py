@@ -150,6 +150,68 @@ def cs_bench(args) -> int:
     return 0 if ok else 1
 
 
+def ingest_bench(args) -> int:
+    """Upload of a data batch (createDeviceCSRGraph, graph.hpp:251-310) two ways over the C ABI:
+    bms2_device_csr_create (the batch packed on the host, then validated on the host: endpoints,
+    self-loops, symmetry) and bms2_device_csr_pack (the graphs' local arrays uploaded as they are,
+    offsets added and validation done on the GPU).  Both produce the same device arrays, which are
+    compared.  Timed on the host clock around each call, inputs already in host memory."""
+    import ctypes as C
+    from bms2_amd import _lib, api, graphs
+
+    D0 = args.data_graphs or 1_000_000
+    data, _ = graphs.generate(args.workload, seed=0, num_data_graphs=D0)
+    nodes, ro_local, ci_local = data.local_arrays()
+    nl = np.ascontiguousarray(data.node_labels)
+    el = np.ascontiguousarray(data.edge_labels)
+    q = api.Queue(0)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    host = data.to_c()
+    create_ms, pack_ms = [], []
+    for it in range(args.warmup + args.steps):
+        d1, d2 = _lib.DeviceCsr(), _lib.DeviceCsr()
+        t0 = time.perf_counter()
+        _lib.check(_lib.lib().bms2_device_csr_create(q.h, C.byref(host), C.byref(d1)))
+        t1 = time.perf_counter()
+        _lib.check(_lib.lib().bms2_device_csr_pack(q.h, data.num_graphs, p(nodes), p(ro_local),
+                                                   p(ci_local), p(nl), p(el), data.total_nodes,
+                                                   len(ci_local), C.byref(d2)))
+        t2 = time.perf_counter()
+        if it >= args.warmup:
+            create_ms.append(1e3 * (t1 - t0))
+            pack_ms.append(1e3 * (t2 - t1))
+        if it == 0:
+            same = True
+            for f, n, dt in (("graph_offsets", data.num_graphs + 1, np.uint32),
+                             ("row_offsets", data.total_nodes + 1, np.uint32),
+                             ("column_indices", len(ci_local), np.uint32),
+                             ("node_labels", data.total_nodes, np.uint8),
+                             ("edge_labels", len(ci_local), np.uint8)):
+                a, b = np.empty(n, dt), np.empty(n, dt)
+                _lib.check(_lib.lib().bms2_memcpy_d2h(q.h, p(a), C.c_void_p(getattr(d1, f)), a.nbytes))
+                _lib.check(_lib.lib().bms2_memcpy_d2h(q.h, p(b), C.c_void_p(getattr(d2, f)), b.nbytes))
+                same = same and bool(np.array_equal(a, b))
+        _lib.lib().bms2_device_csr_destroy(q.h, C.byref(d1))
+        _lib.lib().bms2_device_csr_destroy(q.h, C.byref(d2))
+    cm, pm = float(np.mean(create_ms)), float(np.mean(pack_ms))
+    print(f"[ingest] {data.num_graphs} graphs, {data.total_nodes} nodes, {len(ci_local)} CSR "
+          f"entries: host-packed create {cm:.1f} ms, GPU pack {pm:.1f} ms, arrays equal: {same}",
+          file=sys.stderr, flush=True)
+    out = {
+        "metric": "data batch upload (createDeviceCSRGraph) graphs/s, GPU-side packing, 1 GPU",
+        "value": round(data.num_graphs / (pm / 1e3), 1), "unit": "graphs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(pm, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (project generator, seed 0)",
+        "config": {"workload": f"{args.workload}: {data.num_graphs} molecules, "
+                               f"{data.total_nodes} nodes, {len(ci_local)} CSR entries"},
+        "host_create_ms": round(cm, 3), "gpu_pack_ms": round(pm, 3),
+        "speedup": round(cm / pm, 2), "arrays_equal": same,
+    }
+    print(json.dumps(out))
+    return 0 if same else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +231,8 @@ def main() -> int:
                     help="time the learned leg only (profiling runs)")
     ap.add_argument("--plan-graphs", type=int, default=16384,
                     help="data graphs traced to plan the learned order")
+    ap.add_argument("--ingest", action="store_true",
+                    help="time the data batch upload instead (host vs GPU packing)")
     ap.add_argument("--cs", action="store_true",
                     help="time the CS (cluster) join instead (BMS2 --use-cs), see cs_bench")
     args = ap.parse_args()
@@ -181,6 +245,8 @@ def main() -> int:
 
     if args.cs:
         return cs_bench(args)
+    if args.ingest:
+        return ingest_bench(args)
 
     from bms2_amd import api, graphs
 

@@ -475,60 +475,67 @@ __global__ void pack_entries_kernel(const uint32_t* __restrict__ nodes,
 // Error bits of the packed batch, set by pack_csr_kernel.
 constexpr uint32_t kPackRange = 1, kPackLoop = 2, kPackAsym = 4, kPackRows = 8, kPackSize = 16;
 
-// createDeviceCSRGraph (graph.hpp:251-310) on the GPU: one wave per graph, lanes over its nodes.
-// Global row end = local row end + the graph's first entry, global column id = local id + the
-// graph's first node; every entry is checked (endpoint inside the graph, no self-loop, v lists u
-// as often as u lists v) before anything reads through it.  stat[0] error bits, stat[1] largest
-// graph, stat[2] largest row.
-__global__ __launch_bounds__(64) void pack_csr_kernel(
-    const uint32_t* __restrict__ goff, const uint32_t* __restrict__ eoff, uint32_t G,
-    uint32_t max_nodes, const uint32_t* __restrict__ ro_local, const uint32_t* __restrict__ ci_local,
-    uint32_t* __restrict__ ro, uint32_t* __restrict__ ci, uint32_t* __restrict__ stat) {
-  const uint32_t g = blockIdx.x;
+// Graph id of every node (one thread per graph writes its range) and the largest graph.
+__global__ void pack_node_graph_kernel(const uint32_t* __restrict__ goff, uint32_t G,
+                                       uint32_t max_nodes, uint32_t* __restrict__ node_graph,
+                                       uint32_t* __restrict__ stat) {
+  const uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (g >= G) return;
-  const uint32_t lane = lane_id();
-  const uint32_t s = goff[g], n = goff[g + 1] - s, e0 = eoff[g], m = eoff[g + 1] - e0;
-  if (lane == 0) {
-    if (g == 0) ro[0] = 0;
-    atomicMax(stat + 1, n);
-    if (n > max_nodes) atomicOr(stat, kPackSize);
-  }
-  // local row range of node i of this graph, or false when it is not a valid range
-  auto range = [&](uint32_t i, uint32_t* b, uint32_t* e) {
-    *b = i ? ro_local[s + i - 1] : 0u;
-    *e = ro_local[s + i];
+  const uint32_t s = goff[g], e = goff[g + 1];
+  for (uint32_t u = s; u < e; ++u) node_graph[u] = uint32_t(g);
+  if (e - s > max_nodes) atomicOr(stat, kPackSize);
+  atomicMax(stat + 1, e - s);
+}
+
+// createDeviceCSRGraph (graph.hpp:251-310) on the GPU, one thread per node: global row end =
+// local row end + its graph's first entry, global column id = local id + its graph's first node;
+// every entry is checked (endpoint inside the graph, no self-loop, v lists u as often as u lists
+// v) before anything reads through it.  stat[0] error bits, stat[2] largest row.
+__global__ void pack_csr_kernel(const uint32_t* __restrict__ goff, const uint32_t* __restrict__ eoff,
+                                const uint32_t* __restrict__ node_graph, uint64_t N,
+                                const uint32_t* __restrict__ ro_local,
+                                const uint32_t* __restrict__ ci_local, uint32_t* __restrict__ ro,
+                                uint32_t* __restrict__ ci, uint32_t* __restrict__ stat) {
+  const uint64_t u = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (u >= N) return;
+  if (u == 0) ro[0] = 0;
+  const uint32_t g = node_graph[u];
+  const uint32_t s = goff[g], i = uint32_t(u) - s, n = goff[g + 1] - s;
+  const uint32_t e0 = eoff[g], m = eoff[g + 1] - e0;
+  // local row range of node j of this graph, or false when it is not a valid range
+  auto range = [&](uint32_t j, uint32_t* b, uint32_t* e) {
+    *b = j ? ro_local[s + j - 1] : 0u;
+    *e = ro_local[s + j];
     return *b <= *e && *e <= m;
   };
-  for (uint32_t i = lane; i < n; i += 64) {
-    uint32_t b, e;
-    const bool ok = range(i, &b, &e);
-    ro[s + i + 1] = ro_local[s + i] + e0;
-    if (!ok) {
-      atomicOr(stat, kPackRows);
+  uint32_t b, e;
+  const bool ok = range(i, &b, &e);
+  ro[u + 1] = ro_local[u] + e0;
+  if (!ok) {
+    atomicOr(stat, kPackRows);
+    return;
+  }
+  if (e - b) atomicMax(stat + 2, e - b);
+  uint32_t err = 0;
+  for (uint32_t k = b; k < e; ++k) {
+    const uint32_t v = ci_local[e0 + k];
+    ci[e0 + k] = v + s;
+    if (v >= n) {
+      err |= kPackRange;
       continue;
     }
-    atomicMax(stat + 2, e - b);
-    uint32_t err = 0;
-    for (uint32_t k = b; k < e; ++k) {
-      const uint32_t v = ci_local[e0 + k];
-      ci[e0 + k] = v + s;
-      if (v >= n) {
-        err |= kPackRange;
-        continue;
-      }
-      if (v == i) {
-        err |= kPackLoop;
-        continue;
-      }
-      uint32_t vb, ve;
-      if (!range(v, &vb, &ve)) continue;  // flagged by v's own lane
-      uint32_t fwd = 0, back = 0;
-      for (uint32_t j = b; j < e; ++j) fwd += ci_local[e0 + j] == v;
-      for (uint32_t j = vb; j < ve; ++j) back += ci_local[e0 + j] == i;
-      if (fwd != back) err |= kPackAsym;
+    if (v == i) {
+      err |= kPackLoop;
+      continue;
     }
-    if (err) atomicOr(stat, err);
+    uint32_t vb, ve;
+    if (!range(v, &vb, &ve)) continue;  // flagged by v's own thread
+    uint32_t fwd = 0, back = 0;
+    for (uint32_t j = b; j < e; ++j) fwd += ci_local[e0 + j] == v;
+    for (uint32_t j = vb; j < ve; ++j) back += ci_local[e0 + j] == i;
+    if (fwd != back) err |= kPackAsym;
   }
+  if (err) atomicOr(stat, err);
 }
 
 // ------------------------------------------------------------------ CS (cluster) join
@@ -1301,11 +1308,12 @@ int bms2_device_csr_pack(bms2_queue q, uint32_t num_graphs, const uint32_t* node
   uint32_t* t_ent = nullptr;
   uint32_t* t_eoff = nullptr;
   uint32_t* t_stat = nullptr;
+  uint32_t* t_ng = nullptr;
   hipStream_t st = q->stream;
   int rc = BMS2_OK;
   auto cleanup = [&]() {
     for (void* p : {(void*)t_nodes, (void*)t_ro, (void*)t_ci, (void*)t_ent, (void*)t_eoff,
-                    (void*)t_stat})
+                    (void*)t_stat, (void*)t_ng})
       (void)hipFree(p);
   };
   if ((rc = device_alloc((void**)&d.graph_offsets, 4ull * (num_graphs + 1))) ||
@@ -1318,7 +1326,8 @@ int bms2_device_csr_pack(bms2_queue q, uint32_t num_graphs, const uint32_t* node
       (rc = device_alloc((void**)&t_ci, 4ull * total_edges)) ||
       (rc = device_alloc((void**)&t_ent, 4ull * num_graphs)) ||
       (rc = device_alloc((void**)&t_eoff, 4ull * (num_graphs + 1))) ||
-      (rc = device_alloc((void**)&t_stat, 16))) {
+      (rc = device_alloc((void**)&t_stat, 16)) ||
+      (rc = device_alloc((void**)&t_ng, 4ull * total_nodes))) {
     cleanup();
     bms2_device_csr_destroy(q, &d);
     return rc;
@@ -1364,9 +1373,14 @@ int bms2_device_csr_pack(bms2_queue q, uint32_t num_graphs, const uint32_t* node
     return fail(set_error(BMS2_ERR_INVALID, "row ends give %u CSR entries, total_edges is %llu", E,
                           (unsigned long long)total_edges));
   if (num_graphs) {
-    hipLaunchKernelGGL(pack_csr_kernel, dim3(num_graphs), dim3(64), 0, st, d.graph_offsets, t_eoff,
-                       num_graphs, BMS2_MAX_GRAPH_NODES, t_ro, t_ci, d.row_offsets,
-                       d.column_indices, t_stat);
+    hipLaunchKernelGGL(pack_node_graph_kernel, dim3((num_graphs + 255) / 256), dim3(256), 0, st,
+                       d.graph_offsets, num_graphs, BMS2_MAX_GRAPH_NODES, t_ng, t_stat);
+    if (total_nodes)
+      hipLaunchKernelGGL(pack_csr_kernel, dim3(uint32_t((total_nodes + 255) / 256)), dim3(256), 0,
+                         st, d.graph_offsets, t_eoff, t_ng, total_nodes, t_ro, t_ci, d.row_offsets,
+                         d.column_indices, t_stat);
+    else
+      PACK_HIP(hipMemsetAsync(d.row_offsets, 0, 4, st));
   } else {
     PACK_HIP(hipMemsetAsync(d.row_offsets, 0, 4, st));
   }
