@@ -36,9 +36,11 @@ constexpr int kSumDwords = 16;   // prefilter summary of one data graph
 constexpr int kSmallNq = 8;      // queries of up to 8 nodes use the small records and join
 constexpr int kSmallWords = 2 + kSmallNq;
 
-// Screening record layout (u32 words): [0] deg | label << 8 | local id << 16 | level mask << 24,
-// [1..8] path Bloom, [9..10] ring counts, [12 + HW k .. 11 + HW (k + 1)] shell level k (HW = 2
-// compact or 4 full words).  Shell words hold the even / odd nibbles of the histogram, one count
+// Screening record layout (u32 words): [0] lower end of the node's key range (label << 8 |
+// degree, or the degree alone for the wildcard), [1..8] path Bloom, [9..10] ring counts, [11]
+// degree | label << 8, [12 + HW k .. 11 + HW (k + 1)] shell level k < K (HW = 2 compact or 4
+// full words), then at A = 12 + HW K: [A] 255 - degree (the range's span), [A + 1] the data key
+// bits compared (0xffff, 0xff for the wildcard), [A + 2] local id * 8 (its domain's byte offset).  Shell words hold the even / odd nibbles of the histogram, one count
 // per byte; the screen keeps the data node's counts as 15 - d per byte, so one 64-bit add over
 // eight labels gives q + 15 - d per byte (at most 30: no carry between bytes), with bit 4 set
 // exactly when q > d.
@@ -146,36 +148,37 @@ __global__ __launch_bounds__(64) void query_prep_kernel(QueryPrepParams p) {
   uint32_t g = gb;
   while (p.graph_offsets[g + 1] <= node) ++g;
   const uint32_t local = node - p.graph_offsets[g];
-  uint32_t lm = 0;
-#pragma unroll
-  for (int k = 0; k < kMaxLevels; ++k)
-    if (t.hist[k]) lm |= 1u << k;
+  const uint32_t qd = min(t.deg, 255u), lab = t.label & 255u;
+  const bool wild = lab == kWildcard;
   uint32_t* r = p.qrec + size_t(p.rec_pos[node]) * kRecDwords;
-  r[0] = min(t.deg, 255u) | ((t.label & 255u) << 8) | (local << 16) | (lm << 24);
+  r[0] = wild ? qd : (lab << 8) | qd;  // lower end of the (label, degree) key range
 #pragma unroll
   for (int w = 0; w < 8; ++w) r[1 + w] = t.path[w];
   r[9] = uint32_t(t.cycle);
   r[10] = uint32_t(t.cycle >> 32);
-  r[11] = 0;
+  r[11] = qd | (lab << 8);
+  const uint32_t hw = p.compact ? 2u : 4u, aux = 12 + hw * K;
   if (p.compact) {
 #pragma unroll
-    for (int k = 0; k < kMaxLevels; ++k) {
+    for (int k = 0; k < K; ++k) {
       const uint32_t c = compact_hist(t.hist[k], p.hist_map);
       r[12 + 2 * k + 0] = c & 0x0f0f0f0fu;
       r[12 + 2 * k + 1] = (c >> 4) & 0x0f0f0f0fu;
     }
-    for (int w = 12 + 2 * kMaxLevels; w < kRecDwords; ++w) r[w] = 0;
   } else {
 #pragma unroll
-    for (int k = 0; k < kMaxLevels; ++k) {
+    for (int k = 0; k < K; ++k) {
       const uint32_t lo = uint32_t(t.hist[k]), hi = uint32_t(t.hist[k] >> 32);
       r[12 + 4 * k + 0] = lo & 0x0f0f0f0fu;
       r[12 + 4 * k + 1] = (lo >> 4) & 0x0f0f0f0fu;
       r[12 + 4 * k + 2] = hi & 0x0f0f0f0fu;
       r[12 + 4 * k + 3] = (hi >> 4) & 0x0f0f0f0fu;
     }
-    for (int w = 12 + 4 * kMaxLevels; w < kRecDwords; ++w) r[w] = 0;
   }
+  r[aux + 0] = 255u - qd;                 // span of the key range
+  r[aux + 1] = wild ? 0xffu : 0xffffu;    // key bits compared: degree only for the wildcard
+  r[aux + 2] = local * 8u;                // byte offset of the node's domain in qdom
+  for (uint32_t w = aux + 3; w < uint32_t(kRecDwords); ++w) r[w] = 0;
 }
 
 // Per-query prefilter record (kPreDwords): [0..3] presence bits every node needs (label < 16 at
@@ -189,7 +192,7 @@ __global__ void query_pre_kernel(const uint32_t* __restrict__ qhead, const uint3
   uint32_t out[kPreDwords] = {};
   for (uint32_t i = 0; i < nq; ++i) {
     const uint32_t* r = qrec + size_t(rb + i) * kRecDwords;
-    const uint32_t d = r[0] & 255u, l = (r[0] >> 8) & 255u;
+    const uint32_t d = r[11] & 255u, l = (r[11] >> 8) & 255u;
     if (l < 16) {
       const uint32_t bit = l * 8 + min(d, 7u);
       out[bit >> 5] |= 1u << (bit & 31);
@@ -394,20 +397,23 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
       uint64_t lanes = ballot((gm & myslotbit) != 0);
       uint64_t tops = lanes & T;
       // domains by local query node id go to qdom (written by lane 0)
+      // nq >= 2 here (the prefilter passes only queries with more than one node); the loop
+      // ends after the last node or as soon as no graph is left (then `left` is set to 1 on the
+      // rare path where a graph dies, keeping the common path to one counter test)
       const uint32_t* r = p.qrec + size_t(rb) * kRecDwords;
-      for (uint32_t ti = 0; ti < nq; ++ti, r += kRecDwords) {
+      uint32_t left = nq;
+      do {
         // the whole record in one batch of scalar loads, then a branch-free test
-        uint32_t w[12 + HW * K];
+        constexpr int A = 12 + HW * K;  // key span, key mask, domain offset
+        uint32_t w[A + 3];
 #pragma unroll
-        for (int x = 0; x < 12 + HW * K; ++x) w[x] = cload(r, x);
+        for (int x = 0; x < A + 3; ++x) w[x] = cload(r, x);
         // keep the record's loads in one batch ahead of the uniform branches below
 #pragma unroll
-        for (int x = 0; x < 12 + HW * K; ++x) asm volatile("" ::"s"(w[x]));
-        const uint32_t qkey = w[0] & 0xffffu, qd = w[0] & 255u, local = (w[0] >> 16) & 255u;
-        // label equal (any label for the wildcard) and degree at least the query node's, each a
-        // single compare: key - (ql << 8 | qd) <= 255 - qd exactly when label == ql, deg >= qd
-        uint64_t m = ((qkey >> 8) == kWildcard ? ballot(d_deg >= qd)
-                                               : ballot(d_key - qkey <= 255u - qd)) & lanes;
+        for (int x = 0; x < A + 3; ++x) asm volatile("" ::"s"(w[x]));
+        // label equal (any label for the wildcard) and degree at least the query node's in one
+        // range compare on the masked key: (key & mask) - lo <= span
+        uint64_t m = ballot(((d_key & w[A + 1]) - w[0]) <= w[A]) & lanes;
         if (m) {
           uint32_t acc = 0;
 #pragma unroll
@@ -425,17 +431,19 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
           fail |= uint32_t(sh | (sh >> 32)) & 0x10101010u;
           m &= ballot((acc | fail) == 0);
         }
-        if (lane == 0) qdom[local] = m;
+        if (lane == 0) *reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(qdom) + w[A + 2]) = m;
         // Graphs left with a candidate: per segment, (m | top) - first has its top bit set
         // unless every non-top bit of m is clear, in which case m's own top bit decides.
+        // m has bits only in lanes of live graphs, so a dead graph's top stays clear.
         const uint64_t u = (m | T) - S;
-        const uint64_t alive = ((u | m) & T) & tops;
+        const uint64_t alive = (u | m) & T;
         if (alive != tops) {
           tops = alive;
-          if (!tops) break;
+          if (!tops) left = 1;
           lanes = ballot(mytop < 64 && ((tops >> mytop) & 1)) & node_mask;
         }
-      }
+        r += kRecDwords;
+      } while (--left);
       if (!tops) continue;
       if (lane < ng && slot_top < 64 && ((tops >> slot_top) & 1)) vbits |= uint64_t(1) << (qu & 63);
       __syncthreads();  // lane 0's qdom writes visible to every lane (one wave: no waiting)
