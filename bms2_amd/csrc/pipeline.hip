@@ -35,6 +35,7 @@ constexpr int kPreDwords = 20;   // prefilter record of one query
 constexpr int kSumDwords = 16;   // prefilter summary of one data graph
 constexpr int kSmallNq = 8;      // queries of up to 8 nodes use the small records and join
 constexpr int kSmallWords = 2 + kSmallNq;
+constexpr int kRingSlotShift = 28;  // small-join queue entry: record index | slot << 28
 
 // Screening record layout (u32 words): [0] lower end of the node's key range (label << 8 |
 // degree, or the degree alone for the wildcard), [1..8] path Bloom, [9..10] ring counts, [11]
@@ -629,101 +630,149 @@ struct JoinSmallParams {
 // Depth-first join of the small-query records (up to kSmallNq nodes), one (query, graph) pair
 // per lane (isomorphism.hpp:291-472 semantics with standard backtracking).  A wave claims whole
 // segments (the small records one screen flush wrote, all of one pack) into S slots, stages each
-// slot's pack adjacency rows in LDS and queues the segment's records in an LDS ring; idle lanes
-// take records from the ring once kBatch of them are idle, so the wave pays the record loads'
-// latency once per batch.  A record {q | g << 32, query adjacency by depth | nq, domains} is
-// read in one round of independent loads; its domain words go to LDS and its query adjacency to
-// one register, so a DFS step touches LDS only.  USE[s] counts the records of slot s not yet
-// finished; a slot takes a new segment when it reaches zero.
+// slot's pack adjacency rows in LDS and queues the segment's records in an LDS ring.  Global
+// memory is touched only at handouts, which happen once kBatch lanes are idle: idle lanes start
+// the record they hold in reserve (in registers, loaded at an earlier handout), the pairs
+// finished since the last handout store their counts, and lanes without a reserve take ring
+// entries and issue their loads, which nothing waits for until the next handout.  Segments are
+// staged the same way: descriptors arrive a claim (kSegClaim segments) at a time, and the next
+// segment's adjacency row is loaded one refill before a slot takes it.  A started
+// record's domain words go to LDS and its query adjacency to one register, so a DFS step touches
+// LDS only.  USE[s] counts the records of slot s not yet finished (reserved ones included); a
+// slot takes a new segment when it reaches zero.
 template <int S>
 __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
+  // A query of nq nodes (2 <= nq <= NQ) takes candidates at depths 0 .. nq - 2 and descends
+  // from depths 0 .. nq - 3, so the per-depth arrays stop at NQ - 2 (candidates) and NQ - 3
+  // (matches); X[d] = D[d + 1] & (constraint on depth d + 1 from depths < d) & ~(nodes matched
+  // at depths < d), set when the search descends to d, so a step at depth d reads X[d] alone.
   constexpr int NQ = kSmallNq;
-  constexpr uint32_t kRing = 256;  // >= S * (largest segment)
+  constexpr uint32_t kRing = 256;
   constexpr uint32_t kBatch = 16;  // idle lanes that trigger a handout
   constexpr uint32_t kSegClaim = 8;  // segments claimed per atomic on the global cursor
-  static_assert(S * kSegRecs <= int(kRing), "ring must hold every slot's segment");
-  __shared__ uint64_t R[NQ][64];   // remaining candidates per depth
-  __shared__ uint64_t NF[NQ][64];  // neighbour row of the node matched per depth
-  __shared__ uint64_t D[NQ][64];   // domain per depth (matching order)
-  __shared__ uint64_t PC[NQ][64];  // per depth d: constraint on depth d + 1 from depths < d
-  __shared__ uint64_t ADJ[S][64];  // adjacency rows of each slot's pack
-  __shared__ uint32_t RING[kRing];  // queued record index | slot << 29
+  // a refill pushes a segment only while fewer than 128 entries are queued
+  static_assert(128 + kSegRecs <= int(kRing), "ring must hold what a refill may queue");
+  static_assert(S <= (1 << (32 - kRingSlotShift)), "slot must fit the ring entry");
+  __shared__ uint64_t R[NQ - 1][64];   // remaining candidates per depth
+  __shared__ uint64_t X[NQ - 1][64];   // candidate base per depth (above)
+  __shared__ uint64_t D[NQ - 2][64];   // domain of depth d + 2 (matching order)
+  __shared__ uint64_t NF[NQ - 2][64];  // neighbour row of the node matched per depth
+  __shared__ uint64_t ADJ[S][64];      // adjacency rows of each slot's pack
+  __shared__ uint32_t RING[kRing];     // queued record index | slot << kRingSlotShift
   __shared__ uint32_t USE[S];
-  __shared__ uint8_t F[NQ][64];    // node matched per depth
+  __shared__ uint8_t F[NQ - 2][64];    // node matched per depth
   const uint32_t lane = lane_id();
   if (p.ctl[kCtlOverflow]) return;  // the host redoes the chunk
   const uint64_t nseg = min(p.ctl[kCtlSmall] >> kSegShift, p.cap_seg);
   if (lane < uint32_t(S)) USE[lane] = 0;
   __syncthreads();
   uint32_t qh = 0, qt = 0;  // ring head and tail (uniform)
-  uint64_t lseg = 0, lseg_end = 0;  // segments claimed by this wave, not yet in a slot
-  bool exhausted = false, busy = false;
-  uint32_t slot = 0, q = 0, g = 0, nq = 0;
+  // Segments claimed by this wave and not yet staged: [lseg, lseg_end); lane i holds the
+  // descriptor of segment cbase + i.  The staged segment (st_ok) has its adjacency row load
+  // issued one refill ahead of the slot that takes it.
+  uint64_t lseg = 0, lseg_end = 0, cbase = 0;
+  uint4 cdesc = make_uint4(0, 0, 0, 0);
+  bool st_ok = false;
+  uint32_t st_first = 0, st_count = 0;
+  uint64_t st_adj = 0;
+  const auto stage = [&]() {
+    if (lseg >= lseg_end) {
+      uint64_t base = 0;
+      if (lane == 0)
+        base = atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + kCtlSegCursor),
+                         (unsigned long long)kSegClaim);
+      base = uniform64(readlane64(base, 0));
+      cbase = lseg = base;
+      lseg_end = base >= nseg ? base : min(base + kSegClaim, nseg);
+      if (lane < lseg_end - lseg) cdesc = p.segs[lseg + lane];
+    }
+    if (lseg < lseg_end) {
+      const int i = int(lseg - cbase);
+      st_first = __builtin_amdgcn_readlane(cdesc.x, i);
+      st_count = __builtin_amdgcn_readlane(cdesc.y, i);
+      const uint32_t rows = __builtin_amdgcn_readlane(cdesc.z, i);
+      const uint32_t nn = __builtin_amdgcn_readlane(cdesc.w, i);
+      st_adj = lane < nn ? p.adj[rows + lane] : 0;
+      st_ok = true;
+      ++lseg;
+    }
+  };
+  bool exhausted = false, busy = false, pf = false, has_res = false;
+  uint32_t slot = 0, pslot = 0, q = 0, g = 0, nq = 0;
   int depth = 0;
   uint64_t used = 0, cnt = 0, nodes = 0, qa = 0;
+  uint64_t pw[kSmallWords];  // the lane's reserved record, loaded one handout ahead
+#pragma unroll
+  for (int x = 0; x < kSmallWords; ++x) pw[x] = 0;
+  uint64_t res_at = 0, res = 0;  // count of the lane's last finished pair, stored at the handout
   uint64_t st_matched = 0, st_emb = 0, st_nodes = 0;
   while (true) {
     const uint64_t idle = ballot(!busy);
     const uint32_t nidle = uint32_t(popc64(idle));
     if (nidle >= kBatch || nidle == 64) {
-      if (!exhausted && qt - qh < nidle) {
-        // refill free slots (nothing queued or running) with new segments
-        bool loaded = false;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-          if (!exhausted && qt - qh < nidle + 64 && uniform(USE[s]) == 0) {
-            if (lseg >= lseg_end) {
-              uint64_t base = 0;
-              if (lane == 0)
-                base = atomicAdd(reinterpret_cast<unsigned long long*>(p.ctl + kCtlSegCursor),
-                                 (unsigned long long)kSegClaim);
-              base = uniform64(readlane64(base, 0));
-              lseg = base;
-              lseg_end = base >= nseg ? base : min(base + kSegClaim, nseg);
-            }
-            const uint64_t si = lseg++;
-            if (si >= lseg_end) {
-              exhausted = true;
-            } else {
-              const uint4 sd = p.segs[si];
-              const uint32_t first = uniform(sd.x), count = uniform(sd.y);
-              ADJ[s][lane] = lane < sd.w ? p.adj[sd.z + lane] : 0;
-              for (uint32_t e = lane; e < count; e += 64)
-                RING[(qt + e) % kRing] = (first + e) | (uint32_t(s) << 29);
-              if (lane == 0) USE[s] = count;
-              qt += count;
-              loaded = true;
-            }
-          }
-        }
-        if (loaded) __syncthreads();  // ADJ rows, ring entries and USE visible to every lane
-      }
-      const uint32_t take = min(qt - qh, nidle);
-      const uint32_t rank = uint32_t(popc64(idle & low_mask(lane)));
-      if (!busy && rank < take) {
+      // Global memory is touched only here: the reserved records' loads and the result stores
+      // of the previous handout are waited for (long landed), the new ones are not.
+      if (!busy && pf) {
         busy = true;
-        const uint32_t ent = RING[(qh + rank) % kRing];
-        slot = ent >> 29;
-        const uint64_t* r = p.recs + uint64_t(ent & ((1u << 29) - 1)) * kSmallWords;
-        uint64_t w[kSmallWords];
+        pf = false;
+        slot = pslot;
+        q = uint32_t(pw[0]);
+        g = uint32_t(pw[0] >> 32);
+        nq = uint32_t(pw[1] & 0xffu);
+        qa = pw[1];  // byte d >= 1: earlier depths adjacent to depth d
 #pragma unroll
-        for (int x = 0; x < kSmallWords; ++x) w[x] = r[x];  // words past 2 + nq are unused
-        q = uint32_t(w[0]);
-        g = uint32_t(w[0] >> 32);
-        nq = uint32_t(w[1] & 0xffu);
-        qa = w[1];  // byte d >= 1: earlier depths adjacent to depth d
-#pragma unroll
-        for (int d = 0; d < NQ; ++d) D[d][lane] = w[2 + d];
-        R[0][lane] = w[2];
-        PC[0][lane] = ~uint64_t(0);
+        for (int d = 0; d < NQ - 2; ++d) D[d][lane] = pw[4 + d];  // words past 2 + nq are unused
+        R[0][lane] = pw[2];
+        X[0][lane] = pw[3];
         depth = 0;
         used = 0;
         cnt = 0;
         nodes = 0;
       }
+      const uint64_t want = ballot(!pf);  // lanes with no record in reserve
+      const uint32_t nwant = uint32_t(popc64(want));
+      if (!exhausted && qt - qh < nwant) {
+        // refill free slots (nothing queued, reserved or running) with the staged segment, and
+        // stage the next one
+        bool loaded = false;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          if (!exhausted && qt - qh < nwant + 64 && uniform(USE[s]) == 0) {
+            if (!st_ok) stage();
+            if (!st_ok) {
+              exhausted = true;
+            } else {
+              ADJ[s][lane] = st_adj;
+              for (uint32_t e = lane; e < st_count; e += 64)
+                RING[(qt + e) % kRing] = (st_first + e) | (uint32_t(s) << kRingSlotShift);
+              if (lane == 0) USE[s] = st_count;
+              qt += st_count;
+              loaded = true;
+              st_ok = false;
+              stage();
+            }
+          }
+        }
+        if (loaded) __syncthreads();  // ADJ rows, ring entries and USE visible to every lane
+      }
+      if (has_res) {
+        p.counts[res_at] = res;
+        has_res = false;
+      }
+      const uint32_t take = min(qt - qh, nwant);
+      const uint32_t rank = uint32_t(popc64(want & low_mask(lane)));
+      if (!pf && rank < take) {
+        // issue the record's loads; nothing waits for them before the next handout
+        pf = true;
+        const uint32_t ent = RING[(qh + rank) % kRing];
+        pslot = ent >> kRingSlotShift;
+        const uint64_t* r = p.recs + uint64_t(ent & ((1u << kRingSlotShift) - 1)) * kSmallWords;
+#pragma unroll
+        for (int x = 0; x < kSmallWords; ++x) pw[x] = r[x];
+      }
       qh += take;
       if (!ballot(busy)) {
-        if (exhausted && qh == qt) break;
+        if (exhausted && qh == qt && !ballot(pf)) break;
         continue;
       }
     }
@@ -746,13 +795,12 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
         const int nd = depth + 1;
         const uint64_t vbit = uint64_t(1) << v;
         const uint32_t am = uint32_t(qa >> (8 * nd)) & 0xffu;
-        const uint64_t Rn = D[nd][lane] & ~(used | vbit) & PC[depth][lane] &
-                            (((am >> depth) & 1) ? nv : ~nv);
+        const uint64_t Rn = X[depth][lane] & ~vbit & (((am >> depth) & 1) ? nv : ~nv);
         if (uint32_t(nd) == nq - 1) {
           cnt += uint64_t(popc64(Rn));
           if (p.find_first && cnt) finished = true;
         } else {
-          // descend: record the match, and the constraint on depth nd + 1 from depths <= depth
+          // descend: record the match, and the candidate base of depth nd
           F[depth][lane] = uint8_t(v);
           NF[depth][lane] = nv;
           const uint32_t an = uint32_t(qa >> (8 * (nd + 1))) & 0xffu;
@@ -761,8 +809,8 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
             const uint64_t x = NF[i][lane];
             c &= ((an >> i) & 1) ? x : ~x;
           }
-          PC[nd][lane] = c;
           used |= vbit;
+          X[nd][lane] = D[nd - 1][lane] & c & ~used;
           depth = nd;
           R[nd][lane] = Rn;
         }
@@ -770,7 +818,9 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
     }
     if (finished) {
       const uint64_t c = p.find_first ? (cnt ? 1 : 0) : cnt;
-      p.counts[uint64_t(g) * p.Q + q] = c;
+      res_at = uint64_t(g) * p.Q + q;  // a finished lane stays idle until the next handout
+      res = c;
+      has_res = true;
       if (p.qnodes) atomicAdd(reinterpret_cast<unsigned long long*>(p.qnodes + q), (unsigned long long)nodes);
       if (p.pair_nodes) p.pair_nodes[uint64_t(g) * p.Q + q] = uint32_t(min(nodes, uint64_t(0xffffffffu)));
       atomicSub(&USE[slot], 1u);
@@ -981,7 +1031,7 @@ struct bms2_pipeline_s {
   double dens_s = 0.0, dens_seg = 0.0, dens_l = 0.0;
   // launch shape (autotuned): waves per CU of each join, LDS slots per small-join wave
   int cus = 256;
-  uint32_t small_waves = 8, large_waves = 12, slots = 8;
+  uint32_t small_waves = 7, large_waves = 12, slots = 16;
   uint32_t* d_packs = nullptr;      // pack p: graphs d_pack_gids[d_packs[p] .. d_packs[p + 1])
   uint32_t* d_pack_gids = nullptr;
   std::vector<uint32_t> win_pack, win_graph;  // pack windows (see build_data_packs)
@@ -1340,8 +1390,8 @@ int bind_queries(bms2_pipeline_s* p, const bms2_device_csr* query) {
   const double share_l = q_small == Q ? 0.0 : (q_small == 0 ? 0.9 : 0.15);
   const uint64_t D = p->data.num_graphs;
   const uint64_t worst_s = D * q_small, worst_l = D * (Q - q_small);
-  // record indices travel in 29 bits through the small join's queue
-  p->cap_s = std::max<uint64_t>(1, std::min<uint64_t>({worst_s, uint64_t(0.95 * (1 - share_l) * budget) / (8 * kSmallWords), (1ull << 29) - 1}));
+  // record indices travel in kRingSlotShift bits through the small join's queue
+  p->cap_s = std::max<uint64_t>(1, std::min<uint64_t>({worst_s, uint64_t(0.95 * (1 - share_l) * budget) / (8 * kSmallWords), (1ull << kRingSlotShift) - 1}));
   p->cap_seg = std::max<uint64_t>(1, std::min<uint64_t>({p->cap_s, uint64_t(0.05 * (1 - share_l) * budget) / 16,
                                                           (uint64_t(1) << (64 - kSegShift)) - 1}));
   p->cap_l = std::max<uint64_t>(1, std::min<uint64_t>(worst_l, uint64_t(share_l * budget) / (8ull * p->rec_words_l)));
@@ -1419,9 +1469,9 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
   p->cus = cus;
-  // defaults: as many waves as LDS allows (small join ~22 KB per wave: 7 per CU)
+  // defaults: as many waves as LDS allows (small join with 16 slots ~22.5 KB per wave: 7 per CU)
   p->small_waves = opt->block_waves ? opt->block_waves : 7;
-  p->slots = 8;
+  p->slots = 16;
   if ((rc = bind_queries(p, query))) return fail(rc);
   *out = p;
   return BMS2_OK;
@@ -1538,10 +1588,10 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
         hipLaunchKernelGGL((screen_kernel<K, 4>), dim3(p1 - p0), dim3(64), stage_bytes, s, sp);
       BMS2_CHECK_LAUNCH("screen_kernel");
       const dim3 gs(uint32_t(p->cus) * p->small_waves), gl(uint32_t(p->cus) * p->large_waves);
-      if (p->slots == 4)
-        hipLaunchKernelGGL(join_small_kernel<4>, gs, dim3(64), 0, s, js);
-      else
+      if (p->slots == 8)
         hipLaunchKernelGGL(join_small_kernel<8>, gs, dim3(64), 0, s, js);
+      else
+        hipLaunchKernelGGL(join_small_kernel<16>, gs, dim3(64), 0, s, js);
       BMS2_CHECK_LAUNCH("join_small_kernel");
       if (p->max_nq > uint32_t(kSmallNq)) {
         if (p->nq_cap <= 16)
@@ -1681,9 +1731,9 @@ extern "C" int bms2_pipeline_autotune(bms2_pipeline p, uint64_t* d_counts, uint6
   };
   std::vector<Cand> cands;
   const uint32_t lw0 = p->nq_cap <= 16 ? 12 : 6;
-  for (uint32_t sw : {5u, 6u, 7u})
-    for (uint32_t sl : {4u, 8u}) cands.push_back({sw, lw0, sl});
-  for (uint32_t lw : {lw0 / 2, lw0 * 3 / 4}) cands.push_back({7, lw, 8});
+  for (uint32_t sw : {6u, 7u, 8u})
+    for (uint32_t sl : {8u, 16u}) cands.push_back({sw, lw0, sl});
+  for (uint32_t lw : {lw0 / 2, lw0 * 3 / 4}) cands.push_back({7, lw, 16});
   const Cand keep{p->small_waves, p->large_waves, p->slots};
   float best = 0.f;
   Cand best_c = keep;

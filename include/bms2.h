@@ -321,7 +321,7 @@ int bms2_pipeline_query_totals(bms2_pipeline p, const uint64_t* d_counts, uint64
 typedef struct bms2_pipeline_tuning {
   uint32_t small_waves; /* waves per CU of the small-query join */
   uint32_t large_waves; /* waves per CU of the large-query join */
-  uint32_t slots;       /* pack-adjacency slots per small-join wave (4 or 8) */
+  uint32_t slots;       /* pack-adjacency slots per small-join wave (8 or 16) */
   uint32_t candidates;  /* configurations timed */
   float best_ms;        /* fastest kernel span of one run */
 } bms2_pipeline_tuning;
