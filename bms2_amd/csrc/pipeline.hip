@@ -643,9 +643,12 @@ struct JoinSmallParams {
 template <int S>
 __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
   // A query of nq nodes (2 <= nq <= NQ) takes candidates at depths 0 .. nq - 2 and descends
-  // from depths 0 .. nq - 3, so the per-depth arrays stop at NQ - 2 (candidates) and NQ - 3
-  // (matches); X[d] = D[d + 1] & (constraint on depth d + 1 from depths < d) & ~(nodes matched
-  // at depths < d), set when the search descends to d, so a step at depth d reads X[d] alone.
+  // from depths 0 .. nq - 3.  The current depth's remaining candidates (cur) and candidate base
+  // (xcur) live in registers; X[d] = D[d + 1] & (constraint on depth d + 1 from depths < d) &
+  // ~(nodes matched at depths < d) is set when the search descends to d.  Descending from depth
+  // d saves the remaining candidates of d in RS[d] (bit d of `saved` when non-empty), so
+  // backtracking jumps straight to the deepest saved depth; the node matched at depth d is
+  // 6-bit field d of `fpack`, its neighbour row NF[d] (read by deeper descents only).
   constexpr int NQ = kSmallNq;
   constexpr uint32_t kRing = 256;
   constexpr uint32_t kBatch = 16;  // idle lanes that trigger a handout
@@ -653,14 +656,13 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
   // a refill pushes a segment only while fewer than 128 entries are queued
   static_assert(128 + kSegRecs <= int(kRing), "ring must hold what a refill may queue");
   static_assert(S <= (1 << (32 - kRingSlotShift)), "slot must fit the ring entry");
-  __shared__ uint64_t R[NQ - 1][64];   // remaining candidates per depth
-  __shared__ uint64_t X[NQ - 1][64];   // candidate base per depth (above)
+  __shared__ uint64_t RS[NQ - 2][64];  // saved remaining candidates per depth
+  __shared__ uint64_t X[NQ - 2][64];   // candidate base per depth (above)
   __shared__ uint64_t D[NQ - 2][64];   // domain of depth d + 2 (matching order)
-  __shared__ uint64_t NF[NQ - 2][64];  // neighbour row of the node matched per depth
+  __shared__ uint64_t NF[NQ - 3][64];  // neighbour row of the node matched per depth
   __shared__ uint64_t ADJ[S][64];      // adjacency rows of each slot's pack
   __shared__ uint32_t RING[kRing];     // queued record index | slot << kRingSlotShift
   __shared__ uint32_t USE[S];
-  __shared__ uint8_t F[NQ - 2][64];    // node matched per depth
   const uint32_t lane = lane_id();
   if (p.ctl[kCtlOverflow]) return;  // the host redoes the chunk
   const uint64_t nseg = min(p.ctl[kCtlSmall] >> kSegShift, p.cap_seg);
@@ -700,7 +702,8 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
   bool exhausted = false, busy = false, pf = false, has_res = false;
   uint32_t slot = 0, pslot = 0, q = 0, g = 0, nq = 0;
   int depth = 0;
-  uint64_t used = 0, cnt = 0, nodes = 0, qa = 0;
+  uint64_t used = 0, cnt = 0, nodes = 0, qa = 0, cur = 0, xcur = 0, fpack = 0;
+  uint32_t saved = 0;
   uint64_t pw[kSmallWords];  // the lane's reserved record, loaded one handout ahead
 #pragma unroll
   for (int x = 0; x < kSmallWords; ++x) pw[x] = 0;
@@ -722,8 +725,10 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
         qa = pw[1];  // byte d >= 1: earlier depths adjacent to depth d
 #pragma unroll
         for (int d = 0; d < NQ - 2; ++d) D[d][lane] = pw[4 + d];  // words past 2 + nq are unused
-        R[0][lane] = pw[2];
-        X[0][lane] = pw[3];
+        cur = pw[2];
+        xcur = pw[3];
+        X[0][lane] = xcur;
+        saved = 0;
         depth = 0;
         used = 0;
         cnt = 0;
@@ -778,41 +783,61 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
     }
     bool finished = false;
     if (busy) {
-      // pop exhausted frames, then take the next candidate of the current frame
-      uint64_t Rd = R[depth][lane];
-      while (Rd == 0 && depth > 0) {
-        --depth;
-        used &= ~(uint64_t(1) << F[depth][lane]);
-        Rd = R[depth][lane];
+      if (cur == 0) {
+        // backtrack to the deepest depth with candidates left, or end the pair
+        if (saved == 0) {
+          finished = true;
+        } else {
+          const int dp = 31 - __builtin_clz(saved);
+          saved &= ~(1u << dp);
+          cur = RS[dp][lane];
+          xcur = X[dp][lane];
+          for (int i = dp; i < depth; ++i) used &= ~(uint64_t(1) << ((fpack >> (6 * i)) & 63));
+          depth = dp;
+        }
       }
-      if (Rd == 0) {
-        finished = true;
-      } else {
-        const int v = ctz64(Rd);
-        R[depth][lane] = Rd & (Rd - 1);
+      if (!finished) {
+        const int v = ctz64(cur);
+        cur &= cur - 1;
+        // one round of LDS reads: the candidate's neighbour row, and what a descent needs
         const uint64_t nv = ADJ[slot][v];
+        uint64_t nf[NQ - 3];
+#pragma unroll
+        for (int i = 0; i < NQ - 3; ++i) nf[i] = NF[i][lane];
+        const uint64_t dn = D[min(depth, NQ - 3)][lane];
+        // keep the descent's reads in this round instead of a second one inside the branch
+        asm volatile("" ::"v"(nv), "v"(dn));
+#pragma unroll
+        for (int i = 0; i < NQ - 3; ++i) asm volatile("" ::"v"(nf[i]));
         ++nodes;
         const int nd = depth + 1;
         const uint64_t vbit = uint64_t(1) << v;
         const uint32_t am = uint32_t(qa >> (8 * nd)) & 0xffu;
-        const uint64_t Rn = X[depth][lane] & ~vbit & (((am >> depth) & 1) ? nv : ~nv);
+        const uint64_t Rn = xcur & ~vbit & (((am >> depth) & 1) ? nv : ~nv);
         if (uint32_t(nd) == nq - 1) {
           cnt += uint64_t(popc64(Rn));
           if (p.find_first && cnt) finished = true;
-        } else {
-          // descend: record the match, and the candidate base of depth nd
-          F[depth][lane] = uint8_t(v);
-          NF[depth][lane] = nv;
+        } else if (Rn) {
+          // descend: the candidate base of depth nd from the matches at depths <= depth
           const uint32_t an = uint32_t(qa >> (8 * (nd + 1))) & 0xffu;
           uint64_t c = ((an >> depth) & 1) ? nv : ~nv;
-          for (int i = 0; i < depth; ++i) {
-            const uint64_t x = NF[i][lane];
-            c &= ((an >> i) & 1) ? x : ~x;
+#pragma unroll
+          for (int i = 0; i < NQ - 3; ++i) {
+            const uint64_t t = ((an >> i) & 1) ? nf[i] : ~nf[i];
+            c &= i < depth ? t : ~uint64_t(0);
           }
           used |= vbit;
-          X[nd][lane] = D[nd - 1][lane] & c & ~used;
+          const uint64_t xn = dn & c & ~used;
+          if (cur) {
+            RS[depth][lane] = cur;
+            saved |= 1u << depth;
+          }
+          if (depth < NQ - 3) NF[depth][lane] = nv;
+          fpack = (fpack & ~(uint64_t(63) << (6 * depth))) | (uint64_t(v) << (6 * depth));
+          if (nd < NQ - 2) X[nd][lane] = xn;  // backtracking returns to depths <= NQ - 3 only
           depth = nd;
-          R[nd][lane] = Rn;
+          cur = Rn;
+          xcur = xn;
         }
       }
     }
