@@ -35,7 +35,6 @@ constexpr int kPreDwords = 20;   // prefilter record of one query
 constexpr int kSumDwords = 16;   // prefilter summary of one data graph
 constexpr int kSmallNq = 8;      // queries of up to 8 nodes use the small records and join
 constexpr int kSmallWords = 2 + kSmallNq;
-constexpr int kRingSlotShift = 28;  // small-join queue entry: record index | slot << 28
 
 // Screening record layout (u32 words): [0] lower end of the node's key range (label << 8 |
 // degree, or the degree alone for the wildcard), [1..8] path Bloom, [9..10] ring counts, [11]
@@ -650,18 +649,19 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
   // backtracking jumps straight to the deepest saved depth; the node matched at depth d is
   // 6-bit field d of `fpack`, its neighbour row NF[d] (read by deeper descents only).
   constexpr int NQ = kSmallNq;
-  constexpr uint32_t kRing = 256;
+  constexpr uint32_t kRing = 160;
   constexpr uint32_t kBatch = 16;  // idle lanes that trigger a handout
   constexpr uint32_t kSegClaim = 8;  // segments claimed per atomic on the global cursor
   // a refill pushes a segment only while fewer than 128 entries are queued
   static_assert(128 + kSegRecs <= int(kRing), "ring must hold what a refill may queue");
-  static_assert(S <= (1 << (32 - kRingSlotShift)), "slot must fit the ring entry");
+  static_assert(S <= 16 && kSegRecs <= 32, "slot and offset must fit the ring entry");
   __shared__ uint64_t RS[NQ - 2][64];  // saved remaining candidates per depth
   __shared__ uint64_t X[NQ - 2][64];   // candidate base per depth (above)
   __shared__ uint64_t D[NQ - 2][64];   // domain of depth d + 2 (matching order)
   __shared__ uint64_t NF[NQ - 3][64];  // neighbour row of the node matched per depth
   __shared__ uint64_t ADJ[S][64];      // adjacency rows of each slot's pack
-  __shared__ uint32_t RING[kRing];     // queued record index | slot << kRingSlotShift
+  __shared__ uint16_t RING[kRing];     // queued record: offset in its slot's segment | slot << 5
+  __shared__ uint32_t FIRST[S];        // first record of each slot's segment
   __shared__ uint32_t USE[S];
   const uint32_t lane = lane_id();
   if (p.ctl[kCtlOverflow]) return;  // the host redoes the chunk
@@ -749,8 +749,11 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
             } else {
               ADJ[s][lane] = st_adj;
               for (uint32_t e = lane; e < st_count; e += 64)
-                RING[(qt + e) % kRing] = (st_first + e) | (uint32_t(s) << kRingSlotShift);
-              if (lane == 0) USE[s] = st_count;
+                RING[(qt + e) % kRing] = uint16_t(e | (uint32_t(s) << 5));
+              if (lane == 0) {
+                USE[s] = st_count;
+                FIRST[s] = st_first;
+              }
               qt += st_count;
               loaded = true;
               st_ok = false;
@@ -770,8 +773,8 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
         // issue the record's loads; nothing waits for them before the next handout
         pf = true;
         const uint32_t ent = RING[(qh + rank) % kRing];
-        pslot = ent >> kRingSlotShift;
-        const uint64_t* r = p.recs + uint64_t(ent & ((1u << kRingSlotShift) - 1)) * kSmallWords;
+        pslot = ent >> 5;
+        const uint64_t* r = p.recs + uint64_t(FIRST[pslot] + (ent & 31u)) * kSmallWords;
 #pragma unroll
         for (int x = 0; x < kSmallWords; ++x) pw[x] = r[x];
       }
@@ -1056,7 +1059,7 @@ struct bms2_pipeline_s {
   double dens_s = 0.0, dens_seg = 0.0, dens_l = 0.0;
   // launch shape (autotuned): waves per CU of each join, LDS slots per small-join wave
   int cus = 256;
-  uint32_t small_waves = 7, large_waves = 12, slots = 16;
+  uint32_t small_waves = 8, large_waves = 12, slots = 16;
   uint32_t* d_packs = nullptr;      // pack p: graphs d_pack_gids[d_packs[p] .. d_packs[p + 1])
   uint32_t* d_pack_gids = nullptr;
   std::vector<uint32_t> win_pack, win_graph;  // pack windows (see build_data_packs)
@@ -1415,8 +1418,8 @@ int bind_queries(bms2_pipeline_s* p, const bms2_device_csr* query) {
   const double share_l = q_small == Q ? 0.0 : (q_small == 0 ? 0.9 : 0.15);
   const uint64_t D = p->data.num_graphs;
   const uint64_t worst_s = D * q_small, worst_l = D * (Q - q_small);
-  // record indices travel in kRingSlotShift bits through the small join's queue
-  p->cap_s = std::max<uint64_t>(1, std::min<uint64_t>({worst_s, uint64_t(0.95 * (1 - share_l) * budget) / (8 * kSmallWords), (1ull << kRingSlotShift) - 1}));
+  // segment descriptors hold 32-bit record indices
+  p->cap_s = std::max<uint64_t>(1, std::min<uint64_t>({worst_s, uint64_t(0.95 * (1 - share_l) * budget) / (8 * kSmallWords), (1ull << 32) - 1}));
   p->cap_seg = std::max<uint64_t>(1, std::min<uint64_t>({p->cap_s, uint64_t(0.05 * (1 - share_l) * budget) / 16,
                                                           (uint64_t(1) << (64 - kSegShift)) - 1}));
   p->cap_l = std::max<uint64_t>(1, std::min<uint64_t>(worst_l, uint64_t(share_l * budget) / (8ull * p->rec_words_l)));
@@ -1494,8 +1497,8 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
   p->cus = cus;
-  // defaults: as many waves as LDS allows (small join with 16 slots ~22.5 KB per wave: 7 per CU)
-  p->small_waves = opt->block_waves ? opt->block_waves : 7;
+  // defaults: as many waves as LDS allows (small join with 16 slots ~20 KB per wave: 8 per CU)
+  p->small_waves = opt->block_waves ? opt->block_waves : 8;
   p->slots = 16;
   if ((rc = bind_queries(p, query))) return fail(rc);
   *out = p;
@@ -1756,9 +1759,10 @@ extern "C" int bms2_pipeline_autotune(bms2_pipeline p, uint64_t* d_counts, uint6
   };
   std::vector<Cand> cands;
   const uint32_t lw0 = p->nq_cap <= 16 ? 12 : 6;
-  for (uint32_t sw : {6u, 7u, 8u})
+  // LDS fits 8 small-join waves per CU with 16 slots, 10 with 8
+  for (uint32_t sw : {7u, 8u, 10u})
     for (uint32_t sl : {8u, 16u}) cands.push_back({sw, lw0, sl});
-  for (uint32_t lw : {lw0 / 2, lw0 * 3 / 4}) cands.push_back({7, lw, 16});
+  for (uint32_t lw : {lw0 / 2, lw0 * 3 / 4}) cands.push_back({8, lw, 16});
   const Cand keep{p->small_waves, p->large_waves, p->slots};
   float best = 0.f;
   Cand best_c = keep;
