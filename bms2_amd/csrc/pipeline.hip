@@ -823,12 +823,19 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
         } else if (Rn) {
           // descend: the candidate base of depth nd from the matches at depths <= depth
           const uint32_t an = uint32_t(qa >> (8 * (nd + 1))) & 0xffu;
-          uint64_t c = ((an >> depth) & 1) ? nv : ~nv;
+          const uint64_t c0 = ((an >> depth) & 1) ? nv : ~nv;
+          // term i = (NF[i] & on_i) ^ f_i: on_i all ones for the depths on the path (i < depth),
+          // f_i all ones where the depths are not adjacent or i is off the path (term = ones)
+          const uint32_t on = (1u << depth) - 1, flip = ~(an & on);
+          uint32_t clo = uint32_t(c0), chi = uint32_t(c0 >> 32);
 #pragma unroll
           for (int i = 0; i < NQ - 3; ++i) {
-            const uint64_t t = ((an >> i) & 1) ? nf[i] : ~nf[i];
-            c &= i < depth ? t : ~uint64_t(0);
+            const uint32_t m = uint32_t(__builtin_amdgcn_sbfe(int(on), i, 1));
+            const uint32_t f = uint32_t(__builtin_amdgcn_sbfe(int(flip), i, 1));
+            clo &= (uint32_t(nf[i]) & m) ^ f;
+            chi &= (uint32_t(nf[i] >> 32) & m) ^ f;
           }
+          const uint64_t c = clo | (uint64_t(chi) << 32);
           used |= vbit;
           const uint64_t xn = dn & c & ~used;
           if (cur) {
