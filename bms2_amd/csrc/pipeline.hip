@@ -700,8 +700,8 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
     }
   };
   bool exhausted = false, busy = false, pf = false, has_res = false;
-  uint32_t slot = 0, pslot = 0, q = 0, g = 0, nq = 0;
-  int depth = 0;
+  uint32_t slot = 0, pslot = 0, q = 0, g = 0, nadj1 = 0;
+  int depth = 0, last = 0;
   uint64_t used = 0, cnt = 0, nodes = 0, qa = 0, cur = 0, xcur = 0, fpack = 0;
   uint32_t saved = 0;
   uint64_t pw[kSmallWords];  // the lane's reserved record, loaded one handout ahead
@@ -721,8 +721,12 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
         slot = pslot;
         q = uint32_t(pw[0]);
         g = uint32_t(pw[0] >> 32);
-        nq = uint32_t(pw[1] & 0xffu);
+        last = int(pw[1] & 0xffu) - 2;  // the depth whose candidates end a match
         qa = pw[1];  // byte d >= 1: earlier depths adjacent to depth d
+        // bit d: depths d and d + 1 not adjacent (bit d of byte d + 1 clear)
+        nadj1 = ~0u;
+#pragma unroll
+        for (int d = 0; d < NQ - 1; ++d) nadj1 ^= uint32_t((qa >> (9 * d + 8)) & 1) << d;
 #pragma unroll
         for (int d = 0; d < NQ - 2; ++d) D[d][lane] = pw[4 + d];  // words past 2 + nq are unused
         cur = pw[2];
@@ -801,7 +805,8 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
       }
       if (!finished) {
         const int v = ctz64(cur);
-        cur &= cur - 1;
+        const uint64_t vbit = uint64_t(1) << v;
+        cur ^= vbit;
         // one round of LDS reads: the candidate's neighbour row, and what a descent needs
         const uint64_t nv = ADJ[slot][v];
         uint64_t nf[NQ - 3];
@@ -814,16 +819,17 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
         for (int i = 0; i < NQ - 3; ++i) asm volatile("" ::"v"(nf[i]));
         ++nodes;
         const int nd = depth + 1;
-        const uint64_t vbit = uint64_t(1) << v;
-        const uint32_t am = uint32_t(qa >> (8 * nd)) & 0xffu;
-        const uint64_t Rn = xcur & ~vbit & (((am >> depth) & 1) ? nv : ~nv);
-        if (uint32_t(nd) == nq - 1) {
+        // adjacent ? N(v) : ~N(v) as N(v) ^ f with f all ones or zero
+        const uint32_t f1 = uint32_t(__builtin_amdgcn_sbfe(int(nadj1), depth, 1));
+        const uint64_t Rn = xcur & (nv ^ ((uint64_t(f1) << 32) | f1)) & ~vbit;
+        if (depth == last) {
           cnt += uint64_t(popc64(Rn));
           if (p.find_first && cnt) finished = true;
         } else if (Rn) {
           // descend: the candidate base of depth nd from the matches at depths <= depth
           const uint32_t an = uint32_t(qa >> (8 * (nd + 1))) & 0xffu;
-          const uint64_t c0 = ((an >> depth) & 1) ? nv : ~nv;
+          const uint32_t f0 = uint32_t(__builtin_amdgcn_sbfe(int(~an), depth, 1));
+          const uint64_t c0 = nv ^ ((uint64_t(f0) << 32) | f0);
           // term i = (NF[i] & on_i) ^ f_i: on_i all ones for the depths on the path (i < depth),
           // f_i all ones where the depths are not adjacent or i is off the path (term = ones)
           const uint32_t on = (1u << depth) - 1, flip = ~(an & on);
