@@ -650,7 +650,7 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
   // 6-bit field d of `fpack`, its neighbour row NF[d] (read by deeper descents only).
   constexpr int NQ = kSmallNq;
   constexpr uint32_t kRing = 160;
-  constexpr uint32_t kBatch = 16;  // idle lanes that trigger a handout
+  constexpr uint32_t kBatch = 24;  // idle lanes that trigger a handout (16: +0.6%, 32: +4%)
   constexpr uint32_t kSegClaim = 8;  // segments claimed per atomic on the global cursor
   // a refill pushes a segment only while fewer than 128 entries are queued
   static_assert(128 + kSegRecs <= int(kRing), "ring must hold what a refill may queue");
