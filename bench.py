@@ -212,6 +212,98 @@ def ingest_bench(args) -> int:
     return 0 if same else 1
 
 
+def staged_bench(args) -> int:
+    """The stage sequence of BMS2.cpp:131-234 through the staged entry points (the reference's API
+    boundary, one C-ABI call per stage) on a shard of the workload, count-all, identity order.
+    Per-stage device times from each call's events (utils::BatchedEvent); the filter and the
+    refinement passes against HBM bandwidth (the filter writes the Qn x Dn candidate bit matrix, a
+    refinement step reads and rewrites it); per-pair counts compared with the fused pipeline's on
+    the same shard."""
+    from bms2_amd import api, graphs
+
+    D0 = args.data_graphs or 100_000
+    data, queries = graphs.generate(args.workload, seed=0, num_data_graphs=D0)
+    K = args.refinement_steps
+    q = api.Queue(0)
+    dq = api.DeviceBatchedCSRGraph(q, queries)
+    dd = api.DeviceBatchedCSRGraph(q, data)
+    names = ("signatures", "filter", "refine signatures", "refine candidates", "gmcr", "join")
+    per = {n: [] for n in names}
+    for it in range(args.warmup + args.steps):
+        # fresh stage objects every run: a Signature accumulates the reachability of its
+        # refinement levels (signature.hpp:340-385), so a second sequence on the same objects
+        # would refine nothing, and the per-pair counts are sized by this run's GMCR
+        sig = api.Signature(q, dd.total_nodes, dq.total_nodes)
+        path = api.PathSignature(q, dd.total_nodes, dq.total_nodes)
+        cyc = api.CycleSignature(q, dd.total_nodes, dq.total_nodes)
+        cand = api.Candidates(q, dq.total_nodes, dd.total_nodes)
+        gm = api.GMCR(q)
+        total = api.DeviceBuffer(q, 8, zero=True)
+        q.wait()
+        t = dict.fromkeys(names, 0.0)
+        evs = [("signatures", sig.generate_data_signatures(dd, event=True)),
+               ("signatures", path.generate_data_path_signatures(dd, event=True)),
+               ("signatures", cyc.generate_data_cycle_signatures(dd, event=True)),
+               ("signatures", sig.generate_query_signatures(dq, event=True)),
+               ("signatures", path.generate_query_path_signatures(dq, event=True)),
+               ("signatures", cyc.generate_query_cycle_signatures(dq, event=True)),
+               ("filter", api.filter_candidates(q, dq, dd, sig, path, cyc, cand, event=True))]
+        for k in range(1, K + 1):
+            evs.append(("refine signatures", sig.refine_data_signatures(dd, k, event=True)))
+            evs.append(("refine signatures", sig.refine_query_signatures(dq, k, event=True)))
+            evs.append(("refine candidates", api.refine_candidates(q, dq, dd, sig, cand, event=True)))
+        evs.append(("gmcr", gm.generate_gmcr(dq, dd, cand, event=True)))
+        per_pair = api.DeviceBuffer(q, 8 * max(gm.total, 1), zero=True)
+        evs.append(("join", api.join_candidates(q, dq, dd, cand, gm, total, False, per_pair,
+                                                event=True)))
+        q.wait()
+        for n, e in evs:
+            t[n] += e.elapsed_ms()
+        if it >= args.warmup:
+            for n in names:
+                per[n].append(t[n])
+        if it + 1 < args.warmup + args.steps:
+            del evs, sig, path, cyc, cand, gm, total, per_pair
+    ms = {n: float(np.mean(v)) for n, v in per.items()}
+    run_ms = sum(ms.values())
+    # per-pair counts against the fused pipeline on the same shard
+    off, idx = gm.host()
+    counts = per_pair.to_numpy(np.uint64, gm.total)
+    staged = np.zeros((data.num_graphs, queries.num_graphs), np.uint64)
+    rows = np.repeat(np.arange(data.num_graphs), np.diff(off.astype(np.int64)))
+    staged[rows, idx] = counts
+    fused, _, _ = api.run_pipeline(q, queries, data, K)
+    same = bool(np.array_equal(staged, fused))
+    matrix = dq.total_nodes * cand.W * 8  # candidate bit matrix bytes
+    filt_gbs = matrix / (ms["filter"] / 1e3) / 1e9
+    ref_gbs = 2 * K * matrix / (ms["refine candidates"] / 1e3) / 1e9 if K else 0.0
+    pairs = data.num_graphs * queries.num_graphs
+    print(f"[staged] {data.num_graphs} graphs x {queries.num_graphs} queries, K={K}: "
+          + ", ".join(f"{n} {v:.2f} ms" for n, v in ms.items())
+          + f"; candidate matrix {matrix / 1e9:.2f} GB, filter {filt_gbs:.0f} GB/s, refinement "
+          f"{ref_gbs:.0f} GB/s; counts equal to the fused pipeline: {same}",
+          file=sys.stderr, flush=True)
+    out = {
+        "metric": "(query,graph) pairs/s, exact counts, staged entry points (one call per stage), 1 GPU",
+        "value": round(pairs / (run_ms / 1e3), 1), "unit": "pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(run_ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+        "data": "synthetic (project generator, seed 0)",
+        "config": {"workload": f"{args.workload}: {data.num_graphs} molecules x "
+                               f"{queries.num_graphs} queries, K={K}, count-all, identity order, "
+                               "staged calls"},
+        "stage_ms": {n: round(v, 3) for n, v in ms.items()},
+        "candidate_matrix_bytes": matrix,
+        "roofline": {"bound": "hbm", "kernel": "refine_candidates (K steps)",
+                     "achieved": round(ref_gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(ref_gbs / 8000.0, 4), "traffic": None,
+                     "filter_achieved": round(filt_gbs, 1)},
+        "gmcr_pairs": gm.total, "counts_equal_fused": same,
+    }
+    print(json.dumps(out))
+    return 0 if same else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,6 +325,8 @@ def main() -> int:
                     help="data graphs traced to plan the learned order")
     ap.add_argument("--ingest", action="store_true",
                     help="time the data batch upload instead (host vs GPU packing)")
+    ap.add_argument("--staged", action="store_true",
+                    help="time the staged entry points stage by stage instead (100,000 graphs)")
     ap.add_argument("--cs", action="store_true",
                     help="time the CS (cluster) join instead (BMS2 --use-cs), see cs_bench")
     args = ap.parse_args()
@@ -243,6 +337,8 @@ def main() -> int:
               "this build runs and reports one GPU's share", file=sys.stderr)
         return 2
 
+    if args.staged:
+        return staged_bench(args)
     if args.cs:
         return cs_bench(args)
     if args.ingest:

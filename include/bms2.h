@@ -174,7 +174,10 @@ int bms2_signature_destroy(bms2_signature s);
 /* generateDataSignatures / generateQuerySignatures (signature.hpp:56-84, 124-151). */
 int bms2_signature_generate(bms2_signature s, const bms2_device_csr* g, int scope, bms2_event* ev);
 /* refineDataSignatures / refineQuerySignatures (signature.hpp:61-73, 336-389): histogram of the
- * nodes within view_size hops, minus the node and every node an earlier call already counted. */
+ * nodes within view_size hops, minus the node and every node an earlier call already counted.
+ * As in the reference, those counted sets persist for the object's lifetime (generate does not
+ * clear them): each run of the stage sequence needs a new Signature, or its refinements come out
+ * empty, refine nothing and leave a larger GMCR than the previous run's. */
 int bms2_signature_refine(bms2_signature s, const bms2_device_csr* g, uint32_t view_size, int scope,
                           bms2_event* ev);
 int bms2_signature_copy_to_host(bms2_signature s, int scope, uint64_t* out);
