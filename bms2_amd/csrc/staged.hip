@@ -462,6 +462,139 @@ __global__ __launch_bounds__(256) void join_kernel(
   if (lane_id() == 0 && c) atomicAdd(reinterpret_cast<unsigned long long*>(total), (unsigned long long)c);
 }
 
+// Adjacency rows of a batch whose graphs have at most 64 nodes: bit y of row x = local node y is a
+// neighbour of local node x (one thread per node).
+__global__ void adj_rows_kernel(const uint32_t* __restrict__ go, uint32_t num_graphs,
+                                const uint32_t* __restrict__ ro, const uint32_t* __restrict__ ci,
+                                uint64_t n, uint64_t* __restrict__ rows) {
+  const uint64_t u = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (u >= n) return;
+  const uint32_t s = go[graph_of(go, num_graphs, uint32_t(u))];
+  uint64_t r = 0;
+  for (uint32_t k = ro[u]; k < ro[u + 1]; ++k) r |= uint64_t(1) << (ci[k] - s);
+  rows[u] = r;
+}
+
+// The staged join for data graphs of up to 64 nodes (JoinCandidates3Optimized,
+// isomorphism.hpp:309-468, same results as join_kernel<1>): persistent waves take GMCR entries
+// 256 at a time from a global cursor and hand them to idle lanes; a lane's search keeps its
+// per-depth domains (from the candidate rows), remaining candidates and matched nodes' neighbour
+// rows in LDS and reads one precomputed adjacency row from global memory per step.
+template <int NQ>
+__global__ __launch_bounds__(64) void join_rows_kernel(
+    const uint64_t* __restrict__ cand, uint64_t W, const uint32_t* __restrict__ qgo,
+    const uint32_t* __restrict__ qro, const uint32_t* __restrict__ qci,
+    const uint32_t* __restrict__ dgo, uint32_t D, const uint64_t* __restrict__ adj,
+    const uint32_t* __restrict__ goff, const uint32_t* __restrict__ gidx, uint64_t T,
+    int find_first, uint64_t* __restrict__ total, uint64_t* __restrict__ per_pair,
+    const uint8_t* __restrict__ jorder, uint64_t* __restrict__ per_pair_nodes,
+    unsigned long long* __restrict__ cursor) {
+  __shared__ uint64_t DOM[NQ][64];  // domain per depth (local ids of the data graph)
+  __shared__ uint64_t R[NQ][64];    // remaining candidates per depth
+  __shared__ uint64_t NF[NQ][64];   // neighbour row of the node matched per depth
+  __shared__ uint32_t QA[NQ][64];   // bit i of QA[d]: depths i and d adjacent in the query
+  __shared__ uint8_t F[NQ][64];     // node matched per depth
+  const uint32_t lane = lane_id();
+  bool busy = false, exhausted = false;
+  uint64_t t = 0, used = 0, c = 0, nodes = 0, sum = 0;
+  uint32_t s = 0, nq = 0;
+  int depth = 0;
+  uint64_t next = 0, end = 0;
+  constexpr uint64_t kClaim = 256;
+  while (true) {
+    const uint64_t idle = ballot(!busy);
+    if (idle) {
+      if (next >= end && !exhausted) {
+        uint64_t base = 0;
+        if (lane == 0) base = atomicAdd(cursor, (unsigned long long)kClaim);
+        base = uniform64(readlane64(base, 0));
+        next = base;
+        end = base >= T ? base : min(base + kClaim, T);
+        if (base + kClaim >= T) exhausted = true;
+      }
+      const uint64_t take = min(end - next, uint64_t(popc64(idle)));
+      const uint64_t rank = uint64_t(popc64(idle & low_mask(lane)));
+      if (!busy && rank < take) {
+        busy = true;
+        t = next + rank;
+        // data graph of entry t (utils.hpp:110-121)
+        uint32_t lo = 0, hi = D + 1;
+        while (lo < hi) {
+          const uint32_t mid = lo + (hi - lo) / 2;
+          if (t >= goff[mid]) lo = mid + 1; else hi = mid;
+        }
+        const uint32_t g = lo - 1, q = gidx[t];
+        s = dgo[g];
+        const uint32_t e = dgo[g + 1], qb = qgo[q];
+        nq = qgo[q + 1] - qb;
+        uint32_t pos[NQ];
+        for (uint32_t i = 0; i < nq; ++i) pos[jorder ? jorder[size_t(q) * 32 + i] : i] = i;
+        for (uint32_t i = 0; i < nq; ++i) {
+          const uint32_t u = jorder ? jorder[size_t(q) * 32 + i] : i;
+          DOM[i][lane] = row_bits(cand + uint64_t(qb + u) * W, s, e, 0);
+          uint32_t a = 0;
+          for (uint32_t k = qro[qb + u]; k < qro[qb + u + 1]; ++k) a |= 1u << pos[qci[k] - qb];
+          QA[i][lane] = a;
+        }
+        R[0][lane] = DOM[0][lane];
+        depth = 0;
+        used = 0;
+        c = 0;
+        nodes = 0;
+      }
+      next += take;
+    }
+    if (!ballot(busy)) {
+      if (exhausted && next >= end) break;
+      continue;
+    }
+    bool finished = false;
+    if (busy) {
+      const uint64_t Rd = R[depth][lane];
+      if (Rd == 0) {
+        if (depth == 0) {
+          finished = true;
+        } else {
+          --depth;
+          used &= ~(uint64_t(1) << F[depth][lane]);
+        }
+      } else {
+        const int v = ctz64(Rd);
+        R[depth][lane] = Rd & (Rd - 1);
+        F[depth][lane] = uint8_t(v);
+        const uint64_t nv = adj[s + v];
+        NF[depth][lane] = nv;
+        ++nodes;
+        const int nd = depth + 1;
+        const uint64_t vbit = uint64_t(1) << v;
+        const uint32_t am = QA[nd][lane];
+        uint64_t Rn = DOM[nd][lane] & ~(used | vbit) & (((am >> depth) & 1) ? nv : ~nv);
+        for (int i = 0; i < depth; ++i) {
+          const uint64_t x = NF[i][lane];
+          Rn &= ((am >> i) & 1) ? x : ~x;
+        }
+        if (uint32_t(nd) == nq - 1) {
+          c += uint64_t(popc64(Rn));
+          if (find_first && c) finished = true;
+        } else {
+          used |= vbit;
+          depth = nd;
+          R[nd][lane] = Rn;
+        }
+      }
+    }
+    if (finished) {
+      const uint64_t cc = find_first ? (c ? 1 : 0) : c;
+      if (per_pair) per_pair[t] = cc;
+      if (per_pair_nodes) per_pair_nodes[t] = nodes;
+      sum += cc;
+      busy = false;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+  if (lane == 0 && sum) atomicAdd(reinterpret_cast<unsigned long long*>(total), (unsigned long long)sum);
+}
+
 // ------------------------------------------------------------------ GPU-side CSR packing
 
 // Entry count of each graph: the local row end of its last node (0 for an empty graph).
@@ -836,20 +969,59 @@ int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_cs
                 bms2_candidates cand, bms2_gmcr gmcr, int find_first, const uint8_t* d_jorder,
                 uint64_t* d_total, uint64_t* d_pair_counts, uint64_t* d_pair_nodes) {
   const uint64_t T = gmcr->total;
-  if (T) {
-    auto args = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(grid_for(T, 256, 1u << 30)), dim3(256), 0, q->stream, cand->bits,
-                         cand->W, query->graph_offsets, query->row_offsets, query->column_indices,
-                         data->graph_offsets, data->num_graphs, data->row_offsets,
-                         data->column_indices, gmcr->offsets, gmcr->indices, T, find_first,
-                         d_total, d_pair_counts, d_jorder, d_pair_nodes);
-    };
-    if (data->max_graph_nodes <= 64)
-      args(join_kernel<1>);
-    else
-      args(join_kernel<4>);
-    BMS2_CHECK_LAUNCH("join_kernel");
+  if (!T) return BMS2_OK;
+  if (data->max_graph_nodes <= 64) {
+    // adjacency rows and the entry cursor, freed once the join has run
+    uint64_t* rows = nullptr;
+    unsigned long long* cursor = nullptr;
+    if (int rc = device_alloc((void**)&rows, 8 * std::max<uint64_t>(1, data->total_nodes))) return rc;
+    if (int rc = device_alloc((void**)&cursor, 8)) {
+      (void)hipFree(rows);
+      return rc;
+    }
+    hipStream_t st = q->stream;
+    hipError_t e = hipMemsetAsync(cursor, 0, 8, st);
+    if (e == hipSuccess && data->total_nodes)
+      hipLaunchKernelGGL(adj_rows_kernel, dim3(grid_for(data->total_nodes, 256, 1u << 30)), dim3(256),
+                         0, st, data->graph_offsets, data->num_graphs, data->row_offsets,
+                         data->column_indices, data->total_nodes, rows);
+    if (e == hipSuccess) e = hipGetLastError();
+    int dev = 0, cus = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+    if (e == hipSuccess) {
+      // waves per CU as LDS allows: 29 KB per wave for queries of up to 16 nodes, 58 KB for 32
+      const bool q16 = query->max_graph_nodes <= 16;
+      const dim3 grid(uint32_t(cus) * (q16 ? 5u : 2u));
+      if (q16)
+        hipLaunchKernelGGL(join_rows_kernel<16>, grid, dim3(64), 0, st, cand->bits, cand->W,
+                           query->graph_offsets, query->row_offsets, query->column_indices,
+                           data->graph_offsets, data->num_graphs, rows, gmcr->offsets,
+                           gmcr->indices, T, find_first, d_total, d_pair_counts, d_jorder,
+                           d_pair_nodes, cursor);
+      else
+        hipLaunchKernelGGL(join_rows_kernel<BMS2_MAX_QUERY_NODES>, grid, dim3(64), 0, st, cand->bits,
+                           cand->W, query->graph_offsets, query->row_offsets, query->column_indices,
+                           data->graph_offsets, data->num_graphs, rows, gmcr->offsets,
+                           gmcr->indices, T, find_first, d_total, d_pair_counts, d_jorder,
+                           d_pair_nodes, cursor);
+      e = hipGetLastError();
+    }
+    // the scratch buffers outlive the kernels: free them after the stream has drained
+    const hipError_t se = hipStreamSynchronize(st);
+    (void)hipFree(rows);
+    (void)hipFree(cursor);
+    if (e == hipSuccess) e = se;
+    if (e != hipSuccess) return hip_fail(e, "join_rows_kernel", __FILE__, __LINE__);
+    return BMS2_OK;
   }
+  hipLaunchKernelGGL(join_kernel<4>, dim3(grid_for(T, 256, 1u << 30)), dim3(256), 0, q->stream,
+                     cand->bits, cand->W, query->graph_offsets, query->row_offsets,
+                     query->column_indices, data->graph_offsets, data->num_graphs, data->row_offsets,
+                     data->column_indices, gmcr->offsets, gmcr->indices, T, find_first, d_total,
+                     d_pair_counts, d_jorder, d_pair_nodes);
+  BMS2_CHECK_LAUNCH("join_kernel");
   return BMS2_OK;
 }
 
