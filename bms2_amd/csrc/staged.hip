@@ -306,42 +306,77 @@ __global__ __launch_bounds__(256) void gmcr_count_kernel(const uint64_t* __restr
   if (lane_id() == 0) cnt[g] = total;
 }
 
-// Exclusive scan of u32 counts into u32 offsets [n + 1] (replaces the serial single_task of
-// gmcr.hpp:92-94): one 1024-thread block, chunked.
+// Exclusive scans of u32 counts into u32 offsets [n + 1] (replace the serial single_task of
+// gmcr.hpp:92-94).  A 1024-thread block scans a chunk of kScanChunk counts at a time.
+constexpr uint32_t kScanPer = 8, kScanChunk = 1024 * kScanPer;
+
+// Counts [base, base + kScanChunk) into offsets from `start`; returns the chunk's sum.
+__device__ inline uint32_t scan_chunk(const uint32_t* __restrict__ cnt, uint64_t n, uint64_t base,
+                                      uint32_t start, uint32_t* __restrict__ offsets,
+                                      uint32_t* part) {
+  uint32_t local[kScanPer];
+  uint32_t sum = 0;
+  for (uint32_t k = 0; k < kScanPer; ++k) {
+    const uint64_t i = base + uint64_t(threadIdx.x) * kScanPer + k;
+    local[k] = i < n ? cnt[i] : 0;
+    sum += local[k];
+  }
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = start + part[threadIdx.x] - sum;
+  for (uint32_t k = 0; k < kScanPer; ++k) {
+    const uint64_t i = base + uint64_t(threadIdx.x) * kScanPer + k;
+    if (i < n) offsets[i] = run;
+    run += local[k];
+  }
+  const uint32_t total = part[1023];
+  __syncthreads();
+  return total;
+}
+
+// One block over all chunks in turn.
 __global__ __launch_bounds__(1024) void scan_kernel(const uint32_t* __restrict__ cnt, uint64_t n,
                                                     uint32_t* __restrict__ offsets) {
   __shared__ uint32_t part[1024];
-  __shared__ uint32_t carry;
-  if (threadIdx.x == 0) carry = 0;
+  uint32_t carry = 0;
+  for (uint64_t base = 0; base < n; base += kScanChunk)
+    carry += scan_chunk(cnt, n, base, carry, offsets, part);
+  if (threadIdx.x == 0) offsets[n] = carry;
+}
+
+// Multi-block scan, step 1: block b sums chunk b.
+__global__ __launch_bounds__(1024) void scan_sums_kernel(const uint32_t* __restrict__ cnt, uint64_t n,
+                                                         uint32_t* __restrict__ sums) {
+  __shared__ uint32_t part[1024];
+  const uint64_t base = uint64_t(blockIdx.x) * kScanChunk;
+  uint32_t sum = 0;
+  for (uint32_t k = 0; k < kScanPer; ++k) {
+    const uint64_t i = base + uint64_t(threadIdx.x) * kScanPer + k;
+    if (i < n) sum += cnt[i];
+  }
+  part[threadIdx.x] = sum;
   __syncthreads();
-  const uint32_t per = 8;
-  for (uint64_t base = 0; base < n; base += 1024 * per) {
-    uint32_t local[8];
-    uint32_t sum = 0;
-    for (uint32_t k = 0; k < per; ++k) {
-      const uint64_t i = base + uint64_t(threadIdx.x) * per + k;
-      local[k] = i < n ? cnt[i] : 0;
-      sum += local[k];
-    }
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-      const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
-      __syncthreads();
-      part[threadIdx.x] += v;
-      __syncthreads();
-    }
-    uint32_t run = carry + part[threadIdx.x] - sum;
-    for (uint32_t k = 0; k < per; ++k) {
-      const uint64_t i = base + uint64_t(threadIdx.x) * per + k;
-      if (i < n) offsets[i] = run;
-      run += local[k];
-    }
-    __syncthreads();
-    if (threadIdx.x == 1023) carry += part[1023];
+  for (uint32_t off = 512; off > 0; off >>= 1) {
+    if (threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0) offsets[n] = carry;
+  if (threadIdx.x == 0) sums[blockIdx.x] = part[0];
+}
+
+// Step 3 (after the chunk sums are scanned into chunk_off [chunks + 1]): block b scans chunk b
+// from its offset; block 0 writes the total.
+__global__ __launch_bounds__(1024) void scan_add_kernel(const uint32_t* __restrict__ cnt, uint64_t n,
+                                                        const uint32_t* __restrict__ chunk_off,
+                                                        uint64_t chunks, uint32_t* __restrict__ offsets) {
+  __shared__ uint32_t part[1024];
+  (void)scan_chunk(cnt, n, uint64_t(blockIdx.x) * kScanChunk, chunk_off[blockIdx.x], offsets, part);
+  if (blockIdx.x == 0 && threadIdx.x == 0) offsets[n] = chunk_off[chunks];
 }
 
 // GMCR k3 (gmcr.hpp:111-126) made deterministic: ascending query ids per data graph.
@@ -947,6 +982,25 @@ __global__ void fill_zero_kernel(uint64_t* __restrict__ p, uint64_t n) {
     p[i] = 0;
 }
 
+// Exclusive scan of n u32 counts into offsets [n + 1] on stream st: one block for a few chunks,
+// otherwise chunk sums, their scan and a rescan of every chunk from its offset (the chunk sums
+// in stream-ordered memory; the single block when that allocation fails).  Errors surface
+// through hipGetLastError.
+void scan_counts(hipStream_t st, const uint32_t* cnt, uint64_t n, uint32_t* offsets) {
+  const uint64_t chunks = (n + kScanChunk - 1) / kScanChunk;
+  uint32_t* tmp = nullptr;
+  if (chunks <= 4 || hipMallocAsync((void**)&tmp, 4 * (2 * chunks + 1), st) != hipSuccess) {
+    (void)hipGetLastError();  // a failed allocation is not an error here
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, cnt, n, offsets);
+    return;
+  }
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(uint32_t(chunks)), dim3(1024), 0, st, cnt, n, tmp);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, tmp, chunks, tmp + chunks);
+  hipLaunchKernelGGL(scan_add_kernel, dim3(uint32_t(chunks)), dim3(1024), 0, st, cnt, n,
+                     tmp + chunks, chunks, offsets);
+  (void)hipFreeAsync(tmp, st);
+}
+
 int staged_reset(bms2_signature sig, bms2_candidates cand) {
   hipStream_t st = sig->q->stream;
   for (int sc = 0; sc < 2; ++sc) {
@@ -1385,7 +1439,7 @@ int bms2_gmcr_generate(bms2_gmcr gm, const bms2_device_csr* query, const bms2_de
                          vbits, VW, cnt);
     else
       e = hipMemsetAsync(cnt, 0, 4ull * D, s);
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, cnt, uint64_t(D), gm->offsets);
+    scan_counts(s, cnt, uint64_t(D), gm->offsets);
   } else {
     e = hipMemsetAsync(gm->offsets, 0, 4, s);
   }
@@ -1531,11 +1585,10 @@ int bms2_device_csr_pack(bms2_queue q, uint32_t num_graphs, const uint32_t* node
                      uint64_t(2));
   uint32_t E = 0;
   if (num_graphs) {
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, t_nodes, uint64_t(num_graphs),
-                       d.graph_offsets);
+    scan_counts(st, t_nodes, uint64_t(num_graphs), d.graph_offsets);
     hipLaunchKernelGGL(pack_entries_kernel, dim3((num_graphs + 255) / 256), dim3(256), 0, st,
                        t_nodes, d.graph_offsets, num_graphs, t_ro, t_ent);
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, t_ent, uint64_t(num_graphs), t_eoff);
+    scan_counts(st, t_ent, uint64_t(num_graphs), t_eoff);
     PACK_HIP(hipMemcpyAsync(&E, t_eoff + num_graphs, 4, hipMemcpyDeviceToHost, st));
   } else {
     PACK_HIP(hipMemsetAsync(d.graph_offsets, 0, 4, st));
@@ -1649,7 +1702,7 @@ int bms2_join_partial_candidates(bms2_queue q, const bms2_device_csr* query,
                        (unsigned long long)total, (unsigned long long)capacity);
     }
     if (rc == BMS2_OK && total) {
-      hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, cnt, uint64_t(D), off);
+      scan_counts(st, cnt, uint64_t(D), off);
       if (small) launch(cs_partial_kernel<1>, 1); else launch(cs_partial_kernel<4>, 1);
       if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess)
         rc = set_error(BMS2_ERR_HIP, "partial-match fill failed");

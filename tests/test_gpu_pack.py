@@ -23,11 +23,14 @@ def _same(a: GraphBatch, b: GraphBatch):
         assert np.array_equal(getattr(a, f), getattr(b, f)), f
 
 
-@pytest.mark.parametrize("which", ["molecules", "queries", "uneven", "empty-graphs"])
+@pytest.mark.parametrize("which", ["molecules", "queries", "uneven", "empty-graphs", "many"])
 def test_pack_equals_host_packing(queue, which):
     if which in ("molecules", "queries"):
         d, q = graphs.generate("molecules-2k", seed=4)
         host = d if which == "molecules" else q
+    elif which == "many":
+        # more graphs than one scan block covers (the offsets go through the multi-block scan)
+        host, _ = graphs.generate("molecules-1m", seed=3, num_data_graphs=40_000)
     elif which == "uneven":
         host, _ = graphs.generate("uneven-1m", seed=2, num_data_graphs=400, num_query_graphs=4)
         assert int(np.diff(host.graph_offsets).max()) > 64
