@@ -1,4 +1,5 @@
 // Queues, events, device memory and device graph batches (C ABI).
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -49,6 +50,26 @@ int device_alloc(void** p, uint64_t bytes) {
   return BMS2_OK;
 }
 
+int queue_scratch(bms2_queue q, uint64_t bytes, void** p) {
+  *p = nullptr;
+  if (bytes > q->scratch_bytes) {
+    if (q->scratch) BMS2_HIP(hipFreeAsync(q->scratch, q->stream));
+    q->scratch = nullptr;
+    q->scratch_bytes = 0;
+    const uint64_t want = std::max<uint64_t>(bytes + bytes / 8, 1u << 20);
+    hipError_t e = hipMallocAsync(&q->scratch, want, q->stream);
+    if (e != hipSuccess) {
+      q->scratch = nullptr;
+      (void)hipGetLastError();
+      return set_error(BMS2_ERR_NOMEM, "hipMallocAsync of %llu bytes failed: %s",
+                       (unsigned long long)want, hipGetErrorString(e));
+    }
+    q->scratch_bytes = want;
+  }
+  *p = q->scratch;
+  return BMS2_OK;
+}
+
 }  // namespace bms2
 
 using namespace bms2;
@@ -72,6 +93,11 @@ int bms2_queue_wrap(int device, void* hip_stream, bms2_queue* out) {
 
 int bms2_queue_destroy(bms2_queue q) {
   if (!q) return BMS2_OK;
+  if (q->scratch) {
+    BMS2_HIP(hipSetDevice(q->device));
+    BMS2_HIP(hipFreeAsync(q->scratch, q->stream));
+    q->scratch = nullptr;
+  }
   if (q->owned) {
     BMS2_HIP(hipSetDevice(q->device));
     BMS2_HIP(hipStreamSynchronize(q->stream));

@@ -9,6 +9,9 @@ struct bms2_queue_s {
   int device;
   hipStream_t stream;
   bool owned;
+  // grow-only scratch ordered on `stream` (join adjacency rows), released with the queue
+  void* scratch = nullptr;
+  uint64_t scratch_bytes = 0;
 };
 
 struct bms2_event_s {
@@ -45,6 +48,10 @@ struct EventScope {
 };
 
 int device_alloc(void** p, uint64_t bytes);
+
+// Points *p at q's scratch of at least `bytes`, reallocated in stream order when it grows.  The
+// contents are undefined; work on q's stream owns it until the next call.
+int queue_scratch(bms2_queue q, uint64_t bytes, void** p);
 
 // Staged join (staged.hip) with an optional matching order (d_jorder [Q][32], null = identity)
 // and optional per-pair counts and DFS nodes; enqueued on q, no event.

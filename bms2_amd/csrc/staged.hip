@@ -1025,16 +1025,14 @@ int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_cs
   const uint64_t T = gmcr->total;
   if (!T) return BMS2_OK;
   if (data->max_graph_nodes <= 64) {
-    // adjacency rows and the entry cursor, freed once the join has run
-    uint64_t* rows = nullptr;
-    unsigned long long* cursor = nullptr;
-    if (int rc = device_alloc((void**)&rows, 8 * std::max<uint64_t>(1, data->total_nodes))) return rc;
-    if (int rc = device_alloc((void**)&cursor, 8)) {
-      (void)hipFree(rows);
-      return rc;
-    }
+    // adjacency rows and the entry cursor in the queue's scratch
     hipStream_t st = q->stream;
-    hipError_t e = hipMemsetAsync(cursor, 0, 8, st);
+    const uint64_t nrows = std::max<uint64_t>(1, data->total_nodes);
+    uint64_t* rows = nullptr;
+    if (int rc = queue_scratch(q, 8 * (nrows + 1), (void**)&rows)) return rc;
+    auto* cursor = reinterpret_cast<unsigned long long*>(rows + nrows);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(64), 0, st, rows + nrows, uint64_t(1));
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess && data->total_nodes)
       hipLaunchKernelGGL(adj_rows_kernel, dim3(grid_for(data->total_nodes, 256, 1u << 30)), dim3(256),
                          0, st, data->graph_offsets, data->num_graphs, data->row_offsets,
@@ -1062,11 +1060,6 @@ int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_cs
                            d_pair_nodes, cursor);
       e = hipGetLastError();
     }
-    // the scratch buffers outlive the kernels: free them after the stream has drained
-    const hipError_t se = hipStreamSynchronize(st);
-    (void)hipFree(rows);
-    (void)hipFree(cursor);
-    if (e == hipSuccess) e = se;
     if (e != hipSuccess) return hip_fail(e, "join_rows_kernel", __FILE__, __LINE__);
     return BMS2_OK;
   }
