@@ -72,6 +72,19 @@ __device__ inline uint64_t readlane64(uint64_t x, int lane) {
   return (uint64_t(hi) << 32) | lo;
 }
 
+// x with lane `lane` replaced by the uniform value v.
+__device__ inline uint64_t writelane64(uint64_t v, int lane, uint64_t x) {
+  return int(lane_id()) == lane ? v : x;
+}
+
+// Block number in an order that keeps consecutive numbers on one XCD: the hardware deals blocks
+// to the 8 XCDs round-robin, so with a grid of a multiple of 8 blocks, block b runs on XCD b % 8
+// and takes number (b % 8) * (grid / 8) + b / 8.
+__device__ inline uint32_t xcd_block() {
+  const uint32_t n = gridDim.x, b = blockIdx.x;
+  return (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3);
+}
+
 __device__ inline int ctz64(uint64_t x) { return __builtin_ctzll(x); }
 __device__ inline int popc64(uint64_t x) { return __popcll(x); }
 

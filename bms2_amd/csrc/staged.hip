@@ -186,67 +186,144 @@ __global__ void cycle_signature_kernel(const uint32_t* __restrict__ go, uint32_t
 
 // ------------------------------------------------------------------ filter / refine
 
-// FilterCandidatesKernel_v2 (isomorphism.hpp:59-97): one wave per 64 data nodes (= one u64 word
-// of every row), query nodes streamed through scalar loads, the row word from one ballot.
+constexpr uint64_t kCycleTop = 0x8000800080008000ull;  // top bit of each 16-bit ring count
+
+// FilterCandidatesKernel_v2 (isomorphism.hpp:59-97): query node u keeps data node v when the labels
+// match (or u's is the wildcard), deg(u) <= deg(v), v's path signature contains u's and v's ring
+// counts are at least u's; bits are ORed into the matrix (written only where they add bits).
+// One wave per 64 data nodes (one u64 word of every row): query nodes stream through scalar loads
+// and each one's row word comes from one ballot; lane j collects the words of rows qb + j for a
+// band of 64 query nodes, whose old words were loaded at the band's start.
 __global__ __launch_bounds__(256) void filter_kernel(
-    const uint32_t* __restrict__ qro, const uint8_t* __restrict__ qlab, uint64_t Qn,
-    const uint64_t* __restrict__ qpath, const uint64_t* __restrict__ qcyc,
+    const uint32_t* __restrict__ qro, const uint32_t* __restrict__ lab32, uint32_t lab_skew,
+    uint64_t Qn, const uint64_t* __restrict__ qpath, const uint64_t* __restrict__ qcyc,
     const uint32_t* __restrict__ dro, const uint8_t* __restrict__ dlab, uint64_t Dn,
     const uint64_t* __restrict__ dpath, const uint64_t* __restrict__ dcyc,
     uint64_t* __restrict__ cand, uint64_t W) {
-  const uint64_t word = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / 64;
-  if (word >= W) return;
-  const uint64_t d = word * 64 + lane_id();
-  const bool active = d < Dn;
-  uint32_t dl = 0x100, ddeg = 0;
-  uint64_t dp[4] = {0, 0, 0, 0}, dc = 0;
-  if (active) {
-    dl = dlab[d];
-    ddeg = dro[d + 1] - dro[d];
-    for (int w = 0; w < 4; ++w) dp[w] = dpath[4 * d + w];
-    dc = dcyc[d];
-  }
-  for (uint64_t qn = 0; qn < Qn; ++qn) {
-    const uint32_t ql = qlab[qn];
-    const uint32_t qdeg = qro[qn + 1] - qro[qn];
-    bool ok = active && (dl == ql || ql == kWildcard) && qdeg <= ddeg;
-    if (ballot(ok)) {
-      for (int w = 0; w < 4; ++w) {
-        const uint64_t qp = qpath[4 * qn + w];
-        ok = ok && (dp[w] & qp) == qp;
+  // query labels come as the aligned dwords that hold them (lab32 = labels - lab_skew): scalar
+  // loads have no byte form
+  const uint32_t lane = lane_id();
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x / 64;
+  for (uint64_t word = (xcd_block() * uint64_t(blockDim.x) + threadIdx.x) / 64; word < W;
+       word += stride) {
+    const uint64_t d = word * 64 + lane;
+    const bool active = d < Dn;
+    uint32_t dl = 0x100, ddeg = 0;
+    uint64_t nd[4] = {~0ull, ~0ull, ~0ull, ~0ull}, dch = 0;
+    if (active) {
+      dl = dlab[d];
+      ddeg = dro[d + 1] - dro[d];
+      for (int w = 0; w < 4; ++w) nd[w] = ~dpath[4 * d + w];
+      // ring counts are at most 0x7fff, so (dc | top bits) - qc borrows across no field and
+      // leaves a field's top bit set exactly when the data count is at least the query count
+      dch = dcyc[d] | kCycleTop;
+    }
+    // the node's values have arrived: without this wait the first use of one inside the band
+    // loop waits for the band's matrix words too
+    __builtin_amdgcn_s_waitcnt(0);
+    for (uint64_t qb = 0; qb < Qn; qb += 64) {
+      const uint64_t r = qb + lane;
+      const bool has = r < Qn;
+      const uint64_t old = has ? cand[r * W + word] : 0;
+      const uint32_t n = uint32_t(std::min<uint64_t>(64, Qn - qb));
+      uint64_t acc = 0;
+#pragma unroll 4
+      for (uint32_t j = 0; j < n; ++j) {
+        const uint64_t qn = qb + j;
+        const uint64_t li = qn + lab_skew;
+        const uint32_t ql = (lab32[li >> 2] >> (8 * (li & 3))) & 0xff;
+        const uint32_t qdeg = qro[qn + 1] - qro[qn];
+        const uint64_t miss = (qpath[4 * qn] & nd[0]) | (qpath[4 * qn + 1] & nd[1]) |
+                              (qpath[4 * qn + 2] & nd[2]) | (qpath[4 * qn + 3] & nd[3]);
+        const bool ok = active & ((dl == ql) | (ql == kWildcard)) & (qdeg <= ddeg) &
+                        (miss == 0) & (((dch - qcyc[qn]) & kCycleTop) == kCycleTop);
+        acc = writelane64(ballot(ok), int(j), acc);
       }
-      const uint64_t qc = qcyc[qn];
-      for (int f = 0; f < 4; ++f) ok = ok && ((dc >> (16 * f)) & 0xffff) >= ((qc >> (16 * f)) & 0xffff);
-      const uint64_t m = ballot(ok);
-      if (m && lane_id() == 0) cand[qn * W + word] |= m;
+      if (has && (acc & ~old)) cand[r * W + word] = old | acc;
     }
   }
 }
 
-__device__ inline bool dominates(uint64_t d, uint64_t q) {
-  for (int l = 0; l < 16; ++l)
-    if (((q >> (4 * l)) & 15) > ((d >> (4 * l)) & 15)) return false;
-  return true;
+// Bit planes of the data signatures for refinement: plane j of word column w (planes[j W + w]) has
+// bit b set when bit j of data node 64 w + b's signature is set (bits 4 l .. 4 l + 3 are its count
+// of label class l).  One wave per column, one ballot per plane.
+__global__ __launch_bounds__(256) void sig_planes_kernel(const uint64_t* __restrict__ dsig,
+                                                         uint64_t Dn, uint64_t W,
+                                                         uint64_t* __restrict__ planes) {
+  const uint32_t lane = lane_id();
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x / 64;
+  for (uint64_t w = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / 64; w < W; w += stride) {
+    const uint64_t d = w * 64 + lane;
+    const uint64_t ds = d < Dn ? dsig[d] : 0;
+    uint64_t mine = 0;  // lane j keeps plane j
+    for (int j = 0; j < 64; ++j) mine = writelane64(ballot((ds >> j) & 1), j, mine);
+    planes[lane * W + w] = mine;
+  }
 }
 
-// RefineCandidatesKernel_v3 (isomorphism.hpp:142-196).
-__global__ void refine_candidates_kernel(const uint64_t* __restrict__ qsig,
-                                         const uint64_t* __restrict__ dsig, uint64_t Dn,
-                                         uint64_t* __restrict__ cand, uint64_t W, uint64_t total) {
-  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < total;
-       i += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t block = cand[i];
-    if (!block) continue;
-    const uint64_t qn = i / W, w = i % W;
-    const uint64_t qs = qsig[qn];
-    uint64_t nb = block, bits = block;
-    while (bits) {
-      const int b = ctz64(bits);
-      bits &= bits - 1;
-      const uint64_t d = w * 64 + b;
-      if (d >= Dn || !dominates(dsig[d], qs)) nb &= ~(uint64_t(1) << b);
+// RefineCandidatesKernel_v3 (isomorphism.hpp:142-196): data node v stays a candidate of query node
+// u when each of the 16 4-bit counts of v's signature is at least u's; nodes past the batch end are
+// cleared.  One wave per 64 word columns and a run of rows: lane c holds column c's 64 signature
+// planes, so a row's 64 words (one coalesced 512-byte load) are refined by a bit-sliced
+// comparison, 64 data nodes per instruction: for each count of the row's query signature that is
+// not zero, the borrow of (data count - query count) runs through the count's four planes, and a
+// node whose count borrows is dropped.  The next rows' words load while a row is refined.
+constexpr uint32_t kRefineRun = 128;  // rows per wave
+constexpr int kRefineAhead = 4;       // rows in flight
+
+__global__ __launch_bounds__(64) void refine_candidates_kernel(const uint64_t* __restrict__ qsig,
+                                                               const uint64_t* __restrict__ planes,
+                                                               uint64_t Dn,
+                                                               uint64_t* __restrict__ cand,
+                                                               uint64_t W, uint64_t rows) {
+  const uint64_t groups = (W + 63) / 64, runs = (rows + kRefineRun - 1) / kRefineRun;
+  const uint64_t col0 = uint64_t(blockIdx.x % groups) * 64;
+  const uint64_t rbeg = uint64_t(blockIdx.x / groups) * kRefineRun;
+  if (rbeg >= runs * kRefineRun) return;
+  const uint64_t rend = std::min<uint64_t>(rows, rbeg + kRefineRun);
+  const uint64_t col = col0 + lane_id();
+  const bool has = col < W;
+  uint64_t P[64];
+#pragma unroll
+  for (int j = 0; j < 64; ++j) P[j] = has ? planes[j * W + col] : 0;
+  const uint64_t valid = has ? low_mask(uint32_t(std::min<uint64_t>(64, Dn - col * 64))) : 0;
+  uint64_t ahead[kRefineAhead];
+#pragma unroll
+  for (int k = 0; k < kRefineAhead; ++k) {
+    const uint64_t r = rbeg + k;
+    ahead[k] = (has && r < rend) ? cand[r * W + col] : 0;
+  }
+  for (uint64_t r0 = rbeg; r0 < rend; r0 += kRefineAhead) {
+    uint64_t cur[kRefineAhead];
+#pragma unroll
+    for (int k = 0; k < kRefineAhead; ++k) {
+      cur[k] = ahead[k];
+      const uint64_t r = r0 + kRefineAhead + k;
+      ahead[k] = (has && r < rend) ? cand[r * W + col] : 0;
     }
-    if (nb != block) cand[i] = nb;
+#pragma unroll
+    for (int k = 0; k < kRefineAhead; ++k) {
+      const uint64_t r = r0 + k;
+      const uint64_t blk = cur[k];
+      if (r >= rend || !ballot(blk != 0)) continue;
+      const uint64_t qs = qsig[r];
+      uint64_t drop = ~valid;
+#pragma unroll
+      for (int l = 0; l < 16; ++l) {
+        const uint32_t c = uint32_t(qs >> (4 * l)) & 15;
+        if (!c) continue;
+        uint64_t borrow = 0;
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) {
+          // query bit 1: borrow' = ~x | borrow; query bit 0: borrow' = ~x & borrow
+          const uint64_t m = ((c >> k2) & 1) ? ~0ull : 0ull;
+          borrow = (~P[4 * l + k2] & (borrow | m)) | (borrow & m);
+        }
+        drop |= borrow;
+      }
+      const uint64_t nb = blk & ~drop;
+      if (nb != blk) cand[r * W + col] = nb;
+    }
   }
 }
 
@@ -917,6 +994,13 @@ inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 1u << 20) {
   return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(g, cap)));
 }
 
+// Blocks of 256 threads for n threads of a wave-strided kernel: a multiple of 8 (whole rounds of
+// the XCDs, for xcd_block) and at most 2^20.
+inline uint32_t grid8(uint64_t n) {
+  const uint64_t b = (n + 255) / 256;
+  return uint32_t(std::min<uint64_t>((b + 7) & ~uint64_t(7), 1u << 20));
+}
+
 }  // namespace bms2
 
 using namespace bms2;
@@ -1351,11 +1435,14 @@ int bms2_filter_candidates(bms2_queue q, const bms2_device_csr* query, const bms
   (void)sig;  // the label histogram is not part of the initial filter (isomorphism.hpp:71)
   EventScope es{q, ev};
   if (int rc = es.begin()) return rc;
+  const uint32_t lab_skew = uint32_t(reinterpret_cast<uintptr_t>(query->node_labels) & 3);
   if (cand->W && query->total_nodes)
-    hipLaunchKernelGGL(filter_kernel, dim3(grid_for(cand->W * 64, 256, 1u << 30)), dim3(256), 0,
-                       q->stream, query->row_offsets, query->node_labels, query->total_nodes,
-                       path->bits[1], cycle->sig[1], data->row_offsets, data->node_labels,
-                       data->total_nodes, path->bits[0], cycle->sig[0], cand->bits, cand->W);
+    hipLaunchKernelGGL(filter_kernel, dim3(grid8(cand->W * 64)), dim3(256), 0, q->stream,
+                       query->row_offsets,
+                       reinterpret_cast<const uint32_t*>(query->node_labels - lab_skew), lab_skew,
+                       query->total_nodes, path->bits[1], cycle->sig[1], data->row_offsets,
+                       data->node_labels, data->total_nodes, path->bits[0], cycle->sig[0],
+                       cand->bits, cand->W);
   BMS2_CHECK_LAUNCH("filter_kernel");
   return es.end();
 }
@@ -1369,10 +1456,20 @@ int bms2_refine_candidates(bms2_queue q, const bms2_device_csr* query, const bms
     return set_error(BMS2_ERR_INVALID, "objects sized for another problem");
   EventScope es{q, ev};
   if (int rc = es.begin()) return rc;
-  const uint64_t total = cand->rows * cand->W;
-  if (total)
-    hipLaunchKernelGGL(refine_candidates_kernel, dim3(grid_for(total, 256)), dim3(256), 0, q->stream,
-                       sig->sig[1], sig->sig[0], data->total_nodes, cand->bits, cand->W, total);
+  if (cand->W && cand->rows) {
+    // the signature planes live in the queue's scratch until the refinement has run
+    uint64_t* planes = nullptr;
+    if (int rc = queue_scratch(q, 64 * 8 * cand->W, (void**)&planes)) return rc;
+    hipLaunchKernelGGL(sig_planes_kernel, dim3(grid_for(cand->W * 64, 256)), dim3(256), 0,
+                       q->stream, sig->sig[0], data->total_nodes, cand->W, planes);
+    BMS2_CHECK_LAUNCH("sig_planes_kernel");
+    const uint64_t blocks = (cand->W + 63) / 64 * ((cand->rows + kRefineRun - 1) / kRefineRun);
+    if (blocks * 64 >= (1ull << 32))
+      return set_error(BMS2_ERR_UNSUPPORTED, "candidate matrix of %llu x %llu words is too large",
+                       (unsigned long long)cand->rows, (unsigned long long)cand->W);
+    hipLaunchKernelGGL(refine_candidates_kernel, dim3(uint32_t(blocks)), dim3(64), 0, q->stream,
+                       sig->sig[1], planes, data->total_nodes, cand->bits, cand->W, cand->rows);
+  }
   BMS2_CHECK_LAUNCH("refine_candidates_kernel");
   return es.end();
 }
