@@ -188,31 +188,57 @@ __global__ void cycle_signature_kernel(const uint32_t* __restrict__ go, uint32_t
 
 constexpr uint64_t kCycleTop = 0x8000800080008000ull;  // top bit of each 16-bit ring count
 
+// A query node's filter operands in one 64-byte record (one scalar load per node in the filter).
+struct alignas(64) FilterQuery {
+  uint32_t label;       // node label
+  uint32_t label_mask;  // 0xff, or 0 for the wildcard (any label matches)
+  uint32_t degree;
+  uint32_t pad0;
+  uint64_t path[4];     // path signature bits the data node must contain
+  uint64_t cycle;       // ring counts, 16 bits each, at most 0x7fff
+  uint64_t pad1;
+};
+
+__global__ void filter_queries_kernel(const uint32_t* __restrict__ qro,
+                                      const uint8_t* __restrict__ qlab, uint64_t Qn,
+                                      const uint64_t* __restrict__ qpath,
+                                      const uint64_t* __restrict__ qcyc,
+                                      FilterQuery* __restrict__ out) {
+  for (uint64_t u = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; u < Qn;
+       u += uint64_t(gridDim.x) * blockDim.x) {
+    FilterQuery f{};
+    f.label = qlab[u];
+    f.label_mask = f.label == kWildcard ? 0u : 0xffu;
+    f.degree = qro[u + 1] - qro[u];
+    for (int k = 0; k < 4; ++k) f.path[k] = qpath[4 * u + k];
+    f.cycle = qcyc[u];
+    out[u] = f;
+  }
+}
+
 // FilterCandidatesKernel_v2 (isomorphism.hpp:59-97): query node u keeps data node v when the labels
 // match (or u's is the wildcard), deg(u) <= deg(v), v's path signature contains u's and v's ring
 // counts are at least u's; bits are ORed into the matrix (written only where they add bits).
-// One wave per 64 data nodes (one u64 word of every row): query nodes stream through scalar loads
-// and each one's row word comes from one ballot; lane j collects the words of rows qb + j for a
-// band of 64 query nodes, whose old words were loaded at the band's start.
+// One wave per 64 data nodes (one u64 word of every row): each query node's record arrives by one
+// scalar load and its row word comes from one ballot over branch-free vector tests; lane j
+// collects the words of rows qb + j for a band of 64 query nodes, whose old words were loaded at
+// the band's start.
 __global__ __launch_bounds__(256) void filter_kernel(
-    const uint32_t* __restrict__ qro, const uint32_t* __restrict__ lab32, uint32_t lab_skew,
-    uint64_t Qn, const uint64_t* __restrict__ qpath, const uint64_t* __restrict__ qcyc,
-    const uint32_t* __restrict__ dro, const uint8_t* __restrict__ dlab, uint64_t Dn,
-    const uint64_t* __restrict__ dpath, const uint64_t* __restrict__ dcyc,
-    uint64_t* __restrict__ cand, uint64_t W) {
-  // query labels come as the aligned dwords that hold them (lab32 = labels - lab_skew): scalar
-  // loads have no byte form
+    const FilterQuery* __restrict__ fq, uint64_t Qn, const uint32_t* __restrict__ dro,
+    const uint8_t* __restrict__ dlab, uint64_t Dn, const uint64_t* __restrict__ dpath,
+    const uint64_t* __restrict__ dcyc, uint64_t* __restrict__ cand, uint64_t W) {
   const uint32_t lane = lane_id();
   const uint64_t stride = uint64_t(gridDim.x) * blockDim.x / 64;
   for (uint64_t word = (xcd_block() * uint64_t(blockDim.x) + threadIdx.x) / 64; word < W;
        word += stride) {
     const uint64_t d = word * 64 + lane;
     const bool active = d < Dn;
-    uint32_t dl = 0x100, ddeg = 0;
-    uint64_t nd[4] = {~0ull, ~0ull, ~0ull, ~0ull}, dch = 0;
+    uint32_t dl = 0;
+    int32_t dd = -1;  // a missing node fails the degree test
+    uint64_t nd[4] = {0, 0, 0, 0}, dch = 0;
     if (active) {
       dl = dlab[d];
-      ddeg = dro[d + 1] - dro[d];
+      dd = int32_t(dro[d + 1] - dro[d]);
       for (int w = 0; w < 4; ++w) nd[w] = ~dpath[4 * d + w];
       // ring counts are at most 0x7fff, so (dc | top bits) - qc borrows across no field and
       // leaves a field's top bit set exactly when the data count is at least the query count
@@ -221,23 +247,28 @@ __global__ __launch_bounds__(256) void filter_kernel(
     // the node's values have arrived: without this wait the first use of one inside the band
     // loop waits for the band's matrix words too
     __builtin_amdgcn_s_waitcnt(0);
+    // row word of data node lane for query node u: one ballot of branch-free tests (a failing
+    // degree test sets the top bit of the difference, folded into the miss bits)
+    auto test = [&](const FilterQuery& f) {
+      const uint64_t miss = (f.path[0] & nd[0]) | (f.path[1] & nd[1]) | (f.path[2] & nd[2]) |
+                            (f.path[3] & nd[3]) | (~(dch - f.cycle) & kCycleTop) |
+                            ((dl ^ f.label) & f.label_mask) |
+                            (uint32_t(dd - int32_t(f.degree)) & 0x80000000u);
+      return ballot(miss == 0);
+    };
     for (uint64_t qb = 0; qb < Qn; qb += 64) {
       const uint64_t r = qb + lane;
       const bool has = r < Qn;
-      const uint64_t old = has ? cand[r * W + word] : 0;
-      const uint32_t n = uint32_t(std::min<uint64_t>(64, Qn - qb));
+      // every lane loads (rows past the end read the last row and store nothing)
+      const uint64_t old = cand[std::min<uint64_t>(r, Qn - 1) * W + word];
+      const FilterQuery* band = fq + qb;
       uint64_t acc = 0;
-#pragma unroll 4
-      for (uint32_t j = 0; j < n; ++j) {
-        const uint64_t qn = qb + j;
-        const uint64_t li = qn + lab_skew;
-        const uint32_t ql = (lab32[li >> 2] >> (8 * (li & 3))) & 0xff;
-        const uint32_t qdeg = qro[qn + 1] - qro[qn];
-        const uint64_t miss = (qpath[4 * qn] & nd[0]) | (qpath[4 * qn + 1] & nd[1]) |
-                              (qpath[4 * qn + 2] & nd[2]) | (qpath[4 * qn + 3] & nd[3]);
-        const bool ok = active & ((dl == ql) | (ql == kWildcard)) & (qdeg <= ddeg) &
-                        (miss == 0) & (((dch - qcyc[qn]) & kCycleTop) == kCycleTop);
-        acc = writelane64(ballot(ok), int(j), acc);
+      if (Qn - qb >= 64) {
+#pragma unroll 8
+        for (uint32_t j = 0; j < 64; ++j) acc = writelane64(test(band[j]), int(j), acc);
+      } else {
+        for (uint32_t j = 0; j < uint32_t(Qn - qb); ++j)
+          acc = writelane64(test(band[j]), int(j), acc);
       }
       if (has && (acc & ~old)) cand[r * W + word] = old | acc;
     }
@@ -1435,14 +1466,18 @@ int bms2_filter_candidates(bms2_queue q, const bms2_device_csr* query, const bms
   (void)sig;  // the label histogram is not part of the initial filter (isomorphism.hpp:71)
   EventScope es{q, ev};
   if (int rc = es.begin()) return rc;
-  const uint32_t lab_skew = uint32_t(reinterpret_cast<uintptr_t>(query->node_labels) & 3);
-  if (cand->W && query->total_nodes)
-    hipLaunchKernelGGL(filter_kernel, dim3(grid8(cand->W * 64)), dim3(256), 0, q->stream,
-                       query->row_offsets,
-                       reinterpret_cast<const uint32_t*>(query->node_labels - lab_skew), lab_skew,
-                       query->total_nodes, path->bits[1], cycle->sig[1], data->row_offsets,
-                       data->node_labels, data->total_nodes, path->bits[0], cycle->sig[0],
-                       cand->bits, cand->W);
+  if (cand->W && query->total_nodes) {
+    // the query records live in the queue's scratch until the filter has run
+    FilterQuery* fq = nullptr;
+    if (int rc = queue_scratch(q, sizeof(FilterQuery) * query->total_nodes, (void**)&fq)) return rc;
+    hipLaunchKernelGGL(filter_queries_kernel, dim3(grid_for(query->total_nodes, 256)), dim3(256), 0,
+                       q->stream, query->row_offsets, query->node_labels, query->total_nodes,
+                       path->bits[1], cycle->sig[1], fq);
+    BMS2_CHECK_LAUNCH("filter_queries_kernel");
+    hipLaunchKernelGGL(filter_kernel, dim3(grid8(cand->W * 64)), dim3(256), 0, q->stream, fq,
+                       query->total_nodes, data->row_offsets, data->node_labels,
+                       data->total_nodes, path->bits[0], cycle->sig[0], cand->bits, cand->W);
+  }
   BMS2_CHECK_LAUNCH("filter_kernel");
   return es.end();
 }
