@@ -644,9 +644,12 @@ __global__ __launch_bounds__(64) void join_rows_kernel(
   int depth = 0;
   uint64_t next = 0, end = 0;
   constexpr uint64_t kClaim = 256;
+  // a handout costs the dependent loads of one entry's setup whatever the number of lanes it
+  // serves, so idle lanes wait until a third of the wave (or all of it) can take entries
+  constexpr int kHandout = 24;
   while (true) {
     const uint64_t idle = ballot(!busy);
-    if (idle) {
+    if (popc64(idle) >= kHandout || idle == ~uint64_t(0)) {
       if (next >= end && !exhausted) {
         uint64_t base = 0;
         if (lane == 0) base = atomicAdd(cursor, (unsigned long long)kClaim);
