@@ -643,6 +643,7 @@ __global__ __launch_bounds__(64) void join_rows_kernel(
   uint32_t s = 0, nq = 0;
   int depth = 0;
   uint64_t next = 0, end = 0;
+  uint32_t gnext = 0;  // data graph of entry `next`
   constexpr uint64_t kClaim = 256;
   // a handout costs the dependent loads of one entry's setup whatever the number of lanes it
   // serves, so idle lanes wait until a third of the wave (or all of it) can take entries
@@ -657,19 +658,26 @@ __global__ __launch_bounds__(64) void join_rows_kernel(
         next = base;
         end = base >= T ? base : min(base + kClaim, T);
         if (base + kClaim >= T) exhausted = true;
+        if (next < end) {
+          // data graph of the claim's first entry (utils.hpp:110-121), searched once per claim
+          uint32_t lo = 0, hi = D + 1;
+          while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (next >= goff[mid]) lo = mid + 1; else hi = mid;
+          }
+          gnext = lo - 1;
+        }
       }
       const uint64_t take = min(end - next, uint64_t(popc64(idle)));
       const uint64_t rank = uint64_t(popc64(idle & low_mask(lane)));
       if (!busy && rank < take) {
         busy = true;
         t = next + rank;
-        // data graph of entry t (utils.hpp:110-121)
-        uint32_t lo = 0, hi = D + 1;
-        while (lo < hi) {
-          const uint32_t mid = lo + (hi - lo) / 2;
-          if (t >= goff[mid]) lo = mid + 1; else hi = mid;
-        }
-        const uint32_t g = lo - 1, q = gidx[t];
+        // data graph of entry t: forward from the graph of the handout's first entry (a claim
+        // spans a few graphs; t < goff[D] = T ends the walk)
+        uint32_t g = gnext;
+        while (t >= goff[g + 1]) ++g;
+        const uint32_t q = gidx[t];
         s = dgo[g];
         const uint32_t e = dgo[g + 1], qb = qgo[q];
         nq = qgo[q + 1] - qb;
@@ -689,6 +697,7 @@ __global__ __launch_bounds__(64) void join_rows_kernel(
         nodes = 0;
       }
       next += take;
+      while (next < end && next >= goff[gnext + 1]) ++gnext;
     }
     if (!ballot(busy)) {
       if (exhausted && next >= end) break;
