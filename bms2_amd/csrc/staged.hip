@@ -647,7 +647,7 @@ __global__ __launch_bounds__(64) void join_rows_kernel(
   constexpr uint64_t kClaim = 256;
   // a handout costs the dependent loads of one entry's setup whatever the number of lanes it
   // serves, so idle lanes wait until a third of the wave (or all of it) can take entries
-  constexpr int kHandout = 24;
+  constexpr int kHandout = 32;
   while (true) {
     const uint64_t idle = ballot(!busy);
     if (popc64(idle) >= kHandout || idle == ~uint64_t(0)) {
