@@ -618,6 +618,33 @@ __global__ void adj_rows_kernel(const uint32_t* __restrict__ go, uint32_t num_gr
   rows[u] = r;
 }
 
+// Query adjacency by matching depth for the staged join: bit j of qa[32 q + i] is set when the query
+// nodes matched at depths i and j of query q are adjacent (depth i matches node order[i], identity
+// without an order).  One thread per (query, depth).
+__global__ void join_depth_adjacency_kernel(const uint32_t* __restrict__ qgo,
+                                            const uint32_t* __restrict__ qro,
+                                            const uint32_t* __restrict__ qci, uint32_t Q,
+                                            const uint8_t* __restrict__ jorder,
+                                            uint32_t* __restrict__ qa) {
+  const uint64_t x = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (x >= uint64_t(Q) * 32) return;
+  const uint32_t q = uint32_t(x / 32), i = uint32_t(x % 32);
+  const uint32_t qb = qgo[q], nq = qgo[q + 1] - qb;
+  if (i >= nq) return;
+  const uint8_t* ord = jorder ? jorder + size_t(q) * 32 : nullptr;
+  const uint32_t u = ord ? ord[i] : i;
+  uint32_t a = 0;
+  for (uint32_t k = qro[qb + u]; k < qro[qb + u + 1]; ++k) {
+    const uint32_t w = qci[k] - qb;
+    uint32_t j = w;
+    if (ord)
+      for (j = 0; j < nq && ord[j] != w; ++j) {
+      }
+    a |= 1u << j;
+  }
+  qa[x] = a;
+}
+
 // The staged join for data graphs of up to 64 nodes (JoinCandidates3Optimized,
 // isomorphism.hpp:309-468, same results as join_kernel<1>): persistent waves take GMCR entries
 // 256 at a time from a global cursor and hand them to idle lanes; a lane's search keeps its
@@ -626,8 +653,8 @@ __global__ void adj_rows_kernel(const uint32_t* __restrict__ go, uint32_t num_gr
 template <int NQ>
 __global__ __launch_bounds__(64) void join_rows_kernel(
     const uint64_t* __restrict__ cand, uint64_t W, const uint32_t* __restrict__ qgo,
-    const uint32_t* __restrict__ qro, const uint32_t* __restrict__ qci,
-    const uint32_t* __restrict__ dgo, uint32_t D, const uint64_t* __restrict__ adj,
+    const uint32_t* __restrict__ qa, const uint32_t* __restrict__ dgo, uint32_t D,
+    const uint64_t* __restrict__ adj,
     const uint32_t* __restrict__ goff, const uint32_t* __restrict__ gidx, uint64_t T,
     int find_first, uint64_t* __restrict__ total, uint64_t* __restrict__ per_pair,
     const uint8_t* __restrict__ jorder, uint64_t* __restrict__ per_pair_nodes,
@@ -681,14 +708,23 @@ __global__ __launch_bounds__(64) void join_rows_kernel(
         s = dgo[g];
         const uint32_t e = dgo[g + 1], qb = qgo[q];
         nq = qgo[q + 1] - qb;
-        uint32_t pos[NQ];
-        for (uint32_t i = 0; i < nq; ++i) pos[jorder ? jorder[size_t(q) * 32 + i] : i] = i;
-        for (uint32_t i = 0; i < nq; ++i) {
-          const uint32_t u = jorder ? jorder[size_t(q) * 32 + i] : i;
-          DOM[i][lane] = row_bits(cand + uint64_t(qb + u) * W, s, e, 0);
-          uint32_t a = 0;
-          for (uint32_t k = qro[qb + u]; k < qro[qb + u + 1]; ++k) a |= 1u << pos[qci[k] - qb];
-          QA[i][lane] = a;
+        // four depths' loads in flight at a time
+        for (uint32_t i0 = 0; i0 < nq; i0 += 4) {
+          uint64_t dom[4];
+          uint32_t a[4];
+#pragma unroll
+          for (uint32_t k = 0; k < 4; ++k) {
+            const uint32_t i = i0 + k;
+            const uint32_t u = i < nq ? (jorder ? jorder[size_t(q) * 32 + i] : i) : 0;
+            dom[k] = i < nq ? row_bits(cand + uint64_t(qb + u) * W, s, e, 0) : 0;
+            a[k] = i < nq ? qa[size_t(q) * 32 + i] : 0;
+          }
+#pragma unroll
+          for (uint32_t k = 0; k < 4; ++k)
+            if (i0 + k < nq) {
+              DOM[i0 + k][lane] = dom[k];
+              QA[i0 + k][lane] = a[k];
+            }
         }
         R[0][lane] = DOM[0][lane];
         depth = 0;
@@ -1152,12 +1188,18 @@ int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_cs
   const uint64_t T = gmcr->total;
   if (!T) return BMS2_OK;
   if (data->max_graph_nodes <= 64) {
-    // adjacency rows and the entry cursor in the queue's scratch
+    // adjacency rows, the entry cursor and the queries' depth adjacency in the queue's scratch
     hipStream_t st = q->stream;
     const uint64_t nrows = std::max<uint64_t>(1, data->total_nodes);
+    const uint64_t nqa = uint64_t(query->num_graphs) * 32;
     uint64_t* rows = nullptr;
-    if (int rc = queue_scratch(q, 8 * (nrows + 1), (void**)&rows)) return rc;
+    if (int rc = queue_scratch(q, 8 * (nrows + 1) + 4 * nqa, (void**)&rows)) return rc;
     auto* cursor = reinterpret_cast<unsigned long long*>(rows + nrows);
+    auto* qa = reinterpret_cast<uint32_t*>(rows + nrows + 1);
+    if (nqa)
+      hipLaunchKernelGGL(join_depth_adjacency_kernel, dim3(grid_for(nqa, 256)), dim3(256), 0, st,
+                         query->graph_offsets, query->row_offsets, query->column_indices,
+                         query->num_graphs, d_jorder, qa);
     hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(64), 0, st, rows + nrows, uint64_t(1));
     hipError_t e = hipGetLastError();
     if (e == hipSuccess && data->total_nodes)
@@ -1175,16 +1217,14 @@ int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_cs
       const dim3 grid(uint32_t(cus) * (q16 ? 5u : 2u));
       if (q16)
         hipLaunchKernelGGL(join_rows_kernel<16>, grid, dim3(64), 0, st, cand->bits, cand->W,
-                           query->graph_offsets, query->row_offsets, query->column_indices,
-                           data->graph_offsets, data->num_graphs, rows, gmcr->offsets,
-                           gmcr->indices, T, find_first, d_total, d_pair_counts, d_jorder,
-                           d_pair_nodes, cursor);
+                           query->graph_offsets, qa, data->graph_offsets, data->num_graphs, rows,
+                           gmcr->offsets, gmcr->indices, T, find_first, d_total, d_pair_counts,
+                           d_jorder, d_pair_nodes, cursor);
       else
         hipLaunchKernelGGL(join_rows_kernel<BMS2_MAX_QUERY_NODES>, grid, dim3(64), 0, st, cand->bits,
-                           cand->W, query->graph_offsets, query->row_offsets, query->column_indices,
-                           data->graph_offsets, data->num_graphs, rows, gmcr->offsets,
-                           gmcr->indices, T, find_first, d_total, d_pair_counts, d_jorder,
-                           d_pair_nodes, cursor);
+                           cand->W, query->graph_offsets, qa, data->graph_offsets,
+                           data->num_graphs, rows, gmcr->offsets, gmcr->indices, T, find_first,
+                           d_total, d_pair_counts, d_jorder, d_pair_nodes, cursor);
       e = hipGetLastError();
     }
     if (e != hipSuccess) return hip_fail(e, "join_rows_kernel", __FILE__, __LINE__);
