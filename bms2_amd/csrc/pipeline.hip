@@ -520,9 +520,10 @@ __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
   // records claimed by this wave from the global cursor, handed to idle lanes without atomics
   uint64_t local_next = 0, local_end = 0;
   constexpr uint64_t kClaim = 256;
+  constexpr int kHandout = 16;  // idle lanes that trigger a handout (or the whole wave)
   while (true) {
     const uint64_t idle = ballot(!busy);
-    if (idle) {
+    if (popc64(idle) >= kHandout || idle == ~uint64_t(0)) {
       if (local_next >= local_end && !exhausted) {
         uint64_t base = 0;
         if (lane == 0)
