@@ -112,7 +112,8 @@ int bms2_queue_create(int device, bms2_queue* out);
 /* A queue over an existing hipStream_t (not owned). */
 int bms2_queue_wrap(int device, void* hip_stream, bms2_queue* out);
 /* Releases the queue (and its stream when owned).  A queue keeps a grow-only scratch buffer for
- * the join (8 bytes per data node of the largest batch joined on it) until it is destroyed. */
+ * the filter, refinement and join calls (about 8 bytes per data node of the largest batch they
+ * ran on) until it is destroyed. */
 int bms2_queue_destroy(bms2_queue q);
 int bms2_queue_wait(bms2_queue q); /* queue.wait_and_throw() */
 void* bms2_queue_stream(bms2_queue q);
