@@ -3,7 +3,10 @@
 Each check is chosen so that a plausible restatement mistake fails it: non-induced matching
 (fixtures, star in star), shells as balls instead of exact-distance rings (P4 in C5 at K=3),
 a dropped wildcard, a transposed hash combine (Bloom bit positions), visited-set leaks
-(order invariance, star in star), a missing degree or path test (fixtures).
+(order invariance, star in star).  The filter's four terms (label or wildcard, degree, path
+Bloom containment, ring counts) are pinned at the domain level by hand-derived iteration-0 domains
+(tests/filter_cases.py), one case per term: every term is a sound prune, so the count fixtures
+cannot see a term that is missing.
 """
 import json
 from pathlib import Path
@@ -11,6 +14,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+import filter_cases
 import oracle
 from bms2_amd.graphs import GraphBatch
 
@@ -79,6 +83,16 @@ def test_benzene_in_benzene_is_automorphism_count():
     g = batch([ring(6)])
     for K in (0, 2, 6):
         assert counts_dense(g, g, K)[0, 0] == 12
+
+
+# ---------------------------------------------------------------- filter domains
+
+@pytest.mark.parametrize("case", filter_cases.CASES, ids=[c[0] for c in filter_cases.CASES])
+def test_filter_iteration0_domains_hand_derived(case):
+    q, d = filter_cases.batches(case)
+    r = oracle.staged(q, d, 0, keep_domains=True)
+    got = filter_cases.domain_sets(r.domains[0], q.total_nodes, d.total_nodes)
+    assert got == case[3]
 
 
 # ---------------------------------------------------------------- closed forms
