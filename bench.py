@@ -44,6 +44,28 @@ def algorithmic_bytes(data, queries) -> float:
     return float(side(data) + side(queries) + 8 * data.num_graphs * queries.num_graphs)
 
 
+def host_cores() -> dict:
+    """Cores of this host: nproc, the affinity mask, OMP_NUM_THREADS, and the number the CPU
+    baseline uses (OMP_NUM_THREADS when set, which is the share of the host given to this job,
+    else every core in the affinity mask)."""
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = nproc
+    omp = os.environ.get("OMP_NUM_THREADS")
+    omp_n = int(omp) if omp and omp.isdigit() and int(omp) > 0 else None
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": nproc, "affinity": aff, "omp": omp_n, "usable": omp_n or aff, "model": model}
+
+
 def load_traffic() -> dict | None:
     p = ROOT / "profiles" / "pipeline_molecules-1m_traffic.json"
     if p.exists():
@@ -315,7 +337,9 @@ def main() -> int:
     ap.add_argument("--find-first", action="store_true")
     ap.add_argument("--cpu-sample-graphs", type=int, default=0,
                     help="data graphs the CPU baseline runs on (0: all)")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, cpu count)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0: every core this process may run on (OMP_NUM_THREADS when the host "
+                         "sets it, else the affinity mask)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="time the plain leg only")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
@@ -424,6 +448,10 @@ def main() -> int:
         "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
         "traffic": (traffic or {}).get("bytes_per_launch"),
+        "traffic_source": None if not traffic else (
+            "profiles/pipeline_molecules-1m_traffic.json: rocprofv3 FETCH_SIZE + WRITE_SIZE of one "
+            f"{traffic.get('order', 'learned')}-order run, tree {traffic.get('tree', 'unrecorded')}, "
+            f"{traffic.get('measured', 'date unrecorded')} (not this run)"),
         "kernel": f"screen_kernel<{K}> + join_kernel (device span per run, {order} order)",
         "kernel_ms_avg": round(float(np.mean(kernel_ms)), 3),
         "algorithmic_bytes_per_launch": abytes,
@@ -434,7 +462,8 @@ def main() -> int:
     parity = None
     if not args.no_cpu_baseline:
         import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        host = host_cores()
+        threads = args.cpu_threads or host["usable"]
         sub = data.subset(0, S)
         print(f"[bench] CPU baseline: oracle fused path on the first {S} graphs x {Q} queries, "
               f"{threads} threads", file=sys.stderr, flush=True)
@@ -444,7 +473,10 @@ def main() -> int:
         cpu = {"value": round(S * Q / cpu_s, 1), "unit": "pairs/s", "cores": threads,
                "kind": "port", "seconds": round(cpu_s, 2),
                "sample": (f"all {D}" if S == D else f"first {S} of {D}") +
-                         f" data graphs x all {Q} queries, same K and mode, from host CSR"}
+                         f" data graphs x all {Q} queries, same K and mode, from host CSR",
+               "nproc": host["nproc"], "affinity_cpus": host["affinity"],
+               "omp_num_threads": host["omp"], "cpu_model": host["model"],
+               "build": "g++ -O3 -march=native -fopenmp, OpenMP dynamic schedule over data graphs"}
         gc, gv = pipe.results(counts, valid, rows=S)
         parity = bool(np.array_equal(gc, rc) and np.array_equal(gv, rv))
         if plain_sample is not None:
@@ -476,6 +508,7 @@ def main() -> int:
         "autotune": tune,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "gpu_vs_cpu": None if cpu is None else round(value / cpu["value"], 1),
         "parity_on_cpu_sample": parity,
     }
     print(json.dumps(line))

@@ -149,6 +149,13 @@ PROTOTYPES: dict[str, tuple] = {
     "bms2_candidates_words_per_row": (u64, [vp]),
     "bms2_candidates_copy_to_host": (i32, [vp, vp]),
     "bms2_candidates_row_counts": (i32, [vp, vp]),
+    "bms2_candidates_copy_rows": (i32, [vp, u64, u64, vp]),
+    "bms2_signature_alloc_size": (u64, [vp, i32]),
+    "bms2_path_signature_alloc_size": (u64, [vp, i32]),
+    "bms2_cycle_signature_alloc_size": (u64, [vp, i32]),
+    "bms2_queue_device_info": (i32, [vp, C.POINTER(C.c_uint64), C.POINTER(u32), C.c_char_p, u64]),
+    "bms2_shared_malloc": (i32, [vp, u64, vpp]),
+    "bms2_shared_free": (i32, [vp, vp]),
     "bms2_filter_candidates": (i32, [vp, DCP, DCP, vp, vp, vp, vp, vpp]),
     "bms2_refine_candidates": (i32, [vp, DCP, DCP, vp, vp, vpp]),
     "bms2_gmcr_create": (i32, [vp, vpp]),
@@ -157,11 +164,11 @@ PROTOTYPES: dict[str, tuple] = {
     "bms2_gmcr_total": (u64, [vp]),
     "bms2_gmcr_device_ptrs": (i32, [vp, vpp, vpp]),
     "bms2_gmcr_copy_to_host": (i32, [vp, vp, vp]),
-    "bms2_join_candidates": (i32, [vp, DCP, DCP, vp, vp, vp, i32, vp, vpp]),
+    "bms2_join_candidates": (i32, [vp, DCP, DCP, vp, vp, vp, i32, vp, u64, vpp]),
     "bms2_join_partial_candidates": (i32, [vp, DCP, DCP, vp, u32, vp, u64, C.POINTER(C.c_uint64),
                                            vpp]),
     "bms2_join_with_partial_matches": (i32, [vp, DCP, DCP, vp, vp, u32, vp, u64, vp, i32, vp,
-                                             vpp]),
+                                             u64, vpp]),
     "bms2_pipeline_create": (i32, [vp, DCP, DCP, C.POINTER(PipelineOptions), vpp]),
     "bms2_pipeline_destroy": (i32, [vp]),
     "bms2_pipeline_run": (i32, [vp, vp, vp, vp, vpp, vpp]),

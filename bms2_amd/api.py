@@ -423,11 +423,14 @@ class GMCR(_Handle):
 
 def join_candidates(q: Queue, query, data, cand: Candidates, gmcr: GMCR, num_matches: DeviceBuffer,
                     find_first: bool = True, pair_counts: DeviceBuffer | None = None, event=False):
-    """joinCandidates: adds the match total into the device u64 `num_matches`."""
+    """joinCandidates: adds the match total into the device u64 `num_matches`; `pair_counts`
+    (optional) receives one u64 per GMCR entry and must hold gmcr.total of them (Bms2Error with
+    code ERR_CAPACITY otherwise, before any device work)."""
     e = _ev_out(event)
     check(_L().bms2_join_candidates(q.h, C.byref(query.c), C.byref(data.c), cand.h, gmcr.h,
                                     C.c_void_p(num_matches.ptr), int(find_first),
                                     C.c_void_p(pair_counts.ptr) if pair_counts else None,
+                                    pair_counts.nbytes // 8 if pair_counts else 0,
                                     C.byref(e) if e is not None else None))
     return _ev(e)
 
@@ -465,7 +468,8 @@ def join_with_partial_matches(q: Queue, query, data, cand: Candidates, gmcr: GMC
     check(_L().bms2_join_with_partial_matches(
         q.h, C.byref(query.c), C.byref(data.c), cand.h, gmcr.h, int(end), C.c_void_p(partial.ptr),
         int(num_partial), C.c_void_p(num_matches.ptr), int(find_first),
-        C.c_void_p(pair_counts.ptr), C.byref(e) if e is not None else None))
+        C.c_void_p(pair_counts.ptr), pair_counts.nbytes // 8,
+        C.byref(e) if e is not None else None))
     return _ev(e)
 
 

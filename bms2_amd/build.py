@@ -75,6 +75,17 @@ def build_driver() -> Path:
     return PKG / "bms2_main"
 
 
+def build_dropin_check() -> Path:
+    """tests/dropin/bms2_callshapes: a driver making the reference BMS2 driver's library calls in
+    the same shapes against include/sigmo.hpp (the drop-in check the tests compile and run)."""
+    src = ROOT / "tests" / "dropin" / "bms2_callshapes.cpp"
+    out = src.with_suffix("")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-pthread", "-I", str(ROOT / "include"),
+          str(src), "-o", str(out), "-L", str(PKG), "-lbms2", f"-Wl,-rpath,{PKG}",
+          "-Wl,-rpath,$ORIGIN/../../bms2_amd"])
+    return out
+
+
 def build_oracle() -> Path:
     _run(["make", "-s", "-C", str(ORACLE), "-B", "liboracle.so"])
     return ORACLE / "liboracle.so"
@@ -83,6 +94,7 @@ def build_oracle() -> Path:
 def build_all() -> None:
     build_library()
     build_driver()
+    build_dropin_check()
     build_oracle()
 
 

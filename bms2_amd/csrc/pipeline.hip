@@ -1234,7 +1234,7 @@ int run_big(bms2_pipeline_s* p, uint64_t* d_counts, uint64_t* d_valid, uint32_t 
   }
   hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_big_stats, 8u);
   if (int rc = staged_join(q, dq, db, p->big_cand, p->big_gmcr, int(p->opt.find_first),
-                           p->d_jorder, p->d_big_stats + 4, p->d_big_pc, p->d_big_pn))
+                           p->d_jorder, p->d_big_stats + 4, p->d_big_pc, p->d_big_pn, nullptr))
     return rc;
   const uint32_t Q = p->query.num_graphs, VW = (Q + 63) / 64;
   hipLaunchKernelGGL(big_zero_kernel, dim3(p->num_big), dim3(256), 0, s, p->d_big_gid, p->num_big,

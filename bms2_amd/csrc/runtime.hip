@@ -29,6 +29,13 @@ int EventScope::begin() {
   return BMS2_OK;
 }
 
+EventScope::~EventScope() {
+  if (!ev) return;
+  if (ev->start) (void)hipEventDestroy(ev->start);
+  if (ev->stop) (void)hipEventDestroy(ev->stop);
+  delete ev;
+}
+
 int EventScope::end() {
   if (!out) return BMS2_OK;
   BMS2_HIP(hipEventRecord(ev->stop, q->stream));
@@ -127,6 +134,42 @@ int bms2_event_destroy(bms2_event e) {
   (void)hipEventDestroy(e->start);
   (void)hipEventDestroy(e->stop);
   delete e;
+  return BMS2_OK;
+}
+
+int bms2_queue_device_info(bms2_queue q, uint64_t* global_mem_bytes, uint32_t* compute_units,
+                           char* name, uint64_t name_capacity) {
+  if (int rc = use_device(q)) return rc;
+  hipDeviceProp_t prop;
+  BMS2_HIP(hipGetDeviceProperties(&prop, q->device));
+  if (global_mem_bytes) *global_mem_bytes = uint64_t(prop.totalGlobalMem);
+  if (compute_units) *compute_units = uint32_t(prop.multiProcessorCount);
+  if (name && name_capacity) {
+    std::strncpy(name, prop.name, name_capacity - 1);
+    name[name_capacity - 1] = 0;
+  }
+  return BMS2_OK;
+}
+
+int bms2_shared_malloc(bms2_queue q, uint64_t bytes, void** out) {
+  if (!out) return set_error(BMS2_ERR_INVALID, "null output");
+  *out = nullptr;
+  if (int rc = use_device(q)) return rc;
+  hipError_t e = hipHostMalloc(out, bytes ? bytes : 8, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    (void)hipGetLastError();
+    return set_error(BMS2_ERR_NOMEM, "hipHostMalloc of %llu bytes failed: %s",
+                     (unsigned long long)bytes, hipGetErrorString(e));
+  }
+  return BMS2_OK;
+}
+
+int bms2_shared_free(bms2_queue q, void* ptr) {
+  if (!ptr) return BMS2_OK;
+  if (int rc = use_device(q)) return rc;
+  BMS2_HIP(hipStreamSynchronize(q->stream));
+  BMS2_HIP(hipHostFree(ptr));
   return BMS2_OK;
 }
 

@@ -9,7 +9,10 @@ struct bms2_queue_s {
   int device;
   hipStream_t stream;
   bool owned;
-  // grow-only scratch ordered on `stream` (join adjacency rows), released with the queue
+  // grow-only scratch ordered on `stream`, released with the queue: the filter's query records
+  // (64 B per query node), the refinement's signature planes (8 B per data node) and the join's
+  // adjacency rows and depth adjacency (8 B per data node + 128 B per query); it grows to
+  // max(64 query nodes, 8 data nodes + 128 queries) bytes plus an eighth, at least 1 MiB
   void* scratch = nullptr;
   uint64_t scratch_bytes = 0;
 };
@@ -38,13 +41,15 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
 // Makes the queue's device current.
 int use_device(bms2_queue q);
 
-// Brackets a stage call with an event pair when `ev` is non-null.
+// Brackets a stage call with an event pair when `ev` is non-null.  An event pair that end()
+// has not handed out (the call failed in between) is destroyed with the scope.
 struct EventScope {
   bms2_queue q;
   bms2_event* out;
   bms2_event_s* ev = nullptr;
   int begin();
   int end();
+  ~EventScope();
 };
 
 int device_alloc(void** p, uint64_t bytes);
@@ -54,10 +59,13 @@ int device_alloc(void** p, uint64_t bytes);
 int queue_scratch(bms2_queue q, uint64_t bytes, void** p);
 
 // Staged join (staged.hip) with an optional matching order (d_jorder [Q][32], null = identity)
-// and optional per-pair counts and DFS nodes; enqueued on q, no event.
+// and optional per-pair counts and DFS nodes; enqueued on q, no event.  Its scratch is
+// `scratch` (staged_join_scratch_bytes of device memory) or, when null, the queue's.
+uint64_t staged_join_scratch_bytes(const bms2_device_csr* query, const bms2_device_csr* data);
 int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
                 bms2_candidates cand, bms2_gmcr gmcr, int find_first, const uint8_t* d_jorder,
-                uint64_t* d_total, uint64_t* d_pair_counts, uint64_t* d_pair_nodes);
+                uint64_t* d_total, uint64_t* d_pair_counts, uint64_t* d_pair_nodes,
+                void* scratch);
 
 // Zeroes a Signature's label signatures and reachability state and a Candidates matrix (fill
 // kernels on the signature's queue), so the staged sequence can run again on the same objects.

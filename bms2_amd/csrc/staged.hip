@@ -673,7 +673,7 @@ __global__ __launch_bounds__(64) void join_rows_kernel(
   uint32_t gnext = 0;  // data graph of entry `next`
   constexpr uint64_t kClaim = 256;
   // a handout costs the dependent loads of one entry's setup whatever the number of lanes it
-  // serves, so idle lanes wait until a third of the wave (or all of it) can take entries
+  // serves, so idle lanes wait until half the wave (or all of it) can take entries
   constexpr int kHandout = 32;
   while (true) {
     const uint64_t idle = ballot(!busy);
@@ -1182,18 +1182,27 @@ int staged_reset(bms2_signature sig, bms2_candidates cand) {
   return BMS2_OK;
 }
 
+uint64_t staged_join_scratch_bytes(const bms2_device_csr* query, const bms2_device_csr* data) {
+  if (data->max_graph_nodes > 64) return 0;
+  // one adjacency row per data node, the entry cursor, 32 depth-adjacency words per query
+  return 8 * (std::max<uint64_t>(1, data->total_nodes) + 1) + 4 * 32 * uint64_t(query->num_graphs);
+}
+
 int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
                 bms2_candidates cand, bms2_gmcr gmcr, int find_first, const uint8_t* d_jorder,
-                uint64_t* d_total, uint64_t* d_pair_counts, uint64_t* d_pair_nodes) {
+                uint64_t* d_total, uint64_t* d_pair_counts, uint64_t* d_pair_nodes,
+                void* scratch) {
   const uint64_t T = gmcr->total;
   if (!T) return BMS2_OK;
   if (data->max_graph_nodes <= 64) {
-    // adjacency rows, the entry cursor and the queries' depth adjacency in the queue's scratch
+    // adjacency rows, the entry cursor and the queries' depth adjacency in the caller's scratch
+    // (or the queue's)
     hipStream_t st = q->stream;
     const uint64_t nrows = std::max<uint64_t>(1, data->total_nodes);
     const uint64_t nqa = uint64_t(query->num_graphs) * 32;
-    uint64_t* rows = nullptr;
-    if (int rc = queue_scratch(q, 8 * (nrows + 1) + 4 * nqa, (void**)&rows)) return rc;
+    uint64_t* rows = static_cast<uint64_t*>(scratch);
+    if (!rows)
+      if (int rc = queue_scratch(q, staged_join_scratch_bytes(query, data), (void**)&rows)) return rc;
     auto* cursor = reinterpret_cast<unsigned long long*>(rows + nrows);
     auto* qa = reinterpret_cast<uint32_t*>(rows + nrows + 1);
     if (nqa)
@@ -1264,6 +1273,19 @@ int bms2_signature_create(bms2_queue q, uint64_t data_nodes, uint64_t query_node
   }
   *out = s;
   return BMS2_OK;
+}
+
+uint64_t bms2_signature_alloc_size(bms2_signature s, int scope) {
+  // label histogram (8 B) + the four u64 planes of counted nodes (32 B) per node
+  return (s && (scope == BMS2_SCOPE_DATA || scope == BMS2_SCOPE_QUERY)) ? 40 * s->n[scope] : 0;
+}
+
+uint64_t bms2_path_signature_alloc_size(bms2_path_signature s, int scope) {
+  return (s && (scope == BMS2_SCOPE_DATA || scope == BMS2_SCOPE_QUERY)) ? 32 * s->n[scope] : 0;
+}
+
+uint64_t bms2_cycle_signature_alloc_size(bms2_cycle_signature s, int scope) {
+  return (s && (scope == BMS2_SCOPE_DATA || scope == BMS2_SCOPE_QUERY)) ? 8 * s->n[scope] : 0;
 }
 
 int bms2_signature_destroy(bms2_signature s) {
@@ -1485,6 +1507,22 @@ int bms2_candidates_copy_to_host(bms2_candidates c, uint64_t* out) {
   return BMS2_OK;
 }
 
+int bms2_candidates_copy_rows(bms2_candidates c, uint64_t first_row, uint64_t num_rows,
+                              uint64_t* out) {
+  if (!c || (num_rows && !out)) return set_error(BMS2_ERR_INVALID, "null argument");
+  if (first_row > c->rows || num_rows > c->rows - first_row)
+    return set_error(BMS2_ERR_INVALID, "rows [%llu, %llu) outside the %llu rows",
+                     (unsigned long long)first_row, (unsigned long long)(first_row + num_rows),
+                     (unsigned long long)c->rows);
+  if (int rc = use_device(c->q)) return rc;
+  if (num_rows * c->W) {
+    BMS2_HIP(hipMemcpyAsync(out, c->bits + first_row * c->W, 8 * num_rows * c->W,
+                            hipMemcpyDeviceToHost, c->q->stream));
+    BMS2_HIP(hipStreamSynchronize(c->q->stream));
+  }
+  return BMS2_OK;
+}
+
 int bms2_candidates_row_counts(bms2_candidates c, uint32_t* out) {
   if (!c || !out) return set_error(BMS2_ERR_INVALID, "null argument");
   if (int rc = use_device(c->q)) return rc;
@@ -1622,9 +1660,26 @@ int bms2_gmcr_generate(bms2_gmcr gm, const bms2_device_csr* query, const bms2_de
   }
   if (e == hipSuccess) e = hipGetLastError();
   uint32_t total = 0;
-  // The total sizes the index array (gmcr.hpp:99 reads it back the same way).
+  // The total sizes the index array (gmcr.hpp:99 reads it back the same way).  Offsets are u32
+  // as in the reference; when D x Q could exceed that, the exact 64-bit sum of the per-graph
+  // counts decides whether the batch fits before anything is sized from a wrapped total.
   if (e == hipSuccess) e = hipMemcpyAsync(&total, gm->offsets + D, 4, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e == hipSuccess && uint64_t(D) * Q > UINT32_MAX) {
+    std::vector<uint32_t> hc(D);
+    e = hipMemcpy(hc.data(), cnt, 4ull * D, hipMemcpyDeviceToHost);
+    uint64_t sum = 0;
+    for (uint32_t v : hc) sum += v;
+    if (e == hipSuccess && sum > UINT32_MAX) {
+      (void)hipFree(vbits);
+      (void)hipFree(cnt);
+      (void)hipFree(gm->offsets);
+      gm->offsets = nullptr;
+      return set_error(BMS2_ERR_UNSUPPORTED,
+                       "%llu GMCR pairs exceed the u32 offsets of the mapping (gmcr.hpp:54-127); "
+                       "split the data batch", (unsigned long long)sum);
+    }
+  }
   if (e == hipSuccess) {
     gm->total = total;
     if ((rc = device_alloc((void**)&gm->indices, 4ull * std::max<uint32_t>(total, 1)))) {
@@ -1668,22 +1723,27 @@ int bms2_gmcr_copy_to_host(bms2_gmcr g, uint32_t* offsets, uint32_t* indices) {
 // ---- join
 int bms2_join_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
                          bms2_candidates cand, bms2_gmcr gmcr, uint64_t* d_num_matches,
-                         int find_first, uint64_t* d_pair_counts, bms2_event* ev) {
+                         int find_first, uint64_t* d_pair_counts, uint64_t pair_capacity,
+                         bms2_event* ev) {
   if (int rc = use_device(q)) return rc;
   if (!query || !data || !cand || !gmcr || !d_num_matches)
     return set_error(BMS2_ERR_INVALID, "null argument");
   if (!gmcr->offsets) return set_error(BMS2_ERR_INVALID, "gmcr not generated");
   if (gmcr->num_data != data->num_graphs) return set_error(BMS2_ERR_INVALID, "gmcr of another batch");
+  if (d_pair_counts && pair_capacity < gmcr->total)
+    return set_error(BMS2_ERR_CAPACITY, "per-pair counts hold %llu entries, the GMCR has %llu",
+                     (unsigned long long)pair_capacity, (unsigned long long)gmcr->total);
   if (query->max_graph_nodes > BMS2_MAX_QUERY_NODES)
     return set_error(BMS2_ERR_UNSUPPORTED, "query with %u nodes (limit %u, isomorphism.hpp:248)",
                      query->max_graph_nodes, BMS2_MAX_QUERY_NODES);
   EventScope es{q, ev};
   if (int rc = es.begin()) return rc;
   if (int rc = staged_join(q, query, data, cand, gmcr, find_first, nullptr, d_num_matches,
-                           d_pair_counts, nullptr))
+                           d_pair_counts, nullptr, nullptr))
     return rc;
   return es.end();
 }
+
 
 // ---- GPU-side CSR packing
 int bms2_device_csr_pack(bms2_queue q, uint32_t num_graphs, const uint32_t* nodes,
@@ -1896,12 +1956,16 @@ int bms2_join_with_partial_matches(bms2_queue q, const bms2_device_csr* query,
                                    const bms2_device_csr* data, bms2_candidates cand,
                                    bms2_gmcr gmcr, uint32_t end, const int32_t* d_partial,
                                    uint64_t num_partial, uint64_t* d_num_matches, int find_first,
-                                   uint64_t* d_pair_counts, bms2_event* ev) {
+                                   uint64_t* d_pair_counts, uint64_t pair_capacity,
+                                   bms2_event* ev) {
   if (int rc = check_cs(q, query, data, cand, end)) return rc;
   if (!gmcr || !d_num_matches || !d_pair_counts || (num_partial && !d_partial))
     return set_error(BMS2_ERR_INVALID, "null argument");
   if (!gmcr->offsets) return set_error(BMS2_ERR_INVALID, "gmcr not generated");
   if (gmcr->num_data != data->num_graphs) return set_error(BMS2_ERR_INVALID, "gmcr of another batch");
+  if (pair_capacity < gmcr->total)
+    return set_error(BMS2_ERR_CAPACITY, "per-pair counts hold %llu entries, the GMCR has %llu",
+                     (unsigned long long)pair_capacity, (unsigned long long)gmcr->total);
   EventScope es{q, ev};
   if (int rc = es.begin()) return rc;
   hipStream_t st = q->stream;

@@ -112,8 +112,11 @@ int bms2_queue_create(int device, bms2_queue* out);
 /* A queue over an existing hipStream_t (not owned). */
 int bms2_queue_wrap(int device, void* hip_stream, bms2_queue* out);
 /* Releases the queue (and its stream when owned).  A queue keeps a grow-only scratch buffer for
- * the filter, refinement and join calls (about 8 bytes per data node of the largest batch they
- * ran on) until it is destroyed. */
+ * the filter, refinement and join calls until it is destroyed: max(64 B per query node, 8 B per
+ * data node + 128 B per query graph) of the largest batch they ran on, plus an eighth, at least
+ * 1 MiB.
+ * A queue is used from one host thread at a time (like one stream's launches, calls from two
+ * threads would interleave and share the scratch buffer); use a queue per thread. */
 int bms2_queue_destroy(bms2_queue q);
 int bms2_queue_wait(bms2_queue q); /* queue.wait_and_throw() */
 void* bms2_queue_stream(bms2_queue q);
@@ -123,8 +126,19 @@ void* bms2_queue_stream(bms2_queue q);
 int bms2_event_elapsed_ms(bms2_event e, float* ms); /* waits for the event */
 int bms2_event_destroy(bms2_event e);
 
+/* The queue's device: global memory bytes, compute units and name (name_capacity bytes, NUL
+ * terminated); every output is nullable.  Replaces queue.get_device().get_info<...>() of
+ * BMS2.cpp:15-16 and device::getDeviceMemorySize (device.hpp:19-21). */
+int bms2_queue_device_info(bms2_queue q, uint64_t* global_mem_bytes, uint32_t* compute_units,
+                           char* name, uint64_t name_capacity);
+
 /* Device memory helpers for callers without another allocator. */
 int bms2_device_malloc(bms2_queue q, uint64_t bytes, void** out);
+/* Pinned host memory that kernels read and write directly (sycl::malloc_shared in BMS2.cpp:205,
+ * 243-244, 269): the drivers' match totals, partial-match buffers and pair flags.  Released with
+ * bms2_shared_free, which first waits for the queue. */
+int bms2_shared_malloc(bms2_queue q, uint64_t bytes, void** out);
+int bms2_shared_free(bms2_queue q, void* ptr);
 int bms2_device_free(bms2_queue q, void* ptr);
 int bms2_memcpy_h2d(bms2_queue q, void* dst, const void* src, uint64_t bytes);
 int bms2_memcpy_d2h(bms2_queue q, void* dst, const void* src, uint64_t bytes); /* synchronous */
@@ -174,6 +188,9 @@ typedef struct bms2_signature_s* bms2_signature;
 int bms2_signature_create(bms2_queue q, uint64_t data_nodes, uint64_t query_nodes,
                           bms2_signature* out);
 int bms2_signature_destroy(bms2_signature s);
+/* getDataSignatureAllocationSize / getQuerySignatureAllocationSize (signature.hpp:180-193): device
+ * bytes held for one scope (40 per node: the histogram and four u64 planes of counted nodes). */
+uint64_t bms2_signature_alloc_size(bms2_signature s, int scope);
 /* generateDataSignatures / generateQuerySignatures (signature.hpp:56-84, 124-151). */
 int bms2_signature_generate(bms2_signature s, const bms2_device_csr* g, int scope, bms2_event* ev);
 /* refineDataSignatures / refineQuerySignatures (signature.hpp:61-73, 336-389): histogram of the
@@ -193,6 +210,8 @@ typedef struct bms2_path_signature_s* bms2_path_signature;
 int bms2_path_signature_create(bms2_queue q, uint64_t data_nodes, uint64_t query_nodes,
                                bms2_path_signature* out); /* signature.hpp:440-447 */
 int bms2_path_signature_destroy(bms2_path_signature s);
+/* signature.hpp:521-522: 32 bytes per node of the scope. */
+uint64_t bms2_path_signature_alloc_size(bms2_path_signature s, int scope);
 int bms2_path_signature_generate(bms2_path_signature s, const bms2_device_csr* g, int scope,
                                  bms2_event* ev); /* signature.hpp:455-519 */
 int bms2_path_signature_copy_to_host(bms2_path_signature s, int scope, uint64_t* out);
@@ -204,6 +223,8 @@ typedef struct bms2_cycle_signature_s* bms2_cycle_signature;
 int bms2_cycle_signature_create(bms2_queue q, uint64_t data_nodes, uint64_t query_nodes,
                                 bms2_cycle_signature* out); /* signature.hpp:557-566 */
 int bms2_cycle_signature_destroy(bms2_cycle_signature s);
+/* signature.hpp:721-722: 8 bytes per node of the scope. */
+uint64_t bms2_cycle_signature_alloc_size(bms2_cycle_signature s, int scope);
 int bms2_cycle_signature_generate(bms2_cycle_signature s, const bms2_device_csr* g, int scope,
                                   bms2_event* ev); /* signature.hpp:574-719 */
 int bms2_cycle_signature_copy_to_host(bms2_cycle_signature s, int scope, uint64_t* out);
@@ -219,6 +240,11 @@ uint64_t bms2_candidates_words_per_row(bms2_candidates c);
 int bms2_candidates_copy_to_host(bms2_candidates c, uint64_t* out); /* candidates.hpp:181-186 */
 /* Per-row popcounts (getCandidatesCount(source_node), candidates.hpp:73-80) into out[rows]. */
 int bms2_candidates_row_counts(bms2_candidates c, uint32_t* out);
+/* Rows [first_row, first_row + num_rows) of the matrix (num_rows x words_per_row u64) into out:
+ * what getCandidatesCount(source_node[, start, end]) and getCandidateAt read on the host
+ * (candidates.hpp:73-158). */
+int bms2_candidates_copy_rows(bms2_candidates c, uint64_t first_row, uint64_t num_rows,
+                              uint64_t* out);
 
 /* ---------------------------------------------------------------- filter / refine
  * filterCandidates (isomorphism.hpp:30-104): sets bit (q, d) when the labels agree (or the query
@@ -234,7 +260,9 @@ int bms2_refine_candidates(bms2_queue q, const bms2_device_csr* query, const bms
 
 /* ---------------------------------------------------------------- GMCR
  * mapping::GMCR (gmcr.hpp:16-145): CSR from data graph to the query graphs with more than one
- * node whose every node keeps a candidate in that data graph; ascending query id per segment. */
+ * node whose every node keeps a candidate in that data graph; ascending query id per segment.
+ * Offsets are u32 as in the reference: a batch with more than 2^32 - 1 GMCR pairs fails with
+ * BMS2_ERR_UNSUPPORTED. */
 typedef struct bms2_gmcr_s* bms2_gmcr;
 int bms2_gmcr_create(bms2_queue q, bms2_gmcr* out); /* gmcr.hpp:26 */
 int bms2_gmcr_destroy(bms2_gmcr g);
@@ -251,10 +279,13 @@ int bms2_gmcr_copy_to_host(bms2_gmcr g, uint32_t* data_graph_offsets /*[D+1]*/,
  * with f(i) in the domain of query node i inside the data graph and query adjacency equal to data
  * adjacency on every mapped pair (induced).  find_first counts 1 per pair with any embedding.
  * d_num_matches is a device u64 the call atomically adds the total to (the caller zeroes it, as
- * BMS2.cpp:205-206 does).  d_pair_counts (nullable) receives one u64 per GMCR entry. */
+ * BMS2.cpp:205-206 does).  d_pair_counts (nullable) receives one u64 per GMCR entry; it holds
+ * pair_capacity entries, and a call whose GMCR has more (bms2_gmcr_total) fails with
+ * BMS2_ERR_CAPACITY before any device work is enqueued. */
 int bms2_join_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
                          bms2_candidates cand, bms2_gmcr gmcr, uint64_t* d_num_matches,
-                         int find_first, uint64_t* d_pair_counts, bms2_event* ev);
+                         int find_first, uint64_t* d_pair_counts, uint64_t pair_capacity,
+                         bms2_event* ev);
 
 /* CS (cluster) join, step 1: joinPartialCandidates2 (isomorphism.hpp:1021-1178), the --use-cs
  * path of BMS2.cpp:217-259.  Every embedding of query 0's first `end` nodes (the cluster
@@ -275,13 +306,15 @@ int bms2_join_partial_candidates(bms2_queue q, const bms2_device_csr* query,
  * n_q > end counts its extensions to depths end..n_q-1 (q's domains, q's induced adjacency against
  * every earlier depth including the prototype part, injective), n_q < end nothing.
  * d_pair_counts (required, one u64 per GMCR entry, zeroed by the call) receives the counts, or in
- * find_first mode a 0/1 flag per entry (the reference's pair_done flags); the total is
- * atomically added to the device u64 *d_num_matches (the caller zeroes it). */
+ * find_first mode a 0/1 flag per entry (the reference's pair_done flags); it holds pair_capacity
+ * entries, and a GMCR with more fails with BMS2_ERR_CAPACITY before any device work.  The total
+ * is atomically added to the device u64 *d_num_matches (the caller zeroes it). */
 int bms2_join_with_partial_matches(bms2_queue q, const bms2_device_csr* query,
                                    const bms2_device_csr* data, bms2_candidates cand,
                                    bms2_gmcr gmcr, uint32_t end, const int32_t* d_partial,
                                    uint64_t num_partial, uint64_t* d_num_matches, int find_first,
-                                   uint64_t* d_pair_counts, bms2_event* ev);
+                                   uint64_t* d_pair_counts, uint64_t pair_capacity,
+                                   bms2_event* ev);
 
 /* ---------------------------------------------------------------- fused pipeline
  * The stage sequence of BMS2.cpp:131-234 (signatures -> filter -> refine x K -> GMCR -> join) as

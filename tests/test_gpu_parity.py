@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 import oracle
-from bms2_amd import api, graphs
+from bms2_amd import _lib, api, graphs
 
 pytestmark = pytest.mark.gpu
 
@@ -248,3 +248,69 @@ def test_shell_histogram_label_sets(queue, qlabels):
         rc, rv, rs = oracle.pipeline(queries, data, K, False)
         assert np.array_equal(valid, rv) and np.array_equal(counts, rc)
         assert np.array_equal(stats[:4], rs)
+
+
+@pytest.fixture(scope="module")
+def large_queries_small_graphs(uneven_2k):
+    """Queries of 17-32 nodes of the uneven workload against its first 1,000 data graphs of at
+    most 64 nodes: the staged join's 32-deep variant over precomputed adjacency rows."""
+    data, queries = uneven_2k
+    n = np.diff(np.asarray(data.graph_offsets))
+    small = [int(i) for i in np.flatnonzero(n <= 64)[:1000]]
+    qn = np.diff(np.asarray(queries.graph_offsets))
+    big = [int(i) for i in np.flatnonzero(qn > 16)]
+    return (graphs.GraphBatch.from_graphs([data.graph(i) for i in small]),
+            graphs.GraphBatch.from_graphs([queries.graph(i) for i in big]))
+
+
+@pytest.mark.parametrize("K,find_first", [(2, False), (6, False), (6, True)])
+def test_staged_large_queries_small_graphs(queue, large_queries_small_graphs, K, find_first):
+    data, queries = large_queries_small_graphs
+    assert np.diff(np.asarray(data.graph_offsets)).max() <= 64
+    assert np.diff(np.asarray(queries.graph_offsets)).min() > 16
+    out = api.run_staged(queue, queries, data, K=K, find_first=find_first, keep_domains=False)
+    ref = oracle.staged(queries, data, K, find_first=find_first, keep_domains=False)
+    assert len(ref.gmcr_indices) > 100
+    off, idx = out["gmcr"]
+    assert np.array_equal(off, ref.gmcr_offsets) and np.array_equal(idx, ref.gmcr_indices)
+    assert np.array_equal(out["counts"], ref.counts)
+    assert out["total"] == ref.total
+
+
+def test_join_rejects_small_pair_buffer(queue, molecules_2k):
+    """A per-pair count buffer smaller than the GMCR is refused on the host before any kernel
+    runs (both joins), and the same call with the right size succeeds."""
+    data, queries = molecules_2k
+    sub = data.subset(0, 200)
+    dq = api.DeviceBatchedCSRGraph(queue, queries)
+    dd = api.DeviceBatchedCSRGraph(queue, sub)
+    sig = api.Signature(queue, dd.total_nodes, dq.total_nodes)
+    path = api.PathSignature(queue, dd.total_nodes, dq.total_nodes)
+    cyc = api.CycleSignature(queue, dd.total_nodes, dq.total_nodes)
+    cand = api.Candidates(queue, dq.total_nodes, dd.total_nodes)
+    for g, fn in ((dd, "data"), (dq, "query")):
+        getattr(sig, f"generate_{fn}_signatures")(g)
+        getattr(path, f"generate_{fn}_path_signatures")(g)
+        getattr(cyc, f"generate_{fn}_cycle_signatures")(g)
+    api.filter_candidates(queue, dq, dd, sig, path, cyc, cand)
+    gm = api.GMCR(queue)
+    gm.generate_gmcr(dq, dd, cand)
+    queue.wait()
+    T = gm.total
+    assert T > 1
+    total = api.DeviceBuffer(queue, 8, zero=True)
+    short = api.DeviceBuffer(queue, 8 * (T - 1), zero=True)
+    with pytest.raises(api.Bms2Error) as e:
+        api.join_candidates(queue, dq, dd, cand, gm, total, False, short)
+    assert e.value.code == _lib.ERR_CAPACITY
+    end = int(queries.graph_offsets[1] - queries.graph_offsets[0])
+    part, n = api.join_partial_candidates(queue, dq, dd, cand, end)
+    with pytest.raises(api.Bms2Error) as e:
+        api.join_with_partial_matches(queue, dq, dd, cand, gm, end, part, n, total, short, False)
+    assert e.value.code == _lib.ERR_CAPACITY
+    assert int(total.to_numpy(np.uint64)[0]) == 0  # nothing ran
+    full = api.DeviceBuffer(queue, 8 * T, zero=True)
+    api.join_candidates(queue, dq, dd, cand, gm, total, False, full)
+    queue.wait()
+    ref = oracle.staged(queries, sub, 0, keep_domains=False)
+    assert np.array_equal(full.to_numpy(np.uint64, T), ref.counts)
