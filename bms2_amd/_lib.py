@@ -165,6 +165,7 @@ PROTOTYPES: dict[str, tuple] = {
     "bms2_gmcr_device_ptrs": (i32, [vp, vpp, vpp]),
     "bms2_gmcr_copy_to_host": (i32, [vp, vp, vp]),
     "bms2_join_candidates": (i32, [vp, DCP, DCP, vp, vp, vp, i32, vp, u64, vpp]),
+    "bms2_join_candidates_noalloc": (i32, [vp, DCP, DCP, vp, vp, vp, i32, vp, u64]),
     "bms2_join_partial_candidates": (i32, [vp, DCP, DCP, vp, u32, vp, u64, C.POINTER(C.c_uint64),
                                            vpp]),
     "bms2_join_with_partial_matches": (i32, [vp, DCP, DCP, vp, vp, u32, vp, u64, vp, i32, vp,

@@ -3,6 +3,7 @@
 // inspect the domain bit-matrix after every refinement step.  Kernels here favour exactness for
 // any graph up to 256 nodes over speed; the fused pipeline is the fast path.
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "device_common.hpp"
@@ -1744,6 +1745,65 @@ int bms2_join_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_
   return es.end();
 }
 
+}  // extern "C"
+
+namespace {
+// The join's scratch for bms2_join_candidates_noalloc: one array per GPU, held for the life of
+// the process.  Host code reaches it only through the address hipGetSymbolAddress returns for
+// the current device (the array's own name is a host-side stand-in).
+constexpr uint64_t kNoAllocScratchBytes = BMS2_JOIN_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) uint8_t join_scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[256] = {};
+
+int noalloc_scratch(int device, void** out) {
+  if (device < 0 || device >= 256) return set_error(BMS2_ERR_INVALID, "device %d", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(join_scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return set_error(BMS2_ERR_HIP, "hipGetSymbolAddress of the join scratch failed: %s",
+                       hipGetErrorString(e));
+    }
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return BMS2_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int bms2_join_candidates_noalloc(bms2_queue q, const bms2_device_csr* query,
+                                 const bms2_device_csr* data, bms2_candidates cand, bms2_gmcr gmcr,
+                                 uint64_t* d_num_matches, int find_first, uint64_t* d_pair_counts,
+                                 uint64_t pair_capacity) {
+  if (int rc = use_device(q)) return rc;
+  if (!query || !data || !cand || !gmcr || !d_num_matches)
+    return set_error(BMS2_ERR_INVALID, "null argument");
+  if (!gmcr->offsets) return set_error(BMS2_ERR_INVALID, "gmcr not generated");
+  if (gmcr->num_data != data->num_graphs) return set_error(BMS2_ERR_INVALID, "gmcr of another batch");
+  if (d_pair_counts && pair_capacity < gmcr->total)
+    return set_error(BMS2_ERR_CAPACITY, "per-pair counts hold %llu entries, the GMCR has %llu",
+                     (unsigned long long)pair_capacity, (unsigned long long)gmcr->total);
+  if (query->max_graph_nodes > BMS2_MAX_QUERY_NODES)
+    return set_error(BMS2_ERR_UNSUPPORTED, "query with %u nodes (limit %u, isomorphism.hpp:248)",
+                     query->max_graph_nodes, BMS2_MAX_QUERY_NODES);
+  const uint64_t need = staged_join_scratch_bytes(query, data);
+  if (need > kNoAllocScratchBytes)
+    return set_error(BMS2_ERR_CAPACITY,
+                     "the join needs %llu bytes of scratch for %llu data nodes and %u queries; the "
+                     "allocation-free join holds %llu (8 bytes per data node + 128 per query)",
+                     (unsigned long long)need, (unsigned long long)data->total_nodes,
+                     query->num_graphs, (unsigned long long)kNoAllocScratchBytes);
+  void* scratch = nullptr;
+  if (int rc = noalloc_scratch(q->device, &scratch)) return rc;
+  return staged_join(q, query, data, cand, gmcr, find_first, nullptr, d_num_matches,
+                     d_pair_counts, nullptr, scratch);
+}
 
 // ---- GPU-side CSR packing
 int bms2_device_csr_pack(bms2_queue q, uint32_t num_graphs, const uint32_t* nodes,

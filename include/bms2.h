@@ -287,6 +287,20 @@ int bms2_join_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_
                          int find_first, uint64_t* d_pair_counts, uint64_t pair_capacity,
                          bms2_event* ev);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): bms2_join_candidates
+ * with no event and no allocation of any kind.  The join's scratch (one adjacency row per data
+ * node, a work cursor, the queries' depth adjacency: 8 bytes per data node + 128 per query graph
+ * + 8, needed only when every data graph has at most 64 nodes) lives in a static device array of
+ * BMS2_JOIN_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the process
+ * ends; a batch that needs more fails with BMS2_ERR_CAPACITY.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (from two streams or two host threads) race. */
+#define BMS2_JOIN_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int bms2_join_candidates_noalloc(bms2_queue q, const bms2_device_csr* query,
+                                 const bms2_device_csr* data, bms2_candidates cand, bms2_gmcr gmcr,
+                                 uint64_t* d_num_matches, int find_first, uint64_t* d_pair_counts,
+                                 uint64_t pair_capacity);
+
 /* CS (cluster) join, step 1: joinPartialCandidates2 (isomorphism.hpp:1021-1178), the --use-cs
  * path of BMS2.cpp:217-259.  Every embedding of query 0's first `end` nodes (the cluster
  * prototype; BMS2 --use-cs takes end = its node count) in every data graph, each node inside its
