@@ -90,7 +90,8 @@ class PipelineOptions(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("small_waves", C.c_uint32), ("large_waves", C.c_uint32), ("slots", C.c_uint32),
-                ("candidates", C.c_uint32), ("best_ms", C.c_float)]
+                ("candidates", C.c_uint32), ("best_ms", C.c_float), ("big_waves", C.c_uint32),
+                ("big_block_waves", C.c_uint32), ("big_row_words", C.c_uint32)]
 
 
 class Adam(C.Structure):

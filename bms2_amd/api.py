@@ -544,7 +544,9 @@ class Pipeline(_Handle):
         check(_L().bms2_pipeline_autotune(self.h, C.c_void_p(counts.ptr), C.c_void_p(valid.ptr),
                                           int(runs), C.byref(t)))
         return dict(small_waves=t.small_waves, large_waves=t.large_waves, slots=t.slots,
-                    candidates=t.candidates, best_ms=round(float(t.best_ms), 3))
+                    big_waves=t.big_waves, big_block_waves=t.big_block_waves,
+                    big_row_words=t.big_row_words, candidates=t.candidates,
+                    best_ms=round(float(t.best_ms), 3))
 
     def set_order(self, order: np.ndarray | None) -> None:
         """Join matching order: order[q, d] = local query node matched at depth d (u8 [Q, 32]);

@@ -1074,6 +1074,7 @@ struct bms2_pipeline_s {
   // launch shape (autotuned): waves per CU of each join, LDS slots per small-join wave
   int cus = 256;
   uint32_t small_waves = 8, large_waves = 12, slots = 16;
+  BigJoinShape big_shape;  // joins of the graphs above 64 nodes
   uint32_t* d_packs = nullptr;      // pack p: graphs d_pack_gids[d_packs[p] .. d_packs[p + 1])
   uint32_t* d_pack_gids = nullptr;
   std::vector<uint32_t> win_pack, win_graph;  // pack windows (see build_data_packs)
@@ -1234,7 +1235,8 @@ int run_big(bms2_pipeline_s* p, uint64_t* d_counts, uint64_t* d_valid, uint32_t 
   }
   hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_big_stats, 8u);
   if (int rc = staged_join(q, dq, db, p->big_cand, p->big_gmcr, int(p->opt.find_first),
-                           p->d_jorder, p->d_big_stats + 4, p->d_big_pc, p->d_big_pn, nullptr))
+                           p->d_jorder, p->d_big_stats + 4, p->d_big_pc, p->d_big_pn, nullptr, 0,
+                           &p->big_shape))
     return rc;
   const uint32_t Q = p->query.num_graphs, VW = (Q + 63) / 64;
   hipLaunchKernelGGL(big_zero_kernel, dim3(p->num_big), dim3(256), 0, s, p->d_big_gid, p->num_big,
@@ -1770,32 +1772,50 @@ extern "C" int bms2_pipeline_autotune(bms2_pipeline p, uint64_t* d_counts, uint6
   runs = std::max<uint32_t>(runs, 1);
   struct Cand {
     uint32_t small_waves, large_waves, slots;
+    BigJoinShape big;
   };
   std::vector<Cand> cands;
   const uint32_t lw0 = p->nq_cap <= 16 ? 12 : 6;
+  const BigJoinShape b0 = p->big_shape;
   // LDS fits 8 small-join waves per CU with 16 slots, 10 with 8
   for (uint32_t sw : {7u, 8u, 10u})
-    for (uint32_t sl : {8u, 16u}) cands.push_back({sw, lw0, sl});
-  for (uint32_t lw : {lw0 / 2, lw0 * 3 / 4}) cands.push_back({8, lw, 16});
-  const Cand keep{p->small_waves, p->large_waves, p->slots};
-  float best = 0.f;
-  Cand best_c = keep;
-  bms2_event ev = nullptr;
-  for (size_t i = 0; i < cands.size(); ++i) {
-    const Cand& c = cands[i];
-    if (c.small_waves != p->small_waves || c.large_waves != p->large_waves || c.slots != p->slots) {
+    for (uint32_t sl : {8u, 16u}) cands.push_back({sw, lw0, sl, b0});
+  for (uint32_t lw : {lw0 / 2, lw0 * 3 / 4}) cands.push_back({8, lw, 16, b0});
+  // graphs above 64 nodes: waves per CU, waves per workgroup and adjacency-row words of the
+  // size-class joins
+  if (p->num_big)
+    for (uint32_t bw : {6u, 12u, 16u})
+      for (uint32_t wg : {1u, 4u})
+        for (uint32_t rw : {2u, 4u}) {
+          BigJoinShape b;
+          b.waves = bw;
+          b.block_waves = wg;
+          b.row_words = rw;
+          cands.push_back({8, lw0, 16, b});
+        }
+  auto same_join_shape = [&](const Cand& c) {
+    return c.small_waves == p->small_waves && c.large_waves == p->large_waves && c.slots == p->slots;
+  };
+  auto apply = [&](const Cand& c) {
+    if (!same_join_shape(c)) {
       if (p->graph) (void)hipGraphExecDestroy(p->graph);
       p->graph = nullptr;  // grid shapes are baked into the graph
     }
     p->small_waves = c.small_waves;
     p->large_waves = c.large_waves;
     p->slots = c.slots;
+    p->big_shape = c.big;
+  };
+  const Cand keep{p->small_waves, p->large_waves, p->slots, p->big_shape};
+  float best = 0.f;
+  Cand best_c = keep;
+  bms2_event ev = nullptr;
+  for (size_t i = 0; i < cands.size(); ++i) {
+    apply(cands[i]);
     // one untimed run (measures the batch, captures the graph), then the timed runs
     for (uint32_t r = 0; r <= runs; ++r) {
       if (int rc = bms2_pipeline_run(p, d_counts, d_valid, nullptr, nullptr, r ? &ev : nullptr)) {
-        p->small_waves = keep.small_waves;
-        p->large_waves = keep.large_waves;
-        p->slots = keep.slots;
+        apply(keep);
         return rc;
       }
       if (r) {
@@ -1805,25 +1825,21 @@ extern "C" int bms2_pipeline_autotune(bms2_pipeline p, uint64_t* d_counts, uint6
         ev = nullptr;
         if (best == 0.f || ms < best) {
           best = ms;
-          best_c = c;
+          best_c = cands[i];
         }
       }
     }
   }
-  if (best_c.small_waves != p->small_waves || best_c.large_waves != p->large_waves ||
-      best_c.slots != p->slots) {
-    if (p->graph) (void)hipGraphExecDestroy(p->graph);
-    p->graph = nullptr;
-  }
-  p->small_waves = best_c.small_waves;
-  p->large_waves = best_c.large_waves;
-  p->slots = best_c.slots;
+  apply(best_c);
   if (out) {
     out->small_waves = best_c.small_waves;
     out->large_waves = best_c.large_waves;
     out->slots = best_c.slots;
     out->candidates = uint32_t(cands.size());
     out->best_ms = best;
+    out->big_waves = best_c.big.waves;
+    out->big_block_waves = best_c.big.block_waves;
+    out->big_row_words = best_c.big.row_words;
   }
   return BMS2_OK;
 }

@@ -11,8 +11,9 @@ struct bms2_queue_s {
   bool owned;
   // grow-only scratch ordered on `stream`, released with the queue: the filter's query records
   // (64 B per query node), the refinement's signature planes (8 B per data node) and the join's
-  // adjacency rows and depth adjacency (8 B per data node + 128 B per query); it grows to
-  // max(64 query nodes, 8 data nodes + 128 queries) bytes plus an eighth, at least 1 MiB
+  // adjacency rows and depth adjacency (8 B per data node + 128 B per query for graphs of up to
+  // 64 nodes; 16 or 32 B per data node + 128 B per query + 8 B per GMCR pair above); it grows to
+  // the largest of these plus an eighth, at least 1 MiB
   void* scratch = nullptr;
   uint64_t scratch_bytes = 0;
 };
@@ -59,13 +60,23 @@ int device_alloc(void** p, uint64_t bytes);
 int queue_scratch(bms2_queue q, uint64_t bytes, void** p);
 
 // Staged join (staged.hip) with an optional matching order (d_jorder [Q][32], null = identity)
-// and optional per-pair counts and DFS nodes; enqueued on q, no event.  Its scratch is
-// `scratch` (staged_join_scratch_bytes of device memory) or, when null, the queue's.
-uint64_t staged_join_scratch_bytes(const bms2_device_csr* query, const bms2_device_csr* data);
+// and optional per-pair counts and DFS nodes; enqueued on q, no event.  Its scratch is the
+// caller's `scratch` of scratch_bytes (the joins of graphs above 64 nodes then run as many waves
+// as their stashes fit) or, when null, the queue's (staged_join_scratch_bytes); `shape`
+// (nullable: the defaults) sets the launch of the joins of graphs above 64 nodes.
+// Launch shape of the joins of graphs above 64 nodes (autotuned by the fused pipeline): waves
+// per CU, waves per workgroup (1, 2 or 4), u64 words per adjacency row for graphs of 65-128
+// nodes (2, or 4 to run every graph in the 4-word instance).
+struct BigJoinShape {
+  uint32_t waves = 12, block_waves = 1, row_words = 2;
+};
+
+uint64_t staged_join_scratch_bytes(const bms2_device_csr* query, const bms2_device_csr* data,
+                                   uint64_t gmcr_pairs);
 int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
                 bms2_candidates cand, bms2_gmcr gmcr, int find_first, const uint8_t* d_jorder,
                 uint64_t* d_total, uint64_t* d_pair_counts, uint64_t* d_pair_nodes,
-                void* scratch);
+                void* scratch, uint64_t scratch_bytes, const BigJoinShape* shape);
 
 // Zeroes a Signature's label signatures and reachability state and a Candidates matrix (fill
 // kernels on the signature's queue), so the staged sequence can run again on the same objects.

@@ -506,105 +506,274 @@ __global__ __launch_bounds__(256) void gmcr_fill_kernel(const uint64_t* __restri
 
 // ------------------------------------------------------------------ join
 
-// JoinCandidates3Optimized (isomorphism.hpp:309-468): one lane per GMCR pair, standard
-// backtracking over bitsets of the data graph's local ids (up to 4 words = 256 nodes).  jorder
-// (nullable, [Q][32]) gives the local query node matched at each depth (identity when null, the
-// reference's order); per_pair_nodes (nullable) receives the DFS nodes expanded per pair.
-template <int NW>
-__global__ __launch_bounds__(256) void join_kernel(
-    const uint64_t* __restrict__ cand, uint64_t W, const uint32_t* __restrict__ qgo,
-    const uint32_t* __restrict__ qro, const uint32_t* __restrict__ qci,
-    const uint32_t* __restrict__ dgo, uint32_t D, const uint32_t* __restrict__ dro,
-    const uint32_t* __restrict__ dci, const uint32_t* __restrict__ goff,
-    const uint32_t* __restrict__ gidx, uint64_t T, int find_first, uint64_t* __restrict__ total,
-    uint64_t* __restrict__ per_pair, const uint8_t* __restrict__ jorder,
-    uint64_t* __restrict__ per_pair_nodes) {
-  const uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  uint64_t c = 0;
-  if (t < T) {
-    // data graph of entry t (utils.hpp:110-121)
-    uint32_t lo = 0, hi = D + 1;
-    while (lo < hi) {
-      const uint32_t mid = lo + (hi - lo) / 2;
-      if (t >= goff[mid]) lo = mid + 1; else hi = mid;
-    }
-    const uint32_t g = lo - 1;
-    const uint32_t q = gidx[t];
-    const uint32_t s = dgo[g], e = dgo[g + 1], n = e - s;
-    const uint32_t qb = qgo[q], nq = qgo[q + 1] - qb;
-    uint64_t dom[32][NW];
-    uint32_t qadj[32];   // bit j of qadj[i]: depths j and i adjacent in the query
-    uint32_t pos[32];    // depth of each local query node
-    for (uint32_t i = 0; i < nq; ++i) pos[jorder ? jorder[size_t(q) * 32 + i] : i] = i;
-    for (uint32_t i = 0; i < nq; ++i) {
-      const uint32_t u = jorder ? jorder[size_t(q) * 32 + i] : i;
-      const uint64_t* row = cand + uint64_t(qb + u) * W;
-      for (int k = 0; k < NW; ++k) dom[i][k] = row_bits(row, s, e, k);
-      uint32_t a = 0;
-      for (uint32_t k = qro[qb + u]; k < qro[qb + u + 1]; ++k) a |= 1u << pos[qci[k] - qb];
-      qadj[i] = a;
-    }
-    auto nrow = [&](uint32_t x, uint64_t* out) {
-      for (int k = 0; k < NW; ++k) out[k] = 0;
-      for (uint32_t k = dro[s + x]; k < dro[s + x + 1]; ++k) {
-        const uint32_t y = dci[k] - s;
-        out[y >> 6] |= uint64_t(1) << (y & 63);
-      }
-    };
-    (void)n;
-    uint64_t R[32][NW];
-    uint32_t f[32];
-    uint64_t used[NW];
-    for (int k = 0; k < NW; ++k) {
-      used[k] = 0;
-      R[0][k] = dom[0][k];
-    }
-    int depth = 0;
-    bool stop = false;
-    uint64_t nodes = 0;
-    while (!stop) {
-      int k0 = -1;
-      for (int k = 0; k < NW; ++k)
-        if (R[depth][k]) {
-          k0 = k;
-          break;
-        }
-      if (k0 < 0) {
-        if (depth == 0) break;
-        --depth;
-        used[f[depth] >> 6] &= ~(uint64_t(1) << (f[depth] & 63));
-        continue;
-      }
-      const uint32_t v = uint32_t(k0 * 64 + ctz64(R[depth][k0]));
-      R[depth][k0] &= R[depth][k0] - 1;
-      f[depth] = v;
-      ++nodes;
-      const int nd = depth + 1;
-      uint64_t Rn[NW];
-      for (int k = 0; k < NW; ++k) Rn[k] = dom[nd][k] & ~used[k];
-      Rn[v >> 6] &= ~(uint64_t(1) << (v & 63));
-      for (int i = 0; i < nd; ++i) {
-        uint64_t a[NW];
-        nrow(f[i], a);
-        const bool adj = (qadj[nd] >> i) & 1;
-        for (int k = 0; k < NW; ++k) Rn[k] &= adj ? a[k] : ~a[k];
-      }
-      if (uint32_t(nd) == nq - 1) {
-        for (int k = 0; k < NW; ++k) c += uint64_t(popc64(Rn[k]));
-        if (find_first && c) stop = true;
-      } else {
-        used[v >> 6] |= uint64_t(1) << (v & 63);
-        depth = nd;
-        for (int k = 0; k < NW; ++k) R[nd][k] = Rn[k];
-      }
-    }
-    if (find_first) c = c ? 1 : 0;
-    if (per_pair) per_pair[t] = c;
-    if (per_pair_nodes) per_pair_nodes[t] = nodes;
+// ---- join for data graphs of 65-256 nodes
+
+// Size classes of the GMCR entries on graphs above 64 nodes: graphs of <= 128 and <= 256 nodes
+// (u64 words per adjacency row: 2 or 4), one join instance per class.
+constexpr int kBigClasses = 2;
+// Class of a graph of n nodes: rows of 2 words up to `wide_above` nodes (128, or 64 to give every
+// graph rows of 4 words), 4 words above.
+__host__ __device__ inline int big_class(uint32_t n, uint32_t wide_above) { return n > wide_above ? 1 : 0; }
+constexpr int kBigDepth = BMS2_MAX_QUERY_NODES;  // depths of a lane's stash
+constexpr uint32_t kBigWavesPerCU = 12;
+
+// Multi-word adjacency rows: words [RW x, RW x + RW) of `rows` hold local node x's neighbours
+// (bit y % 64 of word y / 64 for local neighbour y); one thread per data node.
+__global__ void adj_rows_wide_kernel(const uint32_t* __restrict__ go, uint32_t num_graphs,
+                                     const uint32_t* __restrict__ ro,
+                                     const uint32_t* __restrict__ ci, uint64_t n, uint32_t RW,
+                                     uint64_t* __restrict__ rows) {
+  const uint64_t u = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (u >= n) return;
+  const uint32_t s = go[graph_of(go, num_graphs, uint32_t(u))];
+  uint64_t r[4] = {0, 0, 0, 0};
+  for (uint32_t k = ro[u]; k < ro[u + 1]; ++k) {
+    const uint32_t y = ci[k] - s;
+    r[(y >> 6) & 3] |= uint64_t(1) << (y & 63);
   }
-  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-  if (lane_id() == 0 && c) atomicAdd(reinterpret_cast<unsigned long long*>(total), (unsigned long long)c);
+  for (uint32_t w = 0; w < RW; ++w) rows[u * RW + w] = r[w];
 }
+
+// Sorts the GMCR entries into the size classes, one wave per data graph, 64 entries at a time
+// (one atomic per class and step).  Pass 0 counts per class into ctl[c]; pass 1 writes
+// {entry | graph << 32} into list at the class's offset (the prefix of the counts) plus the
+// class's fill counter ctl[kBigClasses + c].  Entries of one graph stay together, in order.
+__global__ __launch_bounds__(256) void big_classify_kernel(
+    const uint32_t* __restrict__ dgo, uint32_t D, const uint32_t* __restrict__ goff, int pass,
+    uint32_t wide_above, unsigned long long* __restrict__ ctl, uint64_t* __restrict__ list) {
+  const uint32_t lane = lane_id();
+  const uint64_t g = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / 64;
+  if (g >= D) return;
+  const int c = big_class(dgo[g + 1] - dgo[g], wide_above);
+  const uint32_t k0 = goff[g], k1 = goff[g + 1];
+  if (k0 == k1) return;
+  uint64_t at = 0;
+  if (lane == 0)
+    at = atomicAdd(ctl + (pass == 0 ? c : kBigClasses + c), (unsigned long long)(k1 - k0));
+  if (pass == 0) return;
+  at = uniform64(readlane64(at, 0));
+  uint64_t base = 0;
+  for (int cc = 0; cc < c; ++cc) base += ctl[cc];
+  for (uint32_t k = k0 + lane; k < k1; k += 64) list[base + at + (k - k0)] = uint64_t(k) | (g << 32);
+}
+
+// The staged join for data graphs of 65 to 64 NW nodes (JoinCandidates3Optimized,
+// isomorphism.hpp:309-468, with standard backtracking over bitsets of the graph's local ids;
+// identity matching order unless jorder gives one, DFS nodes per pair when asked):
+// persistent waves take GMCR entries of one size class 256 at a time from a global cursor and
+// hand them to idle lanes.  At the start of a pair a lane loads every depth's domain (its words
+// of the candidate rows, loads of four depths in flight) into its stash in global memory.  The
+// search keeps, for the current depth, the remaining candidates R and X = (domain of the next
+// depth) minus the nodes matched at earlier depths, intersected with their adjacency
+// constraints, in registers; a step reads one adjacency row (NW words) and gives the next
+// depth's candidates as X & ~{v} & (adjacent ? N(v) : ~N(v)).  A descent pushes R to the stash
+// and builds the new X from the stashed domain and the rows of the nodes matched so far (loads
+// in flight together); a backtrack pops R and rebuilds X the same way.
+
+struct BigJoinParams {
+  const uint64_t* cand;
+  uint64_t W;
+  const uint32_t* qgo;
+  const uint32_t* qa;
+  const uint32_t* gidx;
+  const uint32_t* dgo;
+  const uint64_t* rows;
+  uint32_t RW;
+  const uint64_t* list;         // GMCR entries {entry | graph << 32} sorted by class
+  unsigned long long* ctl;      // class counts, class fill counters, class cursors
+  uint64_t* stash;
+  uint32_t SD;                  // stash depth
+  int find_first;
+  uint64_t* total;
+  uint64_t* per_pair;           // nullable: count per GMCR entry (find_first: 0/1)
+  const uint8_t* jorder;
+  uint64_t* per_pair_nodes;     // nullable
+};
+
+template <int NW, int BW>  // BW: waves per workgroup
+__global__ __launch_bounds__(64 * BW) void join_big_kernel(BigJoinParams p, int cls) {
+  constexpr int NQ = kBigDepth;
+  __shared__ uint8_t F[NQ][64 * BW];  // node matched per depth, per thread of the workgroup
+  const uint32_t lane = lane_id();
+  const uint32_t tid = threadIdx.x;
+  unsigned long long* const cursor = p.ctl + 2 * kBigClasses + cls;
+  uint64_t off = 0;
+  for (int c = 0; c < cls; ++c) off += p.ctl[c];
+  const uint64_t T = p.ctl[cls];
+  const uint64_t* L = p.list + off;
+  // this lane's stash: the domains [SD][NW], then the pushed R [SD][NW] (SD >= the batch's
+  // largest query)
+  uint64_t* const DS = p.stash + (uint64_t(blockIdx.x) * 64 * BW + tid) * (2 * p.SD * NW);
+  uint64_t* const RSt = DS + p.SD * NW;
+  bool busy = false, exhausted = false;
+  uint64_t t = 0, c = 0, nodes = 0, sum = 0;
+  uint64_t R[NW], X[NW], used[NW];
+  uint32_t s = 0, q = 0, nq = 0;
+  int depth = 0;
+  uint64_t next = 0, end = 0;
+  constexpr uint64_t kClaim = 256;
+  constexpr int kHandout = 32;
+  // X of depth d: the domain of depth d + 1 minus `used`, constrained by the nodes matched at
+  // depths < d: their rows are loaded four at a time, except the row of depth d - 1 when the
+  // caller has it (last_row)
+  auto build_x = [&](int d, const uint64_t* last_row, uint64_t* out) {
+    const uint32_t am = p.qa[size_t(q) * 32 + d + 1];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) out[k] = DS[(d + 1) * NW + k] & ~used[k];
+    const int lim = last_row ? d - 1 : d;
+    for (int i0 = 0; i0 < lim; i0 += 4) {
+      uint64_t r4[4][NW];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (i0 + j < lim) {
+          const uint32_t fi = F[i0 + j][tid];
+#pragma unroll
+          for (int k = 0; k < NW; ++k) r4[j][k] = p.rows[uint64_t(s + fi) * p.RW + k];
+        }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (i0 + j < lim) {
+          const bool a = (am >> (i0 + j)) & 1;
+#pragma unroll
+          for (int k = 0; k < NW; ++k) out[k] &= a ? r4[j][k] : ~r4[j][k];
+        }
+    }
+    if (last_row) {
+      const bool a = (am >> (d - 1)) & 1;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) out[k] &= a ? last_row[k] : ~last_row[k];
+    }
+  };
+  while (true) {
+    const uint64_t idle = ballot(!busy);
+    if (popc64(idle) >= kHandout || idle == ~uint64_t(0)) {
+      if (next >= end && !exhausted) {
+        uint64_t base = 0;
+        if (lane == 0) base = atomicAdd(cursor, (unsigned long long)kClaim);
+        base = uniform64(readlane64(base, 0));
+        next = base;
+        end = base >= T ? base : min(base + kClaim, T);
+        if (base + kClaim >= T) exhausted = true;
+      }
+      const uint64_t take = min(end - next, uint64_t(popc64(idle)));
+      const uint64_t rank = uint64_t(popc64(idle & low_mask(lane)));
+      if (!busy && rank < take) {
+        busy = true;
+        const uint64_t ent = L[next + rank];
+        t = ent & 0xffffffffu;
+        const uint32_t g = uint32_t(ent >> 32);
+        q = p.gidx[t];
+        s = p.dgo[g];
+        const uint32_t e = p.dgo[g + 1], qb = p.qgo[q];
+        nq = p.qgo[q + 1] - qb;
+        // every depth's domain into the stash, four depths' loads in flight at a time
+        for (uint32_t i0 = 0; i0 < nq; i0 += 4) {
+          uint64_t d4[4][NW];
+#pragma unroll
+          for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t i = i0 + j;
+            if (i < nq) {
+              const uint32_t u = p.jorder ? p.jorder[size_t(q) * 32 + i] : i;
+              const uint64_t* row = p.cand + uint64_t(qb + u) * p.W;
+#pragma unroll
+              for (int k = 0; k < NW; ++k) d4[j][k] = row_bits(row, s, e, k);
+            }
+          }
+#pragma unroll
+          for (uint32_t j = 0; j < 4; ++j)
+            if (i0 + j < nq) {
+#pragma unroll
+              for (int k = 0; k < NW; ++k) DS[(i0 + j) * NW + k] = d4[j][k];
+            }
+          if (i0 == 0) {
+#pragma unroll
+            for (int k = 0; k < NW; ++k) {
+              R[k] = d4[0][k];
+              X[k] = d4[1][k];  // nq >= 2: single-node queries are never GMCR pairs
+            }
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < NW; ++k) used[k] = 0;
+        depth = 0;
+        c = 0;
+        nodes = 0;
+      }
+      next += take;
+    }
+    if (!ballot(busy)) {
+      if (exhausted && next >= end) break;
+      continue;
+    }
+    bool finished = false;
+    if (busy) {
+      uint64_t any = 0;
+#pragma unroll
+      for (int k = 0; k < NW; ++k) any |= R[k];
+      if (!any) {
+        if (depth == 0) {
+          finished = true;
+        } else {
+          --depth;
+#pragma unroll
+          for (int k = 0; k < NW; ++k) R[k] = RSt[depth * NW + k];
+          const uint32_t f = F[depth][tid];
+          used[f >> 6] &= ~(uint64_t(1) << (f & 63));
+          build_x(depth, nullptr, X);
+        }
+      } else {
+        // lowest remaining candidate v
+        uint32_t v = 0;
+        bool found = false;
+#pragma unroll
+        for (int k = 0; k < NW; ++k)
+          if (!found && R[k]) {
+            v = uint32_t(64 * k + ctz64(R[k]));
+            R[k] &= R[k] - 1;
+            found = true;
+          }
+        ++nodes;
+        uint64_t nv[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) nv[k] = p.rows[uint64_t(s + v) * p.RW + k];
+        const int nd = depth + 1;
+        const bool adj = (p.qa[size_t(q) * 32 + nd] >> depth) & 1;
+        uint64_t Y[NW], ny = 0;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+          Y[k] = X[k] & (adj ? nv[k] : ~nv[k]);
+          if (k == int(v >> 6)) Y[k] &= ~(uint64_t(1) << (v & 63));
+          ny |= Y[k];
+        }
+        if (uint32_t(nd) == nq - 1) {
+#pragma unroll
+          for (int k = 0; k < NW; ++k) c += uint64_t(popc64(Y[k]));
+          if (p.find_first && c) finished = true;
+        } else if (ny) {
+          // descend: push R, then X of the new depth (v is matched at `depth` now)
+#pragma unroll
+          for (int k = 0; k < NW; ++k) RSt[depth * NW + k] = R[k];
+          F[depth][tid] = uint8_t(v);
+          used[v >> 6] |= uint64_t(1) << (v & 63);
+          build_x(nd, nv, X);
+#pragma unroll
+          for (int k = 0; k < NW; ++k) R[k] = Y[k];
+          depth = nd;
+        }
+      }
+    }
+    if (finished) {
+      const uint64_t cc = p.find_first ? (c ? 1 : 0) : c;
+      if (p.per_pair) p.per_pair[t] = cc;
+      if (p.per_pair_nodes) p.per_pair_nodes[t] = nodes;
+      sum += cc;
+      busy = false;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+  if (lane == 0 && sum) atomicAdd(reinterpret_cast<unsigned long long*>(p.total), (unsigned long long)sum);
+}
+
 
 // Adjacency rows of a batch whose graphs have at most 64 nodes: bit y of row x = local node y is a
 // neighbour of local node x (one thread per node).
@@ -647,7 +816,7 @@ __global__ void join_depth_adjacency_kernel(const uint32_t* __restrict__ qgo,
 }
 
 // The staged join for data graphs of up to 64 nodes (JoinCandidates3Optimized,
-// isomorphism.hpp:309-468, same results as join_kernel<1>): persistent waves take GMCR entries
+// isomorphism.hpp:309-468, standard backtracking): persistent waves take GMCR entries
 // 256 at a time from a global cursor and hand them to idle lanes; a lane's search keeps its
 // per-depth domains (from the candidate rows), remaining candidates and matched nodes' neighbour
 // rows in LDS and reads one precomputed adjacency row from global memory per step.
@@ -1183,16 +1352,49 @@ int staged_reset(bms2_signature sig, bms2_candidates cand) {
   return BMS2_OK;
 }
 
-uint64_t staged_join_scratch_bytes(const bms2_device_csr* query, const bms2_device_csr* data) {
-  if (data->max_graph_nodes > 64) return 0;
-  // one adjacency row per data node, the entry cursor, 32 depth-adjacency words per query
-  return 8 * (std::max<uint64_t>(1, data->total_nodes) + 1) + 4 * 32 * uint64_t(query->num_graphs);
+namespace {
+
+int device_cus() {
+  int dev = 0, cus = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  return cus;
+}
+
+// Depths of a lane's stash in the joins of graphs above 64 nodes: the batch's largest query.
+uint32_t stash_depth(const bms2_device_csr* query) {
+  return std::max<uint32_t>(2, std::min<uint32_t>(query->max_graph_nodes, kBigDepth));
+}
+
+// Bytes of the staged join's scratch; for graphs above 64 nodes with `waves` waves per CU in
+// the size-class joins (their stashes: 2 x stash depth x row words x 8 bytes per lane).
+uint64_t join_scratch_bytes(const bms2_device_csr* query, const bms2_device_csr* data, uint64_t T,
+                            uint32_t waves, int cus, uint32_t rw_above = 0) {
+  const uint64_t nrows = std::max<uint64_t>(1, data->total_nodes);
+  const uint64_t qa = 4 * 32 * uint64_t(query->num_graphs);
+  // graphs of up to 64 nodes: one adjacency row per data node, the entry cursor, 32
+  // depth-adjacency words per query
+  if (data->max_graph_nodes <= 64) return 8 * (nrows + 1) + qa;
+  // larger graphs: rows of 2 or 4 words per node, the depth adjacency, class counters and
+  // cursors, the GMCR entries sorted by size class (8 bytes each), the lanes' stashes
+  const uint64_t rw = rw_above ? rw_above : data->max_graph_nodes > 128 ? 4 : 2;
+  const uint64_t stash = uint64_t(cus) * waves * 64 * 2 * stash_depth(query) * rw * 8;
+  return 8 * rw * nrows + (qa + 7) / 8 * 8 + 8 * 3 * kBigClasses + 8 * std::max<uint64_t>(T, 1) +
+         stash;
+}
+
+}  // namespace
+
+uint64_t staged_join_scratch_bytes(const bms2_device_csr* query, const bms2_device_csr* data,
+                                   uint64_t T) {
+  return join_scratch_bytes(query, data, T, kBigWavesPerCU, device_cus());
 }
 
 int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
                 bms2_candidates cand, bms2_gmcr gmcr, int find_first, const uint8_t* d_jorder,
                 uint64_t* d_total, uint64_t* d_pair_counts, uint64_t* d_pair_nodes,
-                void* scratch) {
+                void* scratch, uint64_t scratch_bytes, const BigJoinShape* shape) {
   const uint64_t T = gmcr->total;
   if (!T) return BMS2_OK;
   if (data->max_graph_nodes <= 64) {
@@ -1203,7 +1405,7 @@ int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_cs
     const uint64_t nqa = uint64_t(query->num_graphs) * 32;
     uint64_t* rows = static_cast<uint64_t*>(scratch);
     if (!rows)
-      if (int rc = queue_scratch(q, staged_join_scratch_bytes(query, data), (void**)&rows)) return rc;
+      if (int rc = queue_scratch(q, staged_join_scratch_bytes(query, data, T), (void**)&rows)) return rc;
     auto* cursor = reinterpret_cast<unsigned long long*>(rows + nrows);
     auto* qa = reinterpret_cast<uint32_t*>(rows + nrows + 1);
     if (nqa)
@@ -1240,12 +1442,64 @@ int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_cs
     if (e != hipSuccess) return hip_fail(e, "join_rows_kernel", __FILE__, __LINE__);
     return BMS2_OK;
   }
-  hipLaunchKernelGGL(join_kernel<4>, dim3(grid_for(T, 256, 1u << 30)), dim3(256), 0, q->stream,
-                     cand->bits, cand->W, query->graph_offsets, query->row_offsets,
-                     query->column_indices, data->graph_offsets, data->num_graphs, data->row_offsets,
-                     data->column_indices, gmcr->offsets, gmcr->indices, T, find_first, d_total,
-                     d_pair_counts, d_jorder, d_pair_nodes);
-  BMS2_CHECK_LAUNCH("join_kernel");
+  // graphs of 65-256 nodes: multi-word adjacency rows, the GMCR entries sorted into size classes
+  // by row width, one persistent join per class present
+  hipStream_t st = q->stream;
+  const int cus = device_cus();
+  const BigJoinShape sh = shape ? *shape : BigJoinShape{};
+  const uint64_t nrows = std::max<uint64_t>(1, data->total_nodes);
+  const uint32_t wide_above = sh.row_words == 4 ? 64 : 128;
+  const uint32_t RW = data->max_graph_nodes > wide_above ? 4 : 2;
+  const uint64_t nqa = uint64_t(query->num_graphs) * 32;
+  const uint32_t bw = sh.block_waves == 4 ? 4 : sh.block_waves == 2 ? 2 : 1;
+  uint32_t waves = std::max<uint32_t>(bw, std::min<uint32_t>(32, sh.waves ? sh.waves : kBigWavesPerCU)) / bw * bw;
+  uint64_t* rows = static_cast<uint64_t*>(scratch);
+  if (!rows) {
+    if (int rc = queue_scratch(q, join_scratch_bytes(query, data, T, waves, cus, RW), (void**)&rows))
+      return rc;
+  } else {
+    // a caller's scratch of scratch_bytes: as many waves per CU as their stashes fit
+    while (waves > bw && join_scratch_bytes(query, data, T, waves, cus, RW) > scratch_bytes) waves -= bw;
+    if (join_scratch_bytes(query, data, T, waves, cus, RW) > scratch_bytes)
+      return set_error(BMS2_ERR_CAPACITY, "join scratch of %llu bytes is too small",
+                       (unsigned long long)scratch_bytes);
+  }
+  auto* qa = reinterpret_cast<uint32_t*>(rows + RW * nrows);
+  auto* ctl = reinterpret_cast<unsigned long long*>(rows + RW * nrows + (nqa + 1) / 2);
+  uint64_t* list = reinterpret_cast<uint64_t*>(ctl + 3 * kBigClasses);
+  uint64_t* stash = list + std::max<uint64_t>(T, 1);
+  const uint32_t SD = stash_depth(query);
+  if (nqa)
+    hipLaunchKernelGGL(join_depth_adjacency_kernel, dim3(grid_for(nqa, 256)), dim3(256), 0, st,
+                       query->graph_offsets, query->row_offsets, query->column_indices,
+                       query->num_graphs, d_jorder, qa);
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<uint64_t*>(ctl),
+                     uint64_t(3 * kBigClasses));
+  hipLaunchKernelGGL(adj_rows_wide_kernel, dim3(grid_for(data->total_nodes, 256, 1u << 30)),
+                     dim3(256), 0, st, data->graph_offsets, data->num_graphs, data->row_offsets,
+                     data->column_indices, data->total_nodes, RW, rows);
+  for (int pass = 0; pass < 2; ++pass)
+    hipLaunchKernelGGL(big_classify_kernel,
+                       dim3(grid_for(uint64_t(data->num_graphs) * 64, 256, 1u << 30)), dim3(256),
+                       0, st, data->graph_offsets, data->num_graphs, gmcr->offsets, pass,
+                       wide_above, ctl, list);
+  BMS2_CHECK_LAUNCH("big_classify_kernel");
+  BigJoinParams bp{cand->bits, cand->W, query->graph_offsets, qa, gmcr->indices,
+                   data->graph_offsets, rows, RW, list, ctl, stash, SD, find_first, d_total,
+                   d_pair_counts, d_jorder, d_pair_nodes};
+  const dim3 grid(uint32_t(cus) * waves / bw), block(64 * bw);
+  auto launch = [&](auto kern, int cls) { hipLaunchKernelGGL(kern, grid, block, 0, st, bp, cls); };
+  if (bw == 4) {
+    launch(join_big_kernel<2, 4>, 0);
+    if (RW == 4) launch(join_big_kernel<4, 4>, 1);
+  } else if (bw == 2) {
+    launch(join_big_kernel<2, 2>, 0);
+    if (RW == 4) launch(join_big_kernel<4, 2>, 1);
+  } else {
+    launch(join_big_kernel<2, 1>, 0);
+    if (RW == 4) launch(join_big_kernel<4, 1>, 1);
+  }
+  BMS2_CHECK_LAUNCH("join_big_kernel");
   return BMS2_OK;
 }
 
@@ -1740,7 +1994,7 @@ int bms2_join_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_
   EventScope es{q, ev};
   if (int rc = es.begin()) return rc;
   if (int rc = staged_join(q, query, data, cand, gmcr, find_first, nullptr, d_num_matches,
-                           d_pair_counts, nullptr, nullptr))
+                           d_pair_counts, nullptr, nullptr, 0, nullptr))
     return rc;
   return es.end();
 }
@@ -1792,17 +2046,20 @@ int bms2_join_candidates_noalloc(bms2_queue q, const bms2_device_csr* query,
   if (query->max_graph_nodes > BMS2_MAX_QUERY_NODES)
     return set_error(BMS2_ERR_UNSUPPORTED, "query with %u nodes (limit %u, isomorphism.hpp:248)",
                      query->max_graph_nodes, BMS2_MAX_QUERY_NODES);
-  const uint64_t need = staged_join_scratch_bytes(query, data);
+  const uint64_t need = join_scratch_bytes(query, data, gmcr->total, 1, device_cus());
   if (need > kNoAllocScratchBytes)
     return set_error(BMS2_ERR_CAPACITY,
-                     "the join needs %llu bytes of scratch for %llu data nodes and %u queries; the "
-                     "allocation-free join holds %llu (8 bytes per data node + 128 per query)",
+                     "the join needs %llu bytes of scratch for %llu data nodes, %u queries and %llu "
+                     "GMCR pairs; the allocation-free join holds %llu (8 bytes per data node + 128 "
+                     "per query for graphs of up to 64 nodes; 16 or 32 bytes per data node + 128 "
+                     "per query + 8 per GMCR pair above)",
                      (unsigned long long)need, (unsigned long long)data->total_nodes,
-                     query->num_graphs, (unsigned long long)kNoAllocScratchBytes);
+                     query->num_graphs, (unsigned long long)gmcr->total,
+                     (unsigned long long)kNoAllocScratchBytes);
   void* scratch = nullptr;
   if (int rc = noalloc_scratch(q->device, &scratch)) return rc;
   return staged_join(q, query, data, cand, gmcr, find_first, nullptr, d_num_matches,
-                     d_pair_counts, nullptr, scratch);
+                     d_pair_counts, nullptr, scratch, kNoAllocScratchBytes, nullptr);
 }
 
 // ---- GPU-side CSR packing

@@ -112,9 +112,12 @@ int bms2_queue_create(int device, bms2_queue* out);
 /* A queue over an existing hipStream_t (not owned). */
 int bms2_queue_wrap(int device, void* hip_stream, bms2_queue* out);
 /* Releases the queue (and its stream when owned).  A queue keeps a grow-only scratch buffer for
- * the filter, refinement and join calls until it is destroyed: max(64 B per query node, 8 B per
- * data node + 128 B per query graph) of the largest batch they ran on, plus an eighth, at least
- * 1 MiB.
+ * the filter, refinement and join calls until it is destroyed, sized for the largest batch they
+ * ran on plus an eighth, at least 1 MiB: the filter's 64 B per query node, the refinement's 8 B
+ * per data node, the join's 8 B per data node + 128 B per query graph (graphs of up to 64 nodes)
+ * or, above 64 nodes, 16 B per data node (32 B above 128 nodes) + 128 B per query graph + 8 B
+ * per GMCR pair + the join lanes' stashes (16 B per query depth and row word for each of 12
+ * waves per CU: about 100 MB for queries of up to 16 nodes and rows of 2 words).
  * A queue is used from one host thread at a time (like one stream's launches, calls from two
  * threads would interleave and share the scratch buffer); use a queue per thread. */
 int bms2_queue_destroy(bms2_queue q);
@@ -289,9 +292,10 @@ int bms2_join_candidates(bms2_queue q, const bms2_device_csr* query, const bms2_
 
 /* No counterpart in the reference.  For callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): bms2_join_candidates
- * with no event and no allocation of any kind.  The join's scratch (one adjacency row per data
- * node, a work cursor, the queries' depth adjacency: 8 bytes per data node + 128 per query graph
- * + 8, needed only when every data graph has at most 64 nodes) lives in a static device array of
+ * with no event and no allocation of any kind.  The join's scratch (adjacency rows of the data
+ * nodes, the queries' depth adjacency, work cursors: 8 bytes per data node + 128 per query graph
+ * when every data graph has at most 64 nodes; otherwise 16 bytes per data node (32 above 128
+ * nodes) + 128 per query graph + 8 per GMCR pair) lives in a static device array of
  * BMS2_JOIN_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the process
  * ends; a batch that needs more fails with BMS2_ERR_CAPACITY.  Every call on one GPU shares that
  * GPU's array: calls that overlap on one GPU (from two streams or two host threads) race. */
@@ -377,11 +381,16 @@ typedef struct bms2_pipeline_tuning {
   uint32_t slots;       /* pack-adjacency slots per small-join wave (8 or 16) */
   uint32_t candidates;  /* configurations timed */
   float best_ms;        /* fastest kernel span of one run */
+  /* joins of the data graphs above 64 nodes (tuned when the batch has any) */
+  uint32_t big_waves;       /* waves per CU */
+  uint32_t big_block_waves; /* waves per workgroup: 1, 2 or 4 */
+  uint32_t big_row_words;   /* u64 words per adjacency row of graphs of 65-128 nodes: 2 or 4 */
 } bms2_pipeline_tuning;
 
 /* Times whole runs of the resident batch under a set of launch shapes (waves per CU of each join,
- * pack-adjacency slots per small-join wave), `runs` timed runs each after one untimed run, and
- * keeps the fastest for later runs.  The outputs hold the last run's results (identical under
+ * pack-adjacency slots per small-join wave; for batches with graphs above 64 nodes also the waves
+ * per CU, waves per workgroup and adjacency-row words of their joins), `runs` timed runs each
+ * after one untimed run, and keeps the fastest for later runs.  The outputs hold the last run's results (identical under
  * every shape).  Refinement depth is not tuned: it changes the reference's results (shells at
  * distance >= 3 can remove true embeddings).  No counterpart in the reference. */
 int bms2_pipeline_autotune(bms2_pipeline p, uint64_t* d_counts, uint64_t* d_valid, uint32_t runs,

@@ -69,7 +69,8 @@ def test_noalloc_join_equals_join(queue, molecules_2k, find_first):
 
 
 def test_noalloc_join_large_graphs_and_queries(queue):
-    """Data graphs above 64 nodes need no scratch; queries of 17-32 nodes take the 32-deep join."""
+    """Data graphs above 64 nodes take the size-classed join (multi-word rows and sorted entries in
+    the array); queries of 17-32 nodes the 32-deep instances."""
     data, queries = graphs.generate("uneven-1m", seed=0, num_data_graphs=600)
     print("noalloc join: uneven 600 graphs", flush=True)
     dq, dd, cand, gm = _to_gmcr(queue, queries, data, 2)
