@@ -234,6 +234,41 @@ def ingest_bench(args) -> int:
     return 0 if same else 1
 
 
+def staged_sequence(q, dq, dd, K: int, find_first: bool = False):
+    """One run of the stage sequence of BMS2.cpp:131-234 through the staged entry points, on fresh
+    stage objects (a Signature keeps the node sets its refinement levels counted,
+    signature.hpp:340-385).  Returns (per-stage device ms, GMCR, per-pair counts, total buffer,
+    candidate matrix bytes); the caller waits on the queue."""
+    from bms2_amd import api
+    sig = api.Signature(q, dd.total_nodes, dq.total_nodes)
+    path = api.PathSignature(q, dd.total_nodes, dq.total_nodes)
+    cyc = api.CycleSignature(q, dd.total_nodes, dq.total_nodes)
+    cand = api.Candidates(q, dq.total_nodes, dd.total_nodes)
+    gm = api.GMCR(q)
+    total = api.DeviceBuffer(q, 8, zero=True)
+    names = ("signatures", "filter", "refine signatures", "refine candidates", "gmcr", "join")
+    t = dict.fromkeys(names, 0.0)
+    evs = [("signatures", sig.generate_data_signatures(dd, event=True)),
+           ("signatures", path.generate_data_path_signatures(dd, event=True)),
+           ("signatures", cyc.generate_data_cycle_signatures(dd, event=True)),
+           ("signatures", sig.generate_query_signatures(dq, event=True)),
+           ("signatures", path.generate_query_path_signatures(dq, event=True)),
+           ("signatures", cyc.generate_query_cycle_signatures(dq, event=True)),
+           ("filter", api.filter_candidates(q, dq, dd, sig, path, cyc, cand, event=True))]
+    for k in range(1, K + 1):
+        evs.append(("refine signatures", sig.refine_data_signatures(dd, k, event=True)))
+        evs.append(("refine signatures", sig.refine_query_signatures(dq, k, event=True)))
+        evs.append(("refine candidates", api.refine_candidates(q, dq, dd, sig, cand, event=True)))
+    evs.append(("gmcr", gm.generate_gmcr(dq, dd, cand, event=True)))
+    per_pair = api.DeviceBuffer(q, 8 * max(gm.total, 1), zero=True)
+    evs.append(("join", api.join_candidates(q, dq, dd, cand, gm, total, find_first, per_pair,
+                                            event=True)))
+    q.wait()
+    for n, e in evs:
+        t[n] += e.elapsed_ms()
+    return t, gm, per_pair, total, dq.total_nodes * cand.W * 8
+
+
 def staged_bench(args) -> int:
     """The stage sequence of BMS2.cpp:131-234 through the staged entry points (the reference's API
     boundary, one C-ABI call per stage) on a shard of the workload, count-all, identity order.
@@ -252,40 +287,10 @@ def staged_bench(args) -> int:
     names = ("signatures", "filter", "refine signatures", "refine candidates", "gmcr", "join")
     per = {n: [] for n in names}
     for it in range(args.warmup + args.steps):
-        # fresh stage objects every run: a Signature accumulates the reachability of its
-        # refinement levels (signature.hpp:340-385), so a second sequence on the same objects
-        # would refine nothing, and the per-pair counts are sized by this run's GMCR
-        sig = api.Signature(q, dd.total_nodes, dq.total_nodes)
-        path = api.PathSignature(q, dd.total_nodes, dq.total_nodes)
-        cyc = api.CycleSignature(q, dd.total_nodes, dq.total_nodes)
-        cand = api.Candidates(q, dq.total_nodes, dd.total_nodes)
-        gm = api.GMCR(q)
-        total = api.DeviceBuffer(q, 8, zero=True)
-        q.wait()
-        t = dict.fromkeys(names, 0.0)
-        evs = [("signatures", sig.generate_data_signatures(dd, event=True)),
-               ("signatures", path.generate_data_path_signatures(dd, event=True)),
-               ("signatures", cyc.generate_data_cycle_signatures(dd, event=True)),
-               ("signatures", sig.generate_query_signatures(dq, event=True)),
-               ("signatures", path.generate_query_path_signatures(dq, event=True)),
-               ("signatures", cyc.generate_query_cycle_signatures(dq, event=True)),
-               ("filter", api.filter_candidates(q, dq, dd, sig, path, cyc, cand, event=True))]
-        for k in range(1, K + 1):
-            evs.append(("refine signatures", sig.refine_data_signatures(dd, k, event=True)))
-            evs.append(("refine signatures", sig.refine_query_signatures(dq, k, event=True)))
-            evs.append(("refine candidates", api.refine_candidates(q, dq, dd, sig, cand, event=True)))
-        evs.append(("gmcr", gm.generate_gmcr(dq, dd, cand, event=True)))
-        per_pair = api.DeviceBuffer(q, 8 * max(gm.total, 1), zero=True)
-        evs.append(("join", api.join_candidates(q, dq, dd, cand, gm, total, False, per_pair,
-                                                event=True)))
-        q.wait()
-        for n, e in evs:
-            t[n] += e.elapsed_ms()
+        t, gm, per_pair, total, matrix = staged_sequence(q, dq, dd, K)
         if it >= args.warmup:
             for n in names:
                 per[n].append(t[n])
-        if it + 1 < args.warmup + args.steps:
-            del evs, sig, path, cyc, cand, gm, total, per_pair
     ms = {n: float(np.mean(v)) for n, v in per.items()}
     run_ms = sum(ms.values())
     # per-pair counts against the fused pipeline on the same shard
@@ -296,7 +301,6 @@ def staged_bench(args) -> int:
     staged[rows, idx] = counts
     fused, _, _ = api.run_pipeline(q, queries, data, K)
     same = bool(np.array_equal(staged, fused))
-    matrix = dq.total_nodes * cand.W * 8  # candidate bit matrix bytes
     filt_gbs = matrix / (ms["filter"] / 1e3) / 1e9
     ref_gbs = 2 * K * matrix / (ms["refine candidates"] / 1e3) / 1e9 if K else 0.0
     pairs = data.num_graphs * queries.num_graphs
@@ -349,6 +353,9 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--skip-plain", action="store_true",
                     help="time the learned leg only (profiling runs)")
+    ap.add_argument("--no-staged", action="store_true",
+                    help="skip the staged leg (the reference's stage calls at full scale)")
+    ap.add_argument("--staged-steps", type=int, default=2)
     ap.add_argument("--plan-graphs", type=int, default=16384,
                     help="data graphs traced to plan the learned order")
     ap.add_argument("--ingest", action="store_true",
@@ -444,6 +451,37 @@ def main() -> int:
     value = pairs * args.steps / t_total
     plain_value = pairs * args.steps / t_plain if plain_sample is not None else None
 
+    # the reference-structured path: the stage sequence through the staged entry points on the
+    # same resident batch (one warm-up run, then timed runs); per-pair counts of the leading
+    # graphs and the total must equal the fused pipeline's
+    staged_info = None
+    if not args.no_staged:
+        st_ms, st_stage = [], None
+        for it in range(1 + args.staged_steps):
+            t_st, gm_s, pp_s, tot_s, _ = staged_sequence(q, dq, dd, K, args.find_first)
+            if it:
+                st_ms.append(sum(t_st.values()))
+                st_stage = t_st
+        off_s, idx_s = gm_s.host()
+        S2 = min(S, D, 20000)
+        n2 = int(off_s[S2])
+        dense = np.zeros((S2, Q), np.uint64)
+        rows_s = np.repeat(np.arange(S2), np.diff(off_s[:S2 + 1].astype(np.int64)))
+        dense[rows_s, idx_s[:n2]] = pp_s.to_numpy(np.uint64, n2)
+        fused_rows, _ = pipe.results(counts, valid, rows=S2)
+        st_total = int(tot_s.to_numpy(np.uint64)[0])
+        st_same = bool(np.array_equal(dense, fused_rows) and
+                       st_total == int(st[1] if args.find_first else st[2]))
+        del gm_s, pp_s, tot_s, dense, fused_rows
+        st_avg = float(np.mean(st_ms))
+        staged_info = {"value": round(pairs / (st_avg / 1e3), 1), "ms_per_step": round(st_avg, 3),
+                       "stage_ms": {k: round(v, 3) for k, v in st_stage.items()},
+                       "steps": args.staged_steps, "counts_equal_fused": st_same}
+        print(f"[bench] staged entry points (reference stage order): {st_avg:.2f} ms/run, "
+              f"counts equal: {st_same}", file=sys.stderr, flush=True)
+        if not st_same:
+            return 1
+
     abytes = algorithmic_bytes(data, queries)
     k_avg_s = float(np.mean(kernel_ms)) / 1000.0
     achieved = abytes / k_avg_s / 1e9
@@ -518,6 +556,8 @@ def main() -> int:
             "break_even_runs": None if plain_value is None or value <= plain_value else round(
                 plan_info["plan_ms"] / (1000 * t_plain / args.steps - ms_per_step), 1)},
         "learned_plan": plan_info,
+        "staged": staged_info,
+        "learned_vs_staged": None if staged_info is None else round(value / staged_info["value"], 4),
         "autotune": tune,
         "roofline": roofline,
         "cpu_baseline": cpu,
