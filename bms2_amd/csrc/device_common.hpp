@@ -93,6 +93,11 @@ __host__ __device__ inline uint64_t low_mask(uint32_t n) {
   return n >= 64 ? ~uint64_t(0) : ((uint64_t(1) << n) - 1);
 }
 
+// Size classes of the data graphs of 65-256 nodes in their joins: adjacency rows and domains of
+// 2 u64 words up to `wide_above` nodes (128, or 64 to give every graph 4 words), 4 words above.
+constexpr int kBigClasses = 2;
+__host__ __device__ inline int big_class(uint32_t n, uint32_t wide_above) { return n > wide_above ? 1 : 0; }
+
 // getGraphID (graph.hpp:96-107): graph holding a global node id.
 __device__ inline uint32_t graph_of(const uint32_t* graph_offsets, uint32_t num_graphs,
                                     uint32_t node) {

@@ -987,57 +987,14 @@ DataPacks build_data_packs(const std::vector<uint32_t>& go) {
   return d;
 }
 
-// Data graphs above 64 nodes (staged path): zero their count and GMCR rows.
+// Data graphs above 64 nodes: zero their count rows (the joins add into them).
 __global__ void big_zero_kernel(const uint32_t* __restrict__ big_gid, uint32_t nbig, uint32_t Q,
-                                uint32_t VW, uint32_t g_limit, uint64_t* __restrict__ counts,
-                                uint64_t* __restrict__ valid) {
+                                uint32_t g_limit, uint64_t* __restrict__ counts) {
   const uint32_t b = blockIdx.x;
   if (b >= nbig) return;
   const uint32_t g = big_gid[b];
   if (g >= g_limit) return;
   for (uint32_t q = threadIdx.x; q < Q; q += blockDim.x) counts[uint64_t(g) * Q + q] = 0;
-  for (uint32_t w = threadIdx.x; w < VW; w += blockDim.x) valid[uint64_t(g) * VW + w] = 0;
-}
-
-// GMCR entries of the big graphs into the dense outputs, with their statistics
-// {GMCR pairs, matched pairs, embeddings, DFS nodes}.
-__global__ void big_scatter_kernel(const uint32_t* __restrict__ goff, const uint32_t* __restrict__ gidx,
-                                   uint64_t T, const uint32_t* __restrict__ big_gid, uint32_t nbig,
-                                   const uint64_t* __restrict__ pc, const uint64_t* __restrict__ pn,
-                                   uint32_t Q, uint32_t VW, uint32_t g_limit,
-                                   uint64_t* __restrict__ counts, uint64_t* __restrict__ valid,
-                                   uint64_t* __restrict__ stats) {
-  const uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-  if (t < T) {
-    uint32_t lo = 0, hi = nbig + 1;  // big graph of entry t: last b with goff[b] <= t
-    while (lo < hi) {
-      const uint32_t mid = lo + (hi - lo) / 2;
-      if (t >= goff[mid]) lo = mid + 1; else hi = mid;
-    }
-    const uint32_t g = big_gid[lo - 1], q = gidx[t];
-    if (g < g_limit) {
-      counts[uint64_t(g) * Q + q] = pc[t];
-      atomicOr(reinterpret_cast<unsigned long long*>(valid + uint64_t(g) * VW + (q >> 6)),
-               (unsigned long long)(uint64_t(1) << (q & 63)));
-      s0 = 1;
-      s1 = pc[t] ? 1 : 0;
-      s2 = pc[t];
-      s3 = pn[t];
-    }
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    s0 += __shfl_xor(s0, off);
-    s1 += __shfl_xor(s1, off);
-    s2 += __shfl_xor(s2, off);
-    s3 += __shfl_xor(s3, off);
-  }
-  if (lane_id() == 0 && s0) {
-    atomicAdd(reinterpret_cast<unsigned long long*>(stats + 0), (unsigned long long)s0);
-    atomicAdd(reinterpret_cast<unsigned long long*>(stats + 1), (unsigned long long)s1);
-    atomicAdd(reinterpret_cast<unsigned long long*>(stats + 2), (unsigned long long)s2);
-    atomicAdd(reinterpret_cast<unsigned long long*>(stats + 3), (unsigned long long)s3);
-  }
 }
 
 template <class F>
@@ -1096,16 +1053,17 @@ struct bms2_pipeline_s {
   // data graphs above 64 nodes: their own compact batch through the staged kernels
   uint32_t num_big = 0;
   bms2_device_csr big{};
-  uint32_t* d_big_gid = nullptr;
-  bms2_signature big_sig = nullptr;
+  uint32_t* d_big_gid = nullptr;   // original id of each graph of the batch
+  // screen order: graphs of 65-128 nodes, then 129-256, each by original id (host: original ids)
+  std::vector<uint32_t> big_narrow_gid, big_wide_gid;
+  uint32_t* d_big_list = nullptr;   // their batch indices
   bms2_path_signature big_path = nullptr;
   bms2_cycle_signature big_cyc = nullptr;
-  bms2_candidates big_cand = nullptr;
-  bms2_gmcr big_gmcr = nullptr;
-  uint64_t big_cap = 0;
-  uint64_t* d_big_pc = nullptr;     // per GMCR entry counts
-  uint64_t* d_big_pn = nullptr;     // per GMCR entry DFS nodes
-  uint64_t* d_big_stats = nullptr;  // [0..3] statistics, [4] join total
+  uint64_t* d_big_lev = nullptr;    // [K][big nodes] shell levels
+  uint64_t* d_big_qlev = nullptr;   // [K][Qn]
+  uint32_t* d_big_qrec = nullptr;   // [Qn][kBigRecDwords] screening records
+  BigEntries be{};                  // the screen's GMCR entries and domain records
+  uint64_t* d_big_stats = nullptr;  // [1..3] statistics of the joins
   // the single-chunk kernels captured in a hipGraph, valid for these outputs and K
   hipGraphExec_t graph = nullptr;
   uint64_t* graph_counts = nullptr;
@@ -1114,6 +1072,32 @@ struct bms2_pipeline_s {
 };
 
 namespace {
+
+// Releases the big-graph screen's entry and record buffers (the control words stay).
+void free_big_entries(bms2_pipeline_s* p) {
+  for (void* b : {(void*)p->be.gidx, (void*)p->be.egraph, (void*)p->be.rec_off, (void*)p->be.recs})
+    (void)hipFree(b);
+  p->be.gidx = p->be.egraph = nullptr;
+  p->be.rec_off = p->be.recs = nullptr;
+  p->be.cap_e = p->be.cap_words = 0;
+}
+
+// Sizes the entry buffers for cap_e entries per class and the records for cap_words words.
+int ensure_big_entries(bms2_pipeline_s* p, uint64_t cap_e, uint64_t cap_words) {
+  if (p->be.gidx && p->be.cap_e >= cap_e && p->be.cap_words >= cap_words) return BMS2_OK;
+  cap_e = std::max(cap_e, p->be.cap_e);
+  cap_words = std::max(cap_words, p->be.cap_words);
+  BMS2_HIP(hipStreamSynchronize(p->q->stream));
+  free_big_entries(p);
+  const uint64_t ne = kBigClasses * cap_e;
+  if (int rc = device_alloc((void**)&p->be.gidx, 4 * ne)) return rc;
+  if (int rc = device_alloc((void**)&p->be.egraph, 4 * ne)) return rc;
+  if (int rc = device_alloc((void**)&p->be.rec_off, 8 * ne)) return rc;
+  if (int rc = device_alloc((void**)&p->be.recs, 8 * cap_words)) return rc;
+  p->be.cap_e = cap_e;
+  p->be.cap_words = cap_words;
+  return BMS2_OK;
+}
 
 void pipeline_free(bms2_pipeline p) {
   if (!p) return;
@@ -1134,14 +1118,15 @@ void pipeline_free(bms2_pipeline p) {
   (void)hipFree(p->d_ctl);
   if (p->graph) (void)hipGraphExecDestroy(p->graph);
   (void)hipFree(p->d_big_gid);
-  (void)hipFree(p->d_big_pc);
-  (void)hipFree(p->d_big_pn);
+  (void)hipFree(p->d_big_list);
+  (void)hipFree(p->d_big_lev);
+  (void)hipFree(p->d_big_qlev);
+  (void)hipFree(p->d_big_qrec);
+  free_big_entries(p);
+  (void)hipFree(p->be.ctl);
   (void)hipFree(p->d_big_stats);
-  if (p->big_sig) bms2_signature_destroy(p->big_sig);
   if (p->big_path) bms2_path_signature_destroy(p->big_path);
   if (p->big_cyc) bms2_cycle_signature_destroy(p->big_cyc);
-  if (p->big_cand) bms2_candidates_destroy(p->big_cand);
-  if (p->big_gmcr) bms2_gmcr_destroy(p->big_gmcr);
   if (p->big.graph_offsets) bms2_device_csr_destroy(p->q, &p->big);
   delete p;
 }
@@ -1191,14 +1176,33 @@ int build_big_batch(bms2_pipeline_s* p, const std::vector<uint32_t>& go) {
   h.node_labels = blab.data();
   h.edge_labels = belab.data();
   if (int rc = bms2_device_csr_create(p->q, &h, &p->big)) return rc;
+  // screen order: graphs of up to 128 nodes (two 64-node slots per lane), then the larger ones
+  std::vector<uint32_t> list;
+  p->big_narrow_gid.clear();
+  p->big_wide_gid.clear();
+  for (int wide = 0; wide < 2; ++wide)
+    for (uint32_t b = 0; b < p->num_big; ++b)
+      if ((bgo[b + 1] - bgo[b] > 128) == bool(wide)) {
+        list.push_back(b);
+        (wide ? p->big_wide_gid : p->big_narrow_gid).push_back(gid[b]);
+      }
+  const uint64_t K = std::max<uint32_t>(p->opt.refinement_steps, 1);
   if (int rc = device_alloc((void**)&p->d_big_gid, 4ull * gid.size())) return rc;
+  if (int rc = device_alloc((void**)&p->d_big_list, 4ull * list.size())) return rc;
+  if (int rc = device_alloc((void**)&p->d_big_lev, 8 * K * std::max<uint64_t>(h.total_nodes, 1))) return rc;
+  if (int rc = device_alloc((void**)&p->be.ctl, 8 * kBigCtlWords)) return rc;
   if (int rc = device_alloc((void**)&p->d_big_stats, 8 * 8)) return rc;
   BMS2_HIP(hipMemcpy(p->d_big_gid, gid.data(), 4ull * gid.size(), hipMemcpyHostToDevice));
+  BMS2_HIP(hipMemcpy(p->d_big_list, list.data(), 4ull * list.size(), hipMemcpyHostToDevice));
   return BMS2_OK;
 }
 
-// Runs the stage sequence on the big graphs and writes their rows of the dense outputs (graphs
-// with id below g_limit); adds their statistics to stats.
+// Screens and joins the graphs above 64 nodes with an original id below g_limit and writes their
+// rows of the dense outputs; adds their statistics to stats.  The signatures are the staged
+// kernels' (path signatures; ring counts, which stay zero on graphs above 64 nodes as in
+// signature.hpp:614) and the shell levels of every refinement step; the screen replaces the
+// filter, the refinement steps and the GMCR of the staged sequence (same domains on every GMCR
+// pair), the join is the staged join's kernel over the screen's entries.
 int run_big(bms2_pipeline_s* p, uint64_t* d_counts, uint64_t* d_valid, uint32_t g_limit,
             uint64_t stats[8]) {
   if (!p->num_big) return BMS2_OK;
@@ -1206,55 +1210,59 @@ int run_big(bms2_pipeline_s* p, uint64_t* d_counts, uint64_t* d_valid, uint32_t 
   hipStream_t s = q->stream;
   const bms2_device_csr* dq = &p->query;
   const bms2_device_csr* db = &p->big;
-  if (int rc = staged_reset(p->big_sig, p->big_cand)) return rc;
-  if (int rc = bms2_signature_generate(p->big_sig, db, BMS2_SCOPE_DATA, nullptr)) return rc;
+  const uint32_t K = p->opt.refinement_steps;
+  const uint32_t n_narrow = uint32_t(std::lower_bound(p->big_narrow_gid.begin(), p->big_narrow_gid.end(), g_limit) - p->big_narrow_gid.begin());
+  const uint32_t n_wide = uint32_t(std::lower_bound(p->big_wide_gid.begin(), p->big_wide_gid.end(), g_limit) - p->big_wide_gid.begin());
+  if (!n_narrow && !n_wide) return BMS2_OK;
   if (int rc = bms2_path_signature_generate(p->big_path, db, BMS2_SCOPE_DATA, nullptr)) return rc;
   if (int rc = bms2_cycle_signature_generate(p->big_cyc, db, BMS2_SCOPE_DATA, nullptr)) return rc;
-  if (int rc = bms2_signature_generate(p->big_sig, dq, BMS2_SCOPE_QUERY, nullptr)) return rc;
   if (int rc = bms2_path_signature_generate(p->big_path, dq, BMS2_SCOPE_QUERY, nullptr)) return rc;
   if (int rc = bms2_cycle_signature_generate(p->big_cyc, dq, BMS2_SCOPE_QUERY, nullptr)) return rc;
-  if (int rc = bms2_filter_candidates(q, dq, db, p->big_sig, p->big_path, p->big_cyc, p->big_cand,
-                                      nullptr))
+  if (int rc = big_levels(s, db, K, p->d_big_lev)) return rc;
+  if (int rc = big_levels(s, dq, K, p->d_big_qlev)) return rc;
+  if (int rc = big_query_records(s, dq, path_signature_bits(p->big_path, BMS2_SCOPE_QUERY),
+                                 cycle_signature_values(p->big_cyc, BMS2_SCOPE_QUERY), p->d_big_qlev,
+                                 K, p->d_qpos, p->d_big_qrec))
     return rc;
-  for (uint32_t k = 1; k <= p->opt.refinement_steps; ++k) {
-    if (int rc = bms2_signature_refine(p->big_sig, db, k, BMS2_SCOPE_DATA, nullptr)) return rc;
-    if (int rc = bms2_signature_refine(p->big_sig, dq, k, BMS2_SCOPE_QUERY, nullptr)) return rc;
-    if (int rc = bms2_refine_candidates(q, dq, db, p->big_sig, p->big_cand, nullptr)) return rc;
-  }
-  if (int rc = bms2_gmcr_generate(p->big_gmcr, dq, db, p->big_cand, nullptr)) return rc;
-  const uint64_t T = bms2_gmcr_total(p->big_gmcr);
-  if (T > p->big_cap) {
-    BMS2_HIP(hipStreamSynchronize(s));
-    (void)hipFree(p->d_big_pc);
-    (void)hipFree(p->d_big_pn);
-    p->d_big_pc = p->d_big_pn = nullptr;
-    p->big_cap = 0;
-    if (int rc = device_alloc((void**)&p->d_big_pc, 8 * T)) return rc;
-    if (int rc = device_alloc((void**)&p->d_big_pn, 8 * T)) return rc;
-    p->big_cap = T;
-  }
-  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_big_stats, 8u);
-  if (int rc = staged_join(q, dq, db, p->big_cand, p->big_gmcr, int(p->opt.find_first),
-                           p->d_jorder, p->d_big_stats + 4, p->d_big_pc, p->d_big_pn, nullptr, 0,
-                           &p->big_shape))
-    return rc;
-  const uint32_t Q = p->query.num_graphs, VW = (Q + 63) / 64;
+  const uint32_t wide_above = big_wide_above(p->big_shape);
+  const uint32_t Q = dq->num_graphs;
+  uint64_t ctl[kBigCtlWords];
   hipLaunchKernelGGL(big_zero_kernel, dim3(p->num_big), dim3(256), 0, s, p->d_big_gid, p->num_big,
-                     Q, VW, g_limit, d_counts, d_valid);
+                     Q, g_limit, d_counts);
   BMS2_CHECK_LAUNCH("big_zero_kernel");
-  if (T) {
-    const uint32_t* goff = nullptr;
-    const uint32_t* gidx = nullptr;
-    if (int rc = bms2_gmcr_device_ptrs(p->big_gmcr, &goff, &gidx)) return rc;
-    hipLaunchKernelGGL(big_scatter_kernel, dim3(uint32_t((T + 255) / 256)), dim3(256), 0, s, goff,
-                       gidx, T, p->d_big_gid, p->num_big, p->d_big_pc, p->d_big_pn, Q, VW, g_limit,
-                       d_counts, d_valid, p->d_big_stats);
-    BMS2_CHECK_LAUNCH("big_scatter_kernel");
+  // first sizes: 128 entries per graph, 48 record words per entry; then what a run measured
+  if (int rc = ensure_big_entries(p, std::max<uint64_t>(1024, 128ull * p->num_big),
+                                  std::max<uint64_t>(1024, 128ull * 48 * p->num_big)))
+    return rc;
+  for (int attempt = 0;; ++attempt) {
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, reinterpret_cast<uint64_t*>(p->be.ctl),
+                       uint32_t(kBigCtlWords));
+    BMS2_CHECK_LAUNCH("zero_kernel");
+    if (int rc = big_screen(s, db, path_signature_bits(p->big_path, BMS2_SCOPE_DATA),
+                            cycle_signature_values(p->big_cyc, BMS2_SCOPE_DATA), p->d_big_lev, K,
+                            p->d_big_list, n_narrow, p->d_big_list + p->big_narrow_gid.size(),
+                            n_wide, wide_above, Q, p->d_qpre, kPreDwords, p->d_big_qrec, p->be,
+                            p->d_big_gid, d_valid))
+      return rc;
+    BMS2_HIP(hipMemcpyAsync(ctl, p->be.ctl, sizeof(ctl), hipMemcpyDeviceToHost, s));
+    BMS2_HIP(hipStreamSynchronize(s));
+    if (!ctl[kBigCtlOverflow]) break;
+    if (attempt)
+      return set_error(BMS2_ERR_CAPACITY, "big-graph screen overflowed its measured buffers");
+    // the counters ran on past the capacities: size to them, with room for the next batch
+    const uint64_t need_e = std::max(ctl[0], ctl[1]), need_w = ctl[kBigCtlRecWords];
+    if (int rc = ensure_big_entries(p, need_e + need_e / 8 + 1024, need_w + need_w / 8 + 1024)) return rc;
   }
+  const uint64_t count[kBigClasses] = {ctl[0], ctl[1]};
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_big_stats, 8u);
+  if (int rc = big_join(q, dq, db, p->be, count, int(p->opt.find_first), p->d_jorder, d_counts,
+                        p->d_big_gid, p->d_big_stats, &p->big_shape))
+    return rc;
   uint64_t st[8];
   BMS2_HIP(hipMemcpyAsync(st, p->d_big_stats, sizeof(st), hipMemcpyDeviceToHost, s));
   BMS2_HIP(hipStreamSynchronize(s));
-  for (int k = 0; k < 4; ++k) stats[k] += st[k];
+  stats[0] += count[0] + count[1];
+  for (int k = 1; k < 4; ++k) stats[k] += st[k];
   return BMS2_OK;
 }
 
@@ -1324,22 +1332,17 @@ int check_queries(const bms2_device_csr* query) {
 void free_query_side(bms2_pipeline_s* p) {
   for (void* b : {(void*)p->d_qpacks, (void*)p->d_qhead, (void*)p->d_qpre, (void*)p->d_qrec,
                   (void*)p->d_qpos, (void*)p->d_jorder, (void*)p->d_qadj, (void*)p->d_qadj8,
-                  (void*)p->d_big_pc, (void*)p->d_big_pn})
+                  (void*)p->d_big_qlev, (void*)p->d_big_qrec})
     (void)hipFree(b);
   p->d_qpacks = p->d_qhead = p->d_qpre = p->d_qrec = p->d_qpos = p->d_qadj = nullptr;
   p->d_jorder = nullptr;
-  p->d_qadj8 = p->d_big_pc = p->d_big_pn = nullptr;
-  p->big_cap = 0;
-  if (p->big_sig) bms2_signature_destroy(p->big_sig);
+  p->d_qadj8 = nullptr;
+  p->d_big_qlev = nullptr;
+  p->d_big_qrec = nullptr;
   if (p->big_path) bms2_path_signature_destroy(p->big_path);
   if (p->big_cyc) bms2_cycle_signature_destroy(p->big_cyc);
-  if (p->big_cand) bms2_candidates_destroy(p->big_cand);
-  if (p->big_gmcr) bms2_gmcr_destroy(p->big_gmcr);
-  p->big_sig = nullptr;
   p->big_path = nullptr;
   p->big_cyc = nullptr;
-  p->big_cand = nullptr;
-  p->big_gmcr = nullptr;
   if (p->graph) (void)hipGraphExecDestroy(p->graph);
   p->graph = nullptr;
   p->dens_s = p->dens_seg = p->dens_l = 0.0;
@@ -1447,11 +1450,11 @@ int bind_queries(bms2_pipeline_s* p, const bms2_device_csr* query) {
   p->alloc_l_words = l_words;
   if (p->num_big) {
     const uint64_t dn = p->big.total_nodes;
-    if ((rc = bms2_signature_create(p->q, dn, Qn, &p->big_sig)) ||
-        (rc = bms2_path_signature_create(p->q, dn, Qn, &p->big_path)) ||
+    const uint64_t K = std::max<uint32_t>(p->opt.refinement_steps, 1);
+    if ((rc = bms2_path_signature_create(p->q, dn, Qn, &p->big_path)) ||
         (rc = bms2_cycle_signature_create(p->q, dn, Qn, &p->big_cyc)) ||
-        (rc = bms2_candidates_create(p->q, Qn, dn, &p->big_cand)) ||
-        (rc = bms2_gmcr_create(p->q, &p->big_gmcr)))
+        (rc = device_alloc((void**)&p->d_big_qlev, 8 * K * std::max<uint64_t>(Qn, 1))) ||
+        (rc = device_alloc((void**)&p->d_big_qrec, 4ull * kBigRecDwords * std::max<uint64_t>(Qn, 1))))
       return rc;
   }
   p->large_waves = p->opt.join_waves ? p->opt.join_waves : (p->nq_cap <= 16 ? 12 : 6);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common.hpp"
+#include "device_common.hpp"
 
 struct bms2_queue_s {
   int device;
@@ -81,5 +82,60 @@ int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_cs
 // Zeroes a Signature's label signatures and reachability state and a Candidates matrix (fill
 // kernels on the signature's queue), so the staged sequence can run again on the same objects.
 int staged_reset(bms2_signature sig, bms2_candidates cand);
+
+// Device arrays of the staged signature objects (scope BMS2_SCOPE_DATA or BMS2_SCOPE_QUERY):
+// path signatures [n][4], ring counts [n].
+const uint64_t* path_signature_bits(bms2_path_signature s, int scope);
+const uint64_t* cycle_signature_values(bms2_cycle_signature s, int scope);
+
+// ---- screen and join of the data graphs of 65-256 nodes in the fused pipeline (big.hip)
+
+constexpr int kBigMaxLevels = 8;   // refinement steps of the screen's shell levels
+constexpr int kBigRecDwords = 48;  // screening record of one query node: 16 + 4 x kBigMaxLevels
+// Control words (u64) shared by the screen and the join: [c] entries of size class c,
+// [kBigClasses + c] unused, [2 kBigClasses + c] join cursors, then record words and overflow.
+constexpr int kBigCtlRecWords = 3 * kBigClasses;
+constexpr int kBigCtlOverflow = kBigCtlRecWords + 1;
+constexpr int kBigCtlWords = kBigCtlOverflow + 1;
+
+// GMCR entries of the big-graph screen: entry c cap_e + i is the i-th pair of size class c.
+struct BigEntries {
+  uint32_t* gidx;            // query of each entry
+  uint32_t* egraph;          // data graph (index in the batch of big graphs) of each entry
+  uint64_t* rec_off;         // first word of the entry's domains in recs
+  uint64_t cap_e;            // entries per class
+  uint64_t* recs;            // domains: the class's row words per query node, by local id
+  uint64_t cap_words;
+  unsigned long long* ctl;   // kBigCtlWords control words
+};
+
+// Shell signatures of levels 1..K of every node of g into out [K][total_nodes].
+int big_levels(hipStream_t st, const bms2_device_csr* g, uint32_t K, uint64_t* out);
+// Screening records (kBigRecDwords per query node, at slot qpos[u]) from the query batch's path
+// signatures, ring counts and shell levels [K][Qn].
+int big_query_records(hipStream_t st, const bms2_device_csr* query, const uint64_t* qpath,
+                      const uint64_t* qcyc, const uint64_t* qlev, uint32_t K, const uint32_t* qpos,
+                      uint32_t* out);
+// Screen of the graphs narrow[0 .. n_narrow) (at most 128 nodes) and wide[0 .. n_wide) (batch
+// indices) against every query (prefilter records qpre, pre_dwords u32 each; first record and
+// node count in words 17 and 18); e.ctl must be zeroed.  Writes the graphs' GMCR bit rows
+// valid[gid[b]] (ceil(Q / 64) words each).
+int big_screen(hipStream_t st, const bms2_device_csr* data, const uint64_t* dpath,
+               const uint64_t* dcyc, const uint64_t* dlev, uint32_t K, const uint32_t* narrow,
+               uint32_t n_narrow, const uint32_t* wide, uint32_t n_wide, uint32_t wide_above,
+               uint32_t Q, const uint32_t* qpre, uint32_t pre_dwords, const uint32_t* bqrec,
+               const BigEntries& e, const uint32_t* gid, uint64_t* valid);
+// Joins of the screen's entries (class c: count[c] entries), the staged join's kernel reading the
+// entries' domains from their records: each pair's count is added into d_counts[d_gid[g] Q + q]
+// (zeroed), and the matched pairs, embeddings and DFS nodes into d_stats[1..3].
+uint64_t big_join_scratch_bytes(const bms2_device_csr* query, const bms2_device_csr* data,
+                                const BigJoinShape* shape);
+int big_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
+             const BigEntries& e, const uint64_t count[kBigClasses], int find_first,
+             const uint8_t* d_jorder, uint64_t* d_counts, const uint32_t* d_gid, uint64_t* d_stats,
+             const BigJoinShape* shape);
+// Size split of the graphs above 64 nodes under a launch shape: graphs above this many nodes take
+// 4-word adjacency rows and domains, the others 2.
+inline uint32_t big_wide_above(const BigJoinShape& sh) { return sh.row_words == 4 ? 64u : 128u; }
 
 }  // namespace bms2

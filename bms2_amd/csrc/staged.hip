@@ -508,12 +508,8 @@ __global__ __launch_bounds__(256) void gmcr_fill_kernel(const uint64_t* __restri
 
 // ---- join for data graphs of 65-256 nodes
 
-// Size classes of the GMCR entries on graphs above 64 nodes: graphs of <= 128 and <= 256 nodes
-// (u64 words per adjacency row: 2 or 4), one join instance per class.
-constexpr int kBigClasses = 2;
-// Class of a graph of n nodes: rows of 2 words up to `wide_above` nodes (128, or 64 to give every
-// graph rows of 4 words), 4 words above.
-__host__ __device__ inline int big_class(uint32_t n, uint32_t wide_above) { return n > wide_above ? 1 : 0; }
+// GMCR entries on graphs above 64 nodes are sorted into the size classes of big_class, one join
+// instance per class.
 constexpr int kBigDepth = BMS2_MAX_QUERY_NODES;  // depths of a lane's stash
 constexpr uint32_t kBigWavesPerCU = 12;
 
@@ -569,6 +565,14 @@ __global__ __launch_bounds__(256) void big_classify_kernel(
 // depth's candidates as X & ~{v} & (adjacent ? N(v) : ~N(v)).  A descent pushes R to the stash
 // and builds the new X from the stashed domain and the rows of the nodes matched so far (loads
 // in flight together); a backtrack pops R and rebuilds X the same way.
+// Work sharing (count-all mode): once a wave has no entries left and at least kShareIdle of its
+// lanes are idle, busy lanes give work to idle ones.  A donor gives all remaining candidates of
+// its shallowest depth that has any (or, when only its current depth has, the upper half of
+// them); the receiver copies the donor's matched nodes above that depth and searches the given
+// candidates' subtrees as a unit based at that depth.  A pair's units add their counts and DFS
+// nodes; the sums equal the serial search's, since every subtree is searched exactly once.  A
+// heavy pair so spreads over the wave's 64 lanes instead of holding one lane while the chip
+// idles.  First-match mode never shares: its DFS-node count depends on the search order.
 
 struct BigJoinParams {
   const uint64_t* cand;
@@ -584,11 +588,39 @@ struct BigJoinParams {
   uint64_t* stash;
   uint32_t SD;                  // stash depth
   int find_first;
-  uint64_t* total;
-  uint64_t* per_pair;           // nullable: count per GMCR entry (find_first: 0/1)
+  uint64_t* total;              // nullable
+  uint64_t* per_pair;           // nullable, zeroed: count per GMCR entry (find_first: 0/1)
   const uint8_t* jorder;
-  uint64_t* per_pair_nodes;     // nullable
+  uint64_t* per_pair_nodes;     // nullable, zeroed
+  // entries of the fused pipeline's big-graph screen (recs non-null): class c's entries are
+  // c class_stride + i for i < ctl[c], with their graph in egraph and their domains (NW words per
+  // query node, by local id) at recs + rec_off; list is unused
+  const uint32_t* egraph;
+  const uint64_t* rec_off;
+  const uint64_t* recs;
+  uint64_t class_stride;
+  // outputs of the fused pipeline (nullable): counts[gid[g] Q + q] (zeroed) and the statistics
+  // {matched pairs, embeddings, DFS nodes} added at stats[1..3]
+  uint64_t* out_counts;
+  const uint32_t* out_gid;
+  uint32_t Q;
+  uint64_t* stats;
 };
+
+constexpr int kShareIdle = 16;  // idle lanes of a wave that start a round of work sharing
+// Entries are taken in the order i -> i x (kSpread mod T) mod T, a bijection of [0, T) for every
+// T < 2^32 (kSpread is a prime above 2^32): a claim of consecutive numbers then holds entries of
+// many data graphs rather than a run of one graph's, whose heavy pairs would share one wave.
+constexpr uint64_t kSpread = 4294967311ull;
+
+// Position of the k-th (from 0) set bit of m; m has more than k set bits.
+__device__ inline uint32_t nth_bit(uint64_t m, uint32_t k) {
+  uint32_t lo = 0;
+#pragma unroll
+  for (uint32_t step = 32; step; step >>= 1)
+    if (uint32_t(popc64(m & low_mask(lo + step))) <= k) lo += step;
+  return lo;
+}
 
 template <int NW, int BW>  // BW: waves per workgroup
 __global__ __launch_bounds__(64 * BW) void join_big_kernel(BigJoinParams p, int cls) {
@@ -596,20 +628,27 @@ __global__ __launch_bounds__(64 * BW) void join_big_kernel(BigJoinParams p, int 
   __shared__ uint8_t F[NQ][64 * BW];  // node matched per depth, per thread of the workgroup
   const uint32_t lane = lane_id();
   const uint32_t tid = threadIdx.x;
+  const uint32_t wbase = tid - lane;  // the wave's first thread
   unsigned long long* const cursor = p.ctl + 2 * kBigClasses + cls;
   uint64_t off = 0;
-  for (int c = 0; c < cls; ++c) off += p.ctl[c];
+  if (p.recs)
+    off = uint64_t(cls) * p.class_stride;
+  else
+    for (int c = 0; c < cls; ++c) off += p.ctl[c];
   const uint64_t T = p.ctl[cls];
+  const uint64_t spread = T > 1 ? kSpread % T : 1;
   const uint64_t* L = p.list + off;
   // this lane's stash: the domains [SD][NW], then the pushed R [SD][NW] (SD >= the batch's
   // largest query)
   uint64_t* const DS = p.stash + (uint64_t(blockIdx.x) * 64 * BW + tid) * (2 * p.SD * NW);
   uint64_t* const RSt = DS + p.SD * NW;
+  const bool share = !p.find_first;
   bool busy = false, exhausted = false;
-  uint64_t t = 0, c = 0, nodes = 0, sum = 0;
+  uint64_t t = 0, c = 0, nodes = 0, sum = 0, matched = 0, dfs = 0;
   uint64_t R[NW], X[NW], used[NW];
-  uint32_t s = 0, q = 0, nq = 0;
-  int depth = 0;
+  uint32_t s = 0, q = 0, nq = 0, g = 0;
+  uint32_t pend = 0;  // depths below the current one whose pushed R is not empty
+  int depth = 0, bd = 0;  // current depth, depth the lane's unit starts at
   uint64_t next = 0, end = 0;
   constexpr uint64_t kClaim = 256;
   constexpr int kHandout = 32;
@@ -644,6 +683,44 @@ __global__ __launch_bounds__(64 * BW) void join_big_kernel(BigJoinParams p, int 
       for (int k = 0; k < NW; ++k) out[k] &= a ? last_row[k] : ~last_row[k];
     }
   };
+  // every depth's domain of the lane's pair into its stash, four depths' loads in flight at a
+  // time; at a pair's start also R (depth 0's domain) and X (depth 1's: nq >= 2, single-node
+  // queries are never GMCR pairs)
+  auto load_domains = [&](bool start) {
+    const uint32_t qb = p.qgo[q], e = p.dgo[g + 1];
+    for (uint32_t i0 = 0; i0 < nq; i0 += 4) {
+      uint64_t d4[4][NW];
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t i = i0 + j;
+        if (i < nq) {
+          const uint32_t u = p.jorder ? p.jorder[size_t(q) * 32 + i] : i;
+          if (p.recs) {
+            const uint64_t* dom = p.recs + p.rec_off[t] + uint64_t(u) * NW;
+#pragma unroll
+            for (int k = 0; k < NW; ++k) d4[j][k] = dom[k];
+          } else {
+            const uint64_t* row = p.cand + uint64_t(qb + u) * p.W;
+#pragma unroll
+            for (int k = 0; k < NW; ++k) d4[j][k] = row_bits(row, s, e, k);
+          }
+        }
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j)
+        if (i0 + j < nq) {
+#pragma unroll
+          for (int k = 0; k < NW; ++k) DS[(i0 + j) * NW + k] = d4[j][k];
+        }
+      if (start && i0 == 0) {
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+          R[k] = d4[0][k];
+          X[k] = d4[1][k];
+        }
+      }
+    }
+  };
   while (true) {
     const uint64_t idle = ballot(!busy);
     if (popc64(idle) >= kHandout || idle == ~uint64_t(0)) {
@@ -659,47 +736,113 @@ __global__ __launch_bounds__(64 * BW) void join_big_kernel(BigJoinParams p, int 
       const uint64_t rank = uint64_t(popc64(idle & low_mask(lane)));
       if (!busy && rank < take) {
         busy = true;
-        const uint64_t ent = L[next + rank];
-        t = ent & 0xffffffffu;
-        const uint32_t g = uint32_t(ent >> 32);
+        const uint64_t pos = (next + rank) * spread % T;
+        if (p.recs) {
+          t = off + pos;
+          g = p.egraph[t];
+        } else {
+          const uint64_t ent = L[pos];
+          t = ent & 0xffffffffu;
+          g = uint32_t(ent >> 32);
+        }
         q = p.gidx[t];
         s = p.dgo[g];
-        const uint32_t e = p.dgo[g + 1], qb = p.qgo[q];
-        nq = p.qgo[q + 1] - qb;
-        // every depth's domain into the stash, four depths' loads in flight at a time
-        for (uint32_t i0 = 0; i0 < nq; i0 += 4) {
-          uint64_t d4[4][NW];
-#pragma unroll
-          for (uint32_t j = 0; j < 4; ++j) {
-            const uint32_t i = i0 + j;
-            if (i < nq) {
-              const uint32_t u = p.jorder ? p.jorder[size_t(q) * 32 + i] : i;
-              const uint64_t* row = p.cand + uint64_t(qb + u) * p.W;
-#pragma unroll
-              for (int k = 0; k < NW; ++k) d4[j][k] = row_bits(row, s, e, k);
-            }
-          }
-#pragma unroll
-          for (uint32_t j = 0; j < 4; ++j)
-            if (i0 + j < nq) {
-#pragma unroll
-              for (int k = 0; k < NW; ++k) DS[(i0 + j) * NW + k] = d4[j][k];
-            }
-          if (i0 == 0) {
-#pragma unroll
-            for (int k = 0; k < NW; ++k) {
-              R[k] = d4[0][k];
-              X[k] = d4[1][k];  // nq >= 2: single-node queries are never GMCR pairs
-            }
-          }
-        }
+        nq = p.qgo[q + 1] - p.qgo[q];
+        load_domains(true);
 #pragma unroll
         for (int k = 0; k < NW; ++k) used[k] = 0;
-        depth = 0;
+        depth = bd = 0;
+        pend = 0;
         c = 0;
         nodes = 0;
       }
       next += take;
+    }
+    if (share && exhausted && next >= end) {
+      const uint64_t free_lanes = ballot(!busy);
+      if (popc64(free_lanes) >= kShareIdle && free_lanes != ~uint64_t(0)) {
+        // a donor's depth: the shallowest with pushed candidates, else its current one when it
+        // holds two or more
+        int d0 = -1;
+        if (busy) {
+          if (pend) {
+            d0 = __builtin_ctz(pend);
+          } else {
+            uint32_t pc = 0;
+#pragma unroll
+            for (int k = 0; k < NW; ++k) pc += uint32_t(popc64(R[k]));
+            if (pc >= 2) d0 = depth;
+          }
+        }
+        const uint64_t donors = ballot(d0 >= 0);
+        const uint32_t npair = min(uint32_t(popc64(donors)), uint32_t(popc64(free_lanes)));
+        if (npair) {
+          uint64_t give[NW];
+#pragma unroll
+          for (int k = 0; k < NW; ++k) give[k] = 0;
+          if (d0 >= 0 && uint32_t(popc64(donors & low_mask(lane))) < npair) {
+            if (d0 < depth) {
+#pragma unroll
+              for (int k = 0; k < NW; ++k) {
+                give[k] = RSt[d0 * NW + k];
+                RSt[d0 * NW + k] = 0;
+              }
+              pend &= ~(1u << d0);
+            } else {
+              // keep the lower half of the current candidates, give the upper half
+              uint32_t pc = 0;
+#pragma unroll
+              for (int k = 0; k < NW; ++k) pc += uint32_t(popc64(R[k]));
+              uint32_t keep = pc - pc / 2;
+#pragma unroll
+              for (int k = 0; k < NW; ++k) {
+                const uint32_t n = uint32_t(popc64(R[k]));
+                if (keep >= n) {
+                  keep -= n;
+                } else {
+                  const uint64_t m = keep ? low_mask(nth_bit(R[k], keep)) : 0;
+                  give[k] = R[k] & ~m;
+                  R[k] &= m;
+                  keep = 0;
+                }
+              }
+            }
+          }
+          // the k-th idle lane takes the k-th donor's gift
+          const uint32_t rr = uint32_t(popc64(free_lanes & low_mask(lane)));
+          const bool recv = !busy && rr < npair;
+          const int src = recv ? int(nth_bit(donors, rr)) : int(lane);
+          const uint64_t st = __shfl(t, src);
+          const uint32_t sg = __shfl(g, src), sq = __shfl(q, src);
+          const int sd = __shfl(d0, src);
+          uint64_t gv[NW];
+#pragma unroll
+          for (int k = 0; k < NW; ++k) gv[k] = __shfl(give[k], src);
+          if (recv) {
+            t = st;
+            g = sg;
+            q = sq;
+            s = p.dgo[g];
+            nq = p.qgo[q + 1] - p.qgo[q];
+#pragma unroll
+            for (int k = 0; k < NW; ++k) used[k] = 0;
+            for (int i = 0; i < sd; ++i) {
+              const uint32_t f = F[i][wbase + uint32_t(src)];
+              F[i][tid] = uint8_t(f);
+              used[f >> 6] |= uint64_t(1) << (f & 63);
+            }
+            load_domains(false);
+            build_x(sd, nullptr, X);
+#pragma unroll
+            for (int k = 0; k < NW; ++k) R[k] = gv[k];
+            depth = bd = sd;
+            pend = 0;
+            c = 0;
+            nodes = 0;
+            busy = true;
+          }
+        }
+      }
     }
     if (!ballot(busy)) {
       if (exhausted && next >= end) break;
@@ -711,12 +854,13 @@ __global__ __launch_bounds__(64 * BW) void join_big_kernel(BigJoinParams p, int 
 #pragma unroll
       for (int k = 0; k < NW; ++k) any |= R[k];
       if (!any) {
-        if (depth == 0) {
+        if (depth == bd) {
           finished = true;
         } else {
           --depth;
 #pragma unroll
           for (int k = 0; k < NW; ++k) R[k] = RSt[depth * NW + k];
+          pend &= ~(1u << depth);
           const uint32_t f = F[depth][tid];
           used[f >> 6] &= ~(uint64_t(1) << (f & 63));
           build_x(depth, nullptr, X);
@@ -751,8 +895,13 @@ __global__ __launch_bounds__(64 * BW) void join_big_kernel(BigJoinParams p, int 
           if (p.find_first && c) finished = true;
         } else if (ny) {
           // descend: push R, then X of the new depth (v is matched at `depth` now)
+          uint64_t rest = 0;
 #pragma unroll
-          for (int k = 0; k < NW; ++k) RSt[depth * NW + k] = R[k];
+          for (int k = 0; k < NW; ++k) {
+            RSt[depth * NW + k] = R[k];
+            rest |= R[k];
+          }
+          if (rest) pend |= 1u << depth;
           F[depth][tid] = uint8_t(v);
           used[v >> 6] |= uint64_t(1) << (v & 63);
           build_x(nd, nv, X);
@@ -764,14 +913,36 @@ __global__ __launch_bounds__(64 * BW) void join_big_kernel(BigJoinParams p, int 
     }
     if (finished) {
       const uint64_t cc = p.find_first ? (c ? 1 : 0) : c;
-      if (p.per_pair) p.per_pair[t] = cc;
-      if (p.per_pair_nodes) p.per_pair_nodes[t] = nodes;
+      if (p.out_counts) {
+        if (cc) {
+          const unsigned long long old = atomicAdd(
+              reinterpret_cast<unsigned long long*>(p.out_counts + uint64_t(p.out_gid[g]) * p.Q + q),
+              (unsigned long long)cc);
+          if (!old) ++matched;
+        }
+      } else if (p.per_pair && cc) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(p.per_pair + t), (unsigned long long)cc);
+      }
+      if (p.per_pair_nodes && nodes)
+        atomicAdd(reinterpret_cast<unsigned long long*>(p.per_pair_nodes + t), (unsigned long long)nodes);
       sum += cc;
+      dfs += nodes;
       busy = false;
     }
   }
-  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
-  if (lane == 0 && sum) atomicAdd(reinterpret_cast<unsigned long long*>(p.total), (unsigned long long)sum);
+  for (int o = 32; o > 0; o >>= 1) {
+    sum += __shfl_xor(sum, o);
+    matched += __shfl_xor(matched, o);
+    dfs += __shfl_xor(dfs, o);
+  }
+  if (lane == 0) {
+    if (p.total && sum) atomicAdd(reinterpret_cast<unsigned long long*>(p.total), (unsigned long long)sum);
+    if (p.stats) {
+      atomicAdd(reinterpret_cast<unsigned long long*>(p.stats + 1), (unsigned long long)matched);
+      atomicAdd(reinterpret_cast<unsigned long long*>(p.stats + 2), (unsigned long long)sum);
+      atomicAdd(reinterpret_cast<unsigned long long*>(p.stats + 3), (unsigned long long)dfs);
+    }
+  }
 }
 
 
@@ -1484,9 +1655,15 @@ int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_cs
                        0, st, data->graph_offsets, data->num_graphs, gmcr->offsets, pass,
                        wide_above, ctl, list);
   BMS2_CHECK_LAUNCH("big_classify_kernel");
+  // units of one pair add their counts and DFS nodes
+  if (d_pair_counts)
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(T, 256, 4096)), dim3(256), 0, st, d_pair_counts, T);
+  if (d_pair_nodes)
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(T, 256, 4096)), dim3(256), 0, st, d_pair_nodes, T);
   BigJoinParams bp{cand->bits, cand->W, query->graph_offsets, qa, gmcr->indices,
                    data->graph_offsets, rows, RW, list, ctl, stash, SD, find_first, d_total,
-                   d_pair_counts, d_jorder, d_pair_nodes};
+                   d_pair_counts, d_jorder, d_pair_nodes, nullptr, nullptr, nullptr, 0,
+                   nullptr, nullptr, 0, nullptr};
   const dim3 grid(uint32_t(cus) * waves / bw), block(64 * bw);
   auto launch = [&](auto kern, int cls) { hipLaunchKernelGGL(kern, grid, block, 0, st, bp, cls); };
   if (bw == 4) {
@@ -1499,6 +1676,76 @@ int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_cs
     launch(join_big_kernel<2, 1>, 0);
     if (RW == 4) launch(join_big_kernel<4, 1>, 1);
   }
+  BMS2_CHECK_LAUNCH("join_big_kernel");
+  return BMS2_OK;
+}
+
+const uint64_t* path_signature_bits(bms2_path_signature s, int scope) { return s->bits[scope]; }
+const uint64_t* cycle_signature_values(bms2_cycle_signature s, int scope) { return s->sig[scope]; }
+
+namespace {
+
+// Launch of the size-class joins under a shape: waves per workgroup and waves per CU.
+void big_launch_shape(const BigJoinShape* shape, uint32_t* bw, uint32_t* waves) {
+  const BigJoinShape sh = shape ? *shape : BigJoinShape{};
+  *bw = sh.block_waves == 4 ? 4 : sh.block_waves == 2 ? 2 : 1;
+  *waves = std::max<uint32_t>(*bw, std::min<uint32_t>(32, sh.waves ? sh.waves : kBigWavesPerCU)) / *bw * *bw;
+}
+
+}  // namespace
+
+uint64_t big_join_scratch_bytes(const bms2_device_csr* query, const bms2_device_csr* data,
+                                const BigJoinShape* shape) {
+  uint32_t bw = 1, waves = 1;
+  big_launch_shape(shape, &bw, &waves);
+  const uint32_t wide_above = big_wide_above(shape ? *shape : BigJoinShape{});
+  const uint64_t RW = data->max_graph_nodes > wide_above ? 4 : 2;
+  const uint64_t nrows = std::max<uint64_t>(1, data->total_nodes);
+  const uint64_t qa = (4 * 32 * uint64_t(query->num_graphs) + 7) / 8 * 8;
+  return 8 * RW * nrows + qa + uint64_t(device_cus()) * waves * 64 * 2 * stash_depth(query) * RW * 8;
+}
+
+int big_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
+             const BigEntries& e, const uint64_t count[kBigClasses], int find_first,
+             const uint8_t* d_jorder, uint64_t* d_counts, const uint32_t* d_gid, uint64_t* d_stats,
+             const BigJoinShape* shape) {
+  if (!count[0] && !count[1]) return BMS2_OK;
+  hipStream_t st = q->stream;
+  uint32_t bw = 1, waves = 1;
+  big_launch_shape(shape, &bw, &waves);
+  const uint32_t wide_above = big_wide_above(shape ? *shape : BigJoinShape{});
+  const uint32_t RW = data->max_graph_nodes > wide_above ? 4 : 2;
+  if (count[1] && RW != 4)
+    return set_error(BMS2_ERR_INVALID, "entries of 4-word graphs in a batch of 2-word graphs");
+  const uint64_t nrows = std::max<uint64_t>(1, data->total_nodes);
+  const uint64_t nqa = uint64_t(query->num_graphs) * 32;
+  uint64_t* rows = nullptr;
+  if (int rc = queue_scratch(q, big_join_scratch_bytes(query, data, shape), (void**)&rows)) return rc;
+  auto* qa = reinterpret_cast<uint32_t*>(rows + RW * nrows);
+  uint64_t* stash = rows + RW * nrows + (nqa + 1) / 2;
+  if (nqa)
+    hipLaunchKernelGGL(join_depth_adjacency_kernel, dim3(grid_for(nqa, 256)), dim3(256), 0, st,
+                       query->graph_offsets, query->row_offsets, query->column_indices,
+                       query->num_graphs, d_jorder, qa);
+  hipLaunchKernelGGL(adj_rows_wide_kernel, dim3(grid_for(data->total_nodes, 256, 1u << 30)),
+                     dim3(256), 0, st, data->graph_offsets, data->num_graphs, data->row_offsets,
+                     data->column_indices, data->total_nodes, RW, rows);
+  BMS2_CHECK_LAUNCH("adj_rows_wide_kernel");
+  BigJoinParams bp{nullptr, 0, query->graph_offsets, qa, e.gidx, data->graph_offsets, rows, RW,
+                   nullptr, e.ctl, stash, stash_depth(query), find_first, nullptr, nullptr,
+                   d_jorder, nullptr, e.egraph, e.rec_off, e.recs, e.cap_e, d_counts, d_gid,
+                   query->num_graphs, d_stats};
+  const dim3 grid(uint32_t(device_cus()) * waves / bw), block(64 * bw);
+  auto launch = [&](auto k2, auto k4) {
+    if (count[0]) hipLaunchKernelGGL(k2, grid, block, 0, st, bp, 0);
+    if (count[1]) hipLaunchKernelGGL(k4, grid, block, 0, st, bp, 1);
+  };
+  if (bw == 4)
+    launch(join_big_kernel<2, 4>, join_big_kernel<4, 4>);
+  else if (bw == 2)
+    launch(join_big_kernel<2, 2>, join_big_kernel<4, 2>);
+  else
+    launch(join_big_kernel<2, 1>, join_big_kernel<4, 1>);
   BMS2_CHECK_LAUNCH("join_big_kernel");
   return BMS2_OK;
 }

@@ -120,8 +120,9 @@ def uneven_2k():
 @pytest.mark.parametrize("K", [0, 2, 6])
 @pytest.mark.parametrize("find_first", [False, True])
 def test_fused_uneven_sizes(queue, uneven_2k, K, find_first):
-    """Graphs above 64 nodes take the staged kernels inside bms2_pipeline_run, queries above 16
-    nodes the 32-deep join; all counts, GMCR rows and statistics equal the oracle's."""
+    """Graphs above 64 nodes take the big-graph screen and size-class joins inside
+    bms2_pipeline_run, queries above 16 nodes the 32-deep join; all counts, GMCR rows and
+    statistics equal the oracle's."""
     data, queries = uneven_2k
     assert np.diff(np.asarray(data.graph_offsets)).max() > 64
     assert np.diff(np.asarray(queries.graph_offsets)).max() > 16
@@ -130,6 +131,49 @@ def test_fused_uneven_sizes(queue, uneven_2k, K, find_first):
     assert np.array_equal(valid, rv)
     assert np.array_equal(counts, rc)
     assert np.array_equal(stats[:4], rs)
+
+
+@pytest.fixture(scope="module")
+def big_graphs():
+    """400 graphs of 65-250 nodes (lognormal around 120 nodes, 10% fused ring systems; about 45%
+    above 128 nodes) with the uneven workload's 1,024 queries: every graph takes the big-graph
+    screen, in both size classes."""
+    return graphs.generate("uneven-1m", seed=3, num_data_graphs=400, data_min_nodes=65,
+                           data_lognormal_mu=4.8, data_lognormal_sigma=0.4)
+
+
+@pytest.mark.parametrize("K", [0, 1, 6])
+@pytest.mark.parametrize("find_first", [False, True])
+def test_fused_big_graphs(queue, big_graphs, K, find_first):
+    data, queries = big_graphs
+    sizes = np.diff(data.graph_offsets.astype(np.int64))
+    assert sizes.min() > 64 and (sizes > 128).sum() >= 100
+    counts, valid, stats = api.run_pipeline(queue, queries, data, K, find_first)
+    rc, rv, rs = oracle.pipeline(queries, data, K, find_first)
+    assert rs[0] > 10000  # GMCR pairs: the screen's entries span several query blocks per graph
+    assert np.array_equal(valid, rv)
+    assert np.array_equal(counts, rc)
+    assert np.array_equal(stats[:4], rs)
+
+
+def test_fused_big_graphs_autotuned(queue, big_graphs):
+    """Every launch shape of the autotuner (both row-word splits of the size classes) leaves the
+    results of the big-graph path unchanged; the best shape's run is compared again."""
+    data, queries = big_graphs
+    rc, rv, rs = oracle.pipeline(queries, data, 6, False)
+    pipe = api.Pipeline(queue, api.DeviceBatchedCSRGraph(queue, queries),
+                        api.DeviceBatchedCSRGraph(queue, data), refinement_steps=6)
+    counts, valid, stats = pipe.alloc_outputs()
+    t = pipe.autotune(counts, valid, runs=1)
+    assert t["big_row_words"] in (2, 4)
+    c, v = pipe.results(counts, valid)
+    assert np.array_equal(c, rc) and np.array_equal(v, rv)
+    counts.zero()
+    pipe.run(counts, valid, stats)
+    queue.wait()
+    c, v = pipe.results(counts, valid)
+    assert np.array_equal(c, rc) and np.array_equal(v, rv)
+    assert np.array_equal(stats.to_numpy(np.uint64, 8)[:4], rs)
 
 
 def test_fused_uneven_order_and_trace(queue, uneven_2k):
