@@ -77,23 +77,31 @@ __global__ void shell_levels_kernel(const uint32_t* __restrict__ go, uint32_t nu
 
 // Screening record of one query node (u32 words): [0] label, [1] label mask (0xff, 0 for the
 // wildcard: any data label), [2] degree, [3] local id, [4..11] path signature, [12..13] ring
-// counts, [14..15] 0, [16 + 4 k .. 19 + 4 k] shell level k < K: the even and the odd 4-bit counts
-// spread one per byte (two u64).  Records sit at the node's slot of the screening order.
+// counts, [14] the node's depth in the matching order (jorder, identity when null), [15] 0,
+// [16 + 4 k .. 19 + 4 k] shell level k < K: the even and the odd 4-bit counts spread one per
+// byte (two u64).  Records sit at the node's slot of the screening order.
 __global__ void big_query_rec_kernel(const uint32_t* __restrict__ qgo, uint32_t Q,
                                      const uint32_t* __restrict__ qro,
                                      const uint8_t* __restrict__ qlab, uint64_t Qn,
                                      const uint64_t* __restrict__ qpath,
                                      const uint64_t* __restrict__ qcyc,
                                      const uint64_t* __restrict__ qlev, uint32_t K,
-                                     const uint32_t* __restrict__ qpos, uint32_t* __restrict__ out) {
+                                     const uint32_t* __restrict__ qpos,
+                                     const uint8_t* __restrict__ jorder, uint32_t* __restrict__ out) {
   const uint64_t u = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (u >= Qn) return;
   uint32_t* r = out + uint64_t(qpos[u]) * kBigRecDwords;
   const uint32_t lab = qlab[u];
+  const uint32_t g = graph_of(qgo, Q, uint32_t(u));
+  const uint32_t local = uint32_t(u) - qgo[g], nq = qgo[g + 1] - qgo[g];
+  uint32_t depth = local;
+  if (jorder)
+    for (uint32_t i = 0; i < nq; ++i)
+      if (jorder[size_t(g) * 32 + i] == local) depth = i;
   r[0] = lab;
   r[1] = lab == kWildcard ? 0u : 0xffu;
   r[2] = qro[u + 1] - qro[u];
-  r[3] = uint32_t(u) - qgo[graph_of(qgo, Q, uint32_t(u))];
+  r[3] = local;
   for (int w = 0; w < 4; ++w) {
     const uint64_t b = qpath[4 * u + w];
     r[4 + 2 * w] = uint32_t(b);
@@ -102,7 +110,8 @@ __global__ void big_query_rec_kernel(const uint32_t* __restrict__ qgo, uint32_t 
   const uint64_t c = qcyc[u];
   r[12] = uint32_t(c);
   r[13] = uint32_t(c >> 32);
-  r[14] = r[15] = 0;
+  r[14] = depth;
+  r[15] = 0;
   for (uint32_t k = 0; k < uint32_t(kBigMaxLevels); ++k) {
     const uint64_t v = k < K ? qlev[uint64_t(k) * Qn + u] : 0;
     const uint64_t e = v & kFifteens, o = (v >> 4) & kFifteens;
@@ -133,16 +142,19 @@ struct BigScreenParams {
   uint32_t VW;
 };
 
-// One wave per data graph (NWS = 64-node slots per lane: 2 up to 128 nodes, 4 above).  Entries
-// of size class c take indices c cap_e + i (i from the class's counter); a block of 64 queries
-// reserves record words for every candidate that passed the prefilter, so a pair's domains are
-// written as its nodes pass, and the block's GMCR pairs get their entries with one atomic and
-// their bits of the graph's GMCR row with one store.  Counters run on past the capacities (the
-// overflow flag is set), so a rerun can size them.
+constexpr uint32_t kRunBuf = 256;  // GMCR pairs of a graph gathered in LDS per run
+
+// One wave per data graph (NWS = 64-node slots per lane: 2 up to 128 nodes, 4 above).  A block of
+// 64 queries reserves record words for every candidate that passed the prefilter, so a pair's
+// domains are written as its nodes pass, and stores its bits of the graph's GMCR row at once.
+// The graph's GMCR pairs gather in LDS and leave as runs of up to kRunBuf entries: one atomic
+// takes the entries (class c: indices c cap_e + i), one the run {graph, count, first entry}
+// (class c: c cap_r + i).  Counters run on past the capacities (the overflow flag is set), so a
+// rerun can size them.
 template <int K, int NWS>
 __global__ __launch_bounds__(64) void big_screen_kernel(BigScreenParams p) {
-  __shared__ uint32_t pair_q[64];
-  __shared__ uint64_t pair_off[64];
+  __shared__ uint32_t pair_q[kRunBuf];
+  __shared__ uint64_t pair_off[kRunBuf];
   const uint32_t lane = lane_id();
   const uint32_t b = p.glist[blockIdx.x];
   const uint32_t s = p.dgo[b], n = p.dgo[b + 1] - s;
@@ -203,6 +215,32 @@ __global__ __launch_bounds__(64) void big_screen_kernel(BigScreenParams p) {
     for (int f = 0; f < 4; ++f) rm[f] = max(rm[f], uint32_t(__shfl_xor(rm[f], off)));
   }
 
+  uint32_t np = 0;  // pairs gathered
+  auto flush = [&]() {
+    if (!np) return;
+    __syncthreads();  // lane 0's pair list visible to every lane (one wave: no waiting)
+    uint64_t eb = 0, ri = 0;
+    if (lane == 0) {
+      eb = atomicAdd(p.e.ctl + cls, (unsigned long long)np);
+      ri = atomicAdd(p.e.ctl + kBigClasses + cls, 1ull);
+    }
+    eb = uniform64(readlane64(eb, 0));
+    ri = uniform64(readlane64(ri, 0));
+    if (eb + np <= p.e.cap_e && ri < p.e.cap_r) {
+      const uint64_t first = cls * p.e.cap_e + eb;
+      for (uint32_t i = lane; i < np; i += 64) {
+        p.e.gidx[first + i] = pair_q[i];
+        p.e.rec_off[first + i] = pair_off[i];
+      }
+      if (lane == 0)
+        p.e.runs[cls * p.e.cap_r + ri] = make_uint4(b, np, uint32_t(first), uint32_t(first >> 32));
+    } else if (lane == 0) {
+      atomicOr(p.e.ctl + kBigCtlOverflow, 1ull);
+    }
+    __syncthreads();
+    np = 0;
+  };
+
   for (uint32_t qb = 0; qb < p.Q; qb += 64) {
     const uint32_t qq = qb + lane;
     bool pass = false;
@@ -223,6 +261,7 @@ __global__ __launch_bounds__(64) void big_screen_kernel(BigScreenParams p) {
       if (lane == 0) *vrow = 0;
       continue;
     }
+    if (np + uint32_t(popc64(cand)) > kRunBuf) flush();
     // record words of every candidate of the block: a candidate that fails leaves a hole
     uint32_t need = pass ? q_nq * RWc : 0u;
 #pragma unroll
@@ -232,7 +271,6 @@ __global__ __launch_bounds__(64) void big_screen_kernel(BigScreenParams p) {
     base = uniform64(readlane64(base, 0));
     const bool rec_ok = base + need <= p.e.cap_words;
     uint64_t roff = base;
-    uint32_t np = 0;
     uint64_t vbits = 0;
     while (cand) {
       const uint32_t li = uint32_t(ctz64(cand));
@@ -274,7 +312,7 @@ __global__ __launch_bounds__(64) void big_screen_kernel(BigScreenParams p) {
 #pragma unroll
           for (int j = 0; j < NWS; ++j)
             if (lane == uint32_t(j)) v = m[j];
-          p.e.recs[roff + uint64_t(w[3]) * RWc + lane] = v;
+          p.e.recs[roff + uint64_t(w[14]) * RWc + lane] = v;
         }
       }
       if (alive) {
@@ -288,25 +326,9 @@ __global__ __launch_bounds__(64) void big_screen_kernel(BigScreenParams p) {
       roff += uint64_t(nq) * RWc;
     }
     if (lane == 0) *vrow = vbits;
-    if (np) {
-      __syncthreads();  // lane 0's pair list visible to every lane (one wave: no waiting)
-      uint64_t eb = 0;
-      if (lane == 0) eb = atomicAdd(p.e.ctl + cls, (unsigned long long)np);
-      eb = uniform64(readlane64(eb, 0));
-      if (eb + np <= p.e.cap_e) {
-        if (lane < np) {
-          const uint64_t t = cls * p.e.cap_e + eb + lane;
-          p.e.gidx[t] = pair_q[lane];
-          p.e.egraph[t] = b;
-          p.e.rec_off[t] = pair_off[lane];
-        }
-      } else if (lane == 0) {
-        atomicOr(p.e.ctl + kBigCtlOverflow, 1ull);
-      }
-      __syncthreads();
-    }
     if (!rec_ok && lane == 0) atomicOr(p.e.ctl + kBigCtlOverflow, 1ull);
   }
+  flush();
 }
 
 template <class F>
@@ -342,11 +364,12 @@ int big_levels(hipStream_t st, const bms2_device_csr* g, uint32_t K, uint64_t* o
 
 int big_query_records(hipStream_t st, const bms2_device_csr* query, const uint64_t* qpath,
                       const uint64_t* qcyc, const uint64_t* qlev, uint32_t K, const uint32_t* qpos,
-                      uint32_t* out) {
+                      const uint8_t* jorder, uint32_t* out) {
   if (query->total_nodes)
     hipLaunchKernelGGL(big_query_rec_kernel, dim3(blocks_for(query->total_nodes, 256)), dim3(256), 0,
                        st, query->graph_offsets, query->num_graphs, query->row_offsets,
-                       query->node_labels, query->total_nodes, qpath, qcyc, qlev, K, qpos, out);
+                       query->node_labels, query->total_nodes, qpath, qcyc, qlev, K, qpos, jorder,
+                       out);
   BMS2_CHECK_LAUNCH("big_query_rec_kernel");
   return BMS2_OK;
 }

@@ -1064,6 +1064,12 @@ struct bms2_pipeline_s {
   uint32_t* d_big_qrec = nullptr;   // [Qn][kBigRecDwords] screening records
   BigEntries be{};                  // the screen's GMCR entries and domain records
   uint64_t* d_big_stats = nullptr;  // [1..3] statistics of the joins
+  // the big-graph path runs on queues of its own beside the fused chunk (and its 4-word class
+  // beside its 2-word class), forked after the run's query records and joined before its end
+  bms2_queue side_q = nullptr, side_q2 = nullptr;
+  hipEvent_t big_fork = nullptr, big_done = nullptr;
+  bool big_pending = false;
+  uint64_t big_pairs = 0;           // GMCR pairs of the pending big-graph path
   // the single-chunk kernels captured in a hipGraph, valid for these outputs and K
   hipGraphExec_t graph = nullptr;
   uint64_t* graph_counts = nullptr;
@@ -1075,32 +1081,39 @@ namespace {
 
 // Releases the big-graph screen's entry and record buffers (the control words stay).
 void free_big_entries(bms2_pipeline_s* p) {
-  for (void* b : {(void*)p->be.gidx, (void*)p->be.egraph, (void*)p->be.rec_off, (void*)p->be.recs})
+  for (void* b : {(void*)p->be.gidx, (void*)p->be.rec_off, (void*)p->be.runs, (void*)p->be.recs})
     (void)hipFree(b);
-  p->be.gidx = p->be.egraph = nullptr;
+  p->be.gidx = nullptr;
   p->be.rec_off = p->be.recs = nullptr;
-  p->be.cap_e = p->be.cap_words = 0;
+  p->be.runs = nullptr;
+  p->be.cap_e = p->be.cap_r = p->be.cap_words = 0;
 }
 
-// Sizes the entry buffers for cap_e entries per class and the records for cap_words words.
-int ensure_big_entries(bms2_pipeline_s* p, uint64_t cap_e, uint64_t cap_words) {
-  if (p->be.gidx && p->be.cap_e >= cap_e && p->be.cap_words >= cap_words) return BMS2_OK;
+// Sizes the entry buffers for cap_e entries and cap_r runs per class and the records for
+// cap_words words.
+int ensure_big_entries(bms2_pipeline_s* p, uint64_t cap_e, uint64_t cap_r, uint64_t cap_words) {
+  if (p->be.gidx && p->be.cap_e >= cap_e && p->be.cap_r >= cap_r && p->be.cap_words >= cap_words)
+    return BMS2_OK;
   cap_e = std::max(cap_e, p->be.cap_e);
+  cap_r = std::max(cap_r, p->be.cap_r);
   cap_words = std::max(cap_words, p->be.cap_words);
-  BMS2_HIP(hipStreamSynchronize(p->q->stream));
+  BMS2_HIP(hipStreamSynchronize(p->side_q->stream));
   free_big_entries(p);
   const uint64_t ne = kBigClasses * cap_e;
   if (int rc = device_alloc((void**)&p->be.gidx, 4 * ne)) return rc;
-  if (int rc = device_alloc((void**)&p->be.egraph, 4 * ne)) return rc;
   if (int rc = device_alloc((void**)&p->be.rec_off, 8 * ne)) return rc;
+  if (int rc = device_alloc((void**)&p->be.runs, 16 * kBigClasses * cap_r)) return rc;
   if (int rc = device_alloc((void**)&p->be.recs, 8 * cap_words)) return rc;
   p->be.cap_e = cap_e;
+  p->be.cap_r = cap_r;
   p->be.cap_words = cap_words;
   return BMS2_OK;
 }
 
 void pipeline_free(bms2_pipeline p) {
   if (!p) return;
+  if (p->side_q) (void)bms2_queue_wait(p->side_q);
+  if (p->side_q2) (void)bms2_queue_wait(p->side_q2);
   (void)hipFree(p->d_packs);
   (void)hipFree(p->d_pack_gids);
   (void)hipFree(p->d_qpacks);
@@ -1128,6 +1141,10 @@ void pipeline_free(bms2_pipeline p) {
   if (p->big_path) bms2_path_signature_destroy(p->big_path);
   if (p->big_cyc) bms2_cycle_signature_destroy(p->big_cyc);
   if (p->big.graph_offsets) bms2_device_csr_destroy(p->q, &p->big);
+  if (p->big_fork) (void)hipEventDestroy(p->big_fork);
+  if (p->big_done) (void)hipEventDestroy(p->big_done);
+  if (p->side_q2) bms2_queue_destroy(p->side_q2);
+  if (p->side_q) bms2_queue_destroy(p->side_q);
   delete p;
 }
 
@@ -1192,21 +1209,27 @@ int build_big_batch(bms2_pipeline_s* p, const std::vector<uint32_t>& go) {
   if (int rc = device_alloc((void**)&p->d_big_lev, 8 * K * std::max<uint64_t>(h.total_nodes, 1))) return rc;
   if (int rc = device_alloc((void**)&p->be.ctl, 8 * kBigCtlWords)) return rc;
   if (int rc = device_alloc((void**)&p->d_big_stats, 8 * 8)) return rc;
+  if (int rc = bms2_queue_create(p->q->device, &p->side_q)) return rc;
+  if (int rc = bms2_queue_create(p->q->device, &p->side_q2)) return rc;
+  BMS2_HIP(hipEventCreateWithFlags(&p->big_fork, hipEventDisableTiming));
+  BMS2_HIP(hipEventCreateWithFlags(&p->big_done, hipEventDisableTiming));
   BMS2_HIP(hipMemcpy(p->d_big_gid, gid.data(), 4ull * gid.size(), hipMemcpyHostToDevice));
   BMS2_HIP(hipMemcpy(p->d_big_list, list.data(), 4ull * list.size(), hipMemcpyHostToDevice));
   return BMS2_OK;
 }
 
-// Screens and joins the graphs above 64 nodes with an original id below g_limit and writes their
-// rows of the dense outputs; adds their statistics to stats.  The signatures are the staged
+// Enqueues the screen and joins of the graphs above 64 nodes with an original id below g_limit
+// on the side queue, after the main stream's work up to p->big_fork (this run's query records);
+// they write those graphs' rows of the dense outputs.  The side queue's stream is synchronised
+// once, to size the screen's buffers; big_finish joins it.  The signatures are the staged
 // kernels' (path signatures; ring counts, which stay zero on graphs above 64 nodes as in
 // signature.hpp:614) and the shell levels of every refinement step; the screen replaces the
 // filter, the refinement steps and the GMCR of the staged sequence (same domains on every GMCR
-// pair), the join is the staged join's kernel over the screen's entries.
-int run_big(bms2_pipeline_s* p, uint64_t* d_counts, uint64_t* d_valid, uint32_t g_limit,
-            uint64_t stats[8]) {
+// pair), the join is the staged join's kernel over the screen's runs of entries.
+int big_start(bms2_pipeline_s* p, uint64_t* d_counts, uint64_t* d_valid, uint32_t g_limit) {
+  p->big_pending = false;
   if (!p->num_big) return BMS2_OK;
-  bms2_queue q = p->q;
+  bms2_queue q = p->side_q;
   hipStream_t s = q->stream;
   const bms2_device_csr* dq = &p->query;
   const bms2_device_csr* db = &p->big;
@@ -1214,6 +1237,7 @@ int run_big(bms2_pipeline_s* p, uint64_t* d_counts, uint64_t* d_valid, uint32_t 
   const uint32_t n_narrow = uint32_t(std::lower_bound(p->big_narrow_gid.begin(), p->big_narrow_gid.end(), g_limit) - p->big_narrow_gid.begin());
   const uint32_t n_wide = uint32_t(std::lower_bound(p->big_wide_gid.begin(), p->big_wide_gid.end(), g_limit) - p->big_wide_gid.begin());
   if (!n_narrow && !n_wide) return BMS2_OK;
+  BMS2_HIP(hipStreamWaitEvent(s, p->big_fork, 0));
   if (int rc = bms2_path_signature_generate(p->big_path, db, BMS2_SCOPE_DATA, nullptr)) return rc;
   if (int rc = bms2_cycle_signature_generate(p->big_cyc, db, BMS2_SCOPE_DATA, nullptr)) return rc;
   if (int rc = bms2_path_signature_generate(p->big_path, dq, BMS2_SCOPE_QUERY, nullptr)) return rc;
@@ -1222,7 +1246,7 @@ int run_big(bms2_pipeline_s* p, uint64_t* d_counts, uint64_t* d_valid, uint32_t 
   if (int rc = big_levels(s, dq, K, p->d_big_qlev)) return rc;
   if (int rc = big_query_records(s, dq, path_signature_bits(p->big_path, BMS2_SCOPE_QUERY),
                                  cycle_signature_values(p->big_cyc, BMS2_SCOPE_QUERY), p->d_big_qlev,
-                                 K, p->d_qpos, p->d_big_qrec))
+                                 K, p->d_qpos, p->d_jorder, p->d_big_qrec))
     return rc;
   const uint32_t wide_above = big_wide_above(p->big_shape);
   const uint32_t Q = dq->num_graphs;
@@ -1230,8 +1254,10 @@ int run_big(bms2_pipeline_s* p, uint64_t* d_counts, uint64_t* d_valid, uint32_t 
   hipLaunchKernelGGL(big_zero_kernel, dim3(p->num_big), dim3(256), 0, s, p->d_big_gid, p->num_big,
                      Q, g_limit, d_counts);
   BMS2_CHECK_LAUNCH("big_zero_kernel");
-  // first sizes: 128 entries per graph, 48 record words per entry; then what a run measured
+  // first sizes: 128 entries and 2 runs per graph, 48 record words per entry; then what a run
+  // measured
   if (int rc = ensure_big_entries(p, std::max<uint64_t>(1024, 128ull * p->num_big),
+                                  std::max<uint64_t>(1024, 2ull * p->num_big),
                                   std::max<uint64_t>(1024, 128ull * 48 * p->num_big)))
     return rc;
   for (int attempt = 0;; ++attempt) {
@@ -1251,17 +1277,32 @@ int run_big(bms2_pipeline_s* p, uint64_t* d_counts, uint64_t* d_valid, uint32_t 
       return set_error(BMS2_ERR_CAPACITY, "big-graph screen overflowed its measured buffers");
     // the counters ran on past the capacities: size to them, with room for the next batch
     const uint64_t need_e = std::max(ctl[0], ctl[1]), need_w = ctl[kBigCtlRecWords];
-    if (int rc = ensure_big_entries(p, need_e + need_e / 8 + 1024, need_w + need_w / 8 + 1024)) return rc;
+    const uint64_t need_r = std::max(ctl[kBigClasses], ctl[kBigClasses + 1]);
+    if (int rc = ensure_big_entries(p, need_e + need_e / 8 + 1024, need_r + need_r / 8 + 1024,
+                                    need_w + need_w / 8 + 1024))
+      return rc;
   }
   const uint64_t count[kBigClasses] = {ctl[0], ctl[1]};
+  const uint64_t runs[kBigClasses] = {ctl[kBigClasses], ctl[kBigClasses + 1]};
   hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_big_stats, 8u);
-  if (int rc = big_join(q, dq, db, p->be, count, int(p->opt.find_first), p->d_jorder, d_counts,
-                        p->d_big_gid, p->d_big_stats, &p->big_shape))
+  if (int rc = big_join(q, p->side_q2, dq, db, p->be, count, runs, int(p->opt.find_first),
+                        p->d_jorder, d_counts, p->d_big_gid, p->d_big_stats, &p->big_shape))
     return rc;
+  BMS2_HIP(hipEventRecord(p->big_done, s));
+  p->big_pairs = count[0] + count[1];
+  p->big_pending = true;
+  return BMS2_OK;
+}
+
+// Makes the main stream wait for the big-graph path and adds its statistics to stats.
+int big_finish(bms2_pipeline_s* p, uint64_t stats[8]) {
+  if (!p->big_pending) return BMS2_OK;
+  p->big_pending = false;
+  BMS2_HIP(hipStreamWaitEvent(p->q->stream, p->big_done, 0));
   uint64_t st[8];
-  BMS2_HIP(hipMemcpyAsync(st, p->d_big_stats, sizeof(st), hipMemcpyDeviceToHost, s));
-  BMS2_HIP(hipStreamSynchronize(s));
-  stats[0] += count[0] + count[1];
+  BMS2_HIP(hipMemcpyAsync(st, p->d_big_stats, sizeof(st), hipMemcpyDeviceToHost, p->side_q->stream));
+  BMS2_HIP(hipStreamSynchronize(p->side_q->stream));
+  stats[0] += p->big_pairs;
   for (int k = 1; k < 4; ++k) stats[k] += st[k];
   return BMS2_OK;
 }
@@ -1451,8 +1492,8 @@ int bind_queries(bms2_pipeline_s* p, const bms2_device_csr* query) {
   if (p->num_big) {
     const uint64_t dn = p->big.total_nodes;
     const uint64_t K = std::max<uint32_t>(p->opt.refinement_steps, 1);
-    if ((rc = bms2_path_signature_create(p->q, dn, Qn, &p->big_path)) ||
-        (rc = bms2_cycle_signature_create(p->q, dn, Qn, &p->big_cyc)) ||
+    if ((rc = bms2_path_signature_create(p->side_q, dn, Qn, &p->big_path)) ||
+        (rc = bms2_cycle_signature_create(p->side_q, dn, Qn, &p->big_cyc)) ||
         (rc = device_alloc((void**)&p->d_big_qlev, 8 * K * std::max<uint64_t>(Qn, 1))) ||
         (rc = device_alloc((void**)&p->d_big_qrec, 4ull * kBigRecDwords * std::max<uint64_t>(Qn, 1))))
       return rc;
@@ -1624,6 +1665,7 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
     hipLaunchKernelGGL(query_pre_kernel, dim3((Q + 255) / 256), dim3(256), 0, s, p->d_qhead,
                        p->d_qrec, Q, p->d_qpre);
     BMS2_CHECK_LAUNCH("query_pre_kernel");
+    if (p->num_big) BMS2_HIP(hipEventRecord(p->big_fork, s));  // the big-graph path's inputs
     if (int krc = ks.begin()) return krc;
     // kernels of one chunk of packs [p0, p1)
     auto launch_chunk = [&](uint32_t p0, uint32_t p1) -> int {
@@ -1650,6 +1692,7 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
       return BMS2_OK;
     };
     const bool tracing = tr.domsum || tr.qnodes || tr.pair_nodes;
+    bool big_started = false;
     uint32_t p0 = 0;
     while (p0 < pack_end) {
       // chunk of packs sized so the expected records of both classes and the segments fit
@@ -1686,6 +1729,12 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
       } else if (int lrc = launch_chunk(p0, p1)) {
         return lrc;
       }
+      if (!big_started) {
+        // the graphs above 64 nodes, beside the first chunk (its rows of the outputs are
+        // disjoint from theirs)
+        big_started = true;
+        if (int brc = big_start(p, d_counts, d_valid, g_limit)) return brc;
+      }
       uint64_t ctl[kCtlWords];
       BMS2_HIP(hipMemcpyAsync(ctl, p->d_ctl, sizeof(ctl), hipMemcpyDeviceToHost, s));
       BMS2_HIP(hipStreamSynchronize(s));
@@ -1715,7 +1764,9 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
       for (int k = 0; k < 4; ++k) stats[k] += ctl[kCtlPairs + k];
       p0 = p1;
     }
-    if (int brc = run_big(p, d_counts, d_valid, g_limit, stats)) return brc;
+    if (!big_started)
+      if (int brc = big_start(p, d_counts, d_valid, g_limit)) return brc;
+    if (int brc = big_finish(p, stats)) return brc;
     return ks.end();
   });
 }

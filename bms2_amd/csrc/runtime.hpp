@@ -93,18 +93,22 @@ const uint64_t* cycle_signature_values(bms2_cycle_signature s, int scope);
 constexpr int kBigMaxLevels = 8;   // refinement steps of the screen's shell levels
 constexpr int kBigRecDwords = 48;  // screening record of one query node: 16 + 4 x kBigMaxLevels
 // Control words (u64) shared by the screen and the join: [c] entries of size class c,
-// [kBigClasses + c] unused, [2 kBigClasses + c] join cursors, then record words and overflow.
+// [kBigClasses + c] runs of class c, [2 kBigClasses + c] join cursors, then record words and
+// overflow.
 constexpr int kBigCtlRecWords = 3 * kBigClasses;
 constexpr int kBigCtlOverflow = kBigCtlRecWords + 1;
 constexpr int kBigCtlWords = kBigCtlOverflow + 1;
 
-// GMCR entries of the big-graph screen: entry c cap_e + i is the i-th pair of size class c.
+// GMCR entries of the big-graph screen: entry c cap_e + i is the i-th pair of size class c; runs
+// of consecutive entries of one data graph (run c cap_r + i is class c's i-th) are the join's
+// units of work.
 struct BigEntries {
   uint32_t* gidx;            // query of each entry
-  uint32_t* egraph;          // data graph (index in the batch of big graphs) of each entry
   uint64_t* rec_off;         // first word of the entry's domains in recs
   uint64_t cap_e;            // entries per class
-  uint64_t* recs;            // domains: the class's row words per query node, by local id
+  uint4* runs;               // {graph (index in the batch of big graphs), entries, first entry}
+  uint64_t cap_r;            // runs per class
+  uint64_t* recs;            // domains: the class's row words per depth of the matching order
   uint64_t cap_words;
   unsigned long long* ctl;   // kBigCtlWords control words
 };
@@ -112,10 +116,11 @@ struct BigEntries {
 // Shell signatures of levels 1..K of every node of g into out [K][total_nodes].
 int big_levels(hipStream_t st, const bms2_device_csr* g, uint32_t K, uint64_t* out);
 // Screening records (kBigRecDwords per query node, at slot qpos[u]) from the query batch's path
-// signatures, ring counts and shell levels [K][Qn].
+// signatures, ring counts and shell levels [K][Qn]; the screen writes a node's domain at its
+// depth in the matching order jorder ([Q][32], null = identity).
 int big_query_records(hipStream_t st, const bms2_device_csr* query, const uint64_t* qpath,
                       const uint64_t* qcyc, const uint64_t* qlev, uint32_t K, const uint32_t* qpos,
-                      uint32_t* out);
+                      const uint8_t* jorder, uint32_t* out);
 // Screen of the graphs narrow[0 .. n_narrow) (at most 128 nodes) and wide[0 .. n_wide) (batch
 // indices) against every query (prefilter records qpre, pre_dwords u32 each; first record and
 // node count in words 17 and 18); e.ctl must be zeroed.  Writes the graphs' GMCR bit rows
@@ -125,14 +130,17 @@ int big_screen(hipStream_t st, const bms2_device_csr* data, const uint64_t* dpat
                uint32_t n_narrow, const uint32_t* wide, uint32_t n_wide, uint32_t wide_above,
                uint32_t Q, const uint32_t* qpre, uint32_t pre_dwords, const uint32_t* bqrec,
                const BigEntries& e, const uint32_t* gid, uint64_t* valid);
-// Joins of the screen's entries (class c: count[c] entries), the staged join's kernel reading the
-// entries' domains from their records: each pair's count is added into d_counts[d_gid[g] Q + q]
-// (zeroed), and the matched pairs, embeddings and DFS nodes into d_stats[1..3].
+// Joins of the screen's entries (class c: count[c] entries in runs[c] runs), the staged join's
+// kernel reading the entries' domains from their records: each pair's count is added into
+// d_counts[d_gid[g] Q + q] (zeroed), and the matched pairs, embeddings and DFS nodes into
+// d_stats[1..3].  Enqueued on q; the 4-word class runs on q_wide (nullable) beside the 2-word
+// class when both have entries, and q waits for it.
 uint64_t big_join_scratch_bytes(const bms2_device_csr* query, const bms2_device_csr* data,
                                 const BigJoinShape* shape);
-int big_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
-             const BigEntries& e, const uint64_t count[kBigClasses], int find_first,
-             const uint8_t* d_jorder, uint64_t* d_counts, const uint32_t* d_gid, uint64_t* d_stats,
+int big_join(bms2_queue q, bms2_queue q_wide, const bms2_device_csr* query,
+             const bms2_device_csr* data, const BigEntries& e, const uint64_t count[kBigClasses],
+             const uint64_t runs[kBigClasses], int find_first, const uint8_t* d_jorder,
+             uint64_t* d_counts, const uint32_t* d_gid, uint64_t* d_stats,
              const BigJoinShape* shape);
 // Size split of the graphs above 64 nodes under a launch shape: graphs above this many nodes take
 // 4-word adjacency rows and domains, the others 2.

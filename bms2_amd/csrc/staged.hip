@@ -530,75 +530,75 @@ __global__ void adj_rows_wide_kernel(const uint32_t* __restrict__ go, uint32_t n
   for (uint32_t w = 0; w < RW; ++w) rows[u * RW + w] = r[w];
 }
 
-// Sorts the GMCR entries into the size classes, one wave per data graph, 64 entries at a time
-// (one atomic per class and step).  Pass 0 counts per class into ctl[c]; pass 1 writes
-// {entry | graph << 32} into list at the class's offset (the prefix of the counts) plus the
-// class's fill counter ctl[kBigClasses + c].  Entries of one graph stay together, in order.
+// Runs of the GMCR entries on graphs above 64 nodes, by size class (big_class): one run per data
+// graph with entries, {graph, entries, first entry, 0}, one thread per graph.  Pass 0 counts the
+// runs of each class into ctl[kBigClasses + c]; pass 1 writes them after the earlier classes'
+// runs, in any order (ctl[c] is the fill counter).
 __global__ __launch_bounds__(256) void big_classify_kernel(
     const uint32_t* __restrict__ dgo, uint32_t D, const uint32_t* __restrict__ goff, int pass,
-    uint32_t wide_above, unsigned long long* __restrict__ ctl, uint64_t* __restrict__ list) {
-  const uint32_t lane = lane_id();
-  const uint64_t g = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / 64;
+    uint32_t wide_above, unsigned long long* __restrict__ ctl, uint4* __restrict__ runs) {
+  const uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   if (g >= D) return;
-  const int c = big_class(dgo[g + 1] - dgo[g], wide_above);
   const uint32_t k0 = goff[g], k1 = goff[g + 1];
   if (k0 == k1) return;
-  uint64_t at = 0;
-  if (lane == 0)
-    at = atomicAdd(ctl + (pass == 0 ? c : kBigClasses + c), (unsigned long long)(k1 - k0));
-  if (pass == 0) return;
-  at = uniform64(readlane64(at, 0));
+  const int c = big_class(dgo[g + 1] - dgo[g], wide_above);
+  if (pass == 0) {
+    atomicAdd(ctl + kBigClasses + c, 1ull);
+    return;
+  }
   uint64_t base = 0;
-  for (int cc = 0; cc < c; ++cc) base += ctl[cc];
-  for (uint32_t k = k0 + lane; k < k1; k += 64) list[base + at + (k - k0)] = uint64_t(k) | (g << 32);
+  for (int cc = 0; cc < c; ++cc) base += ctl[kBigClasses + cc];
+  const uint64_t i = atomicAdd(ctl + c, 1ull);
+  runs[base + i] = make_uint4(uint32_t(g), k1 - k0, k0, 0u);
 }
 
 // The staged join for data graphs of 65 to 64 NW nodes (JoinCandidates3Optimized,
 // isomorphism.hpp:309-468, with standard backtracking over bitsets of the graph's local ids;
-// identity matching order unless jorder gives one, DFS nodes per pair when asked):
-// persistent waves take GMCR entries of one size class 256 at a time from a global cursor and
-// hand them to idle lanes.  At the start of a pair a lane loads every depth's domain (its words
-// of the candidate rows, loads of four depths in flight) into its stash in global memory.  The
-// search keeps, for the current depth, the remaining candidates R and X = (domain of the next
-// depth) minus the nodes matched at earlier depths, intersected with their adjacency
-// constraints, in registers; a step reads one adjacency row (NW words) and gives the next
-// depth's candidates as X & ~{v} & (adjacent ? N(v) : ~N(v)).  A descent pushes R to the stash
-// and builds the new X from the stashed domain and the rows of the nodes matched so far (loads
-// in flight together); a backtrack pops R and rebuilds X the same way.
-// Work sharing (count-all mode): once a wave has no entries left and at least kShareIdle of its
+// identity matching order unless jorder gives one, DFS nodes per pair when asked).  Persistent
+// waves take runs of entries (consecutive GMCR entries of one data graph) from a global cursor,
+// copy the run's graph's adjacency rows (NW u64 words per node) into their LDS and hand the
+// run's entries to idle lanes.  At the start of a pair a lane loads every depth's domain (loads
+// of four depths in flight) into its stash in global memory.  The search keeps, for the current
+// depth, the remaining candidates R and X = (domain of the next depth) minus the nodes matched
+// at earlier depths, intersected with their adjacency constraints, in registers; a step reads
+// one adjacency row from LDS and gives the next depth's candidates as
+// X & ~{v} & (adjacent ? N(v) : ~N(v)).  A descent pushes R to the stash and builds the new X
+// from the stashed domain and the LDS rows of the nodes matched so far; a backtrack pops R and
+// rebuilds X the same way.
+// Work sharing (count-all mode): once a run's entries are handed out and at least kShareIdle
 // lanes are idle, busy lanes give work to idle ones.  A donor gives all remaining candidates of
 // its shallowest depth that has any (or, when only its current depth has, the upper half of
 // them); the receiver copies the donor's matched nodes above that depth and searches the given
 // candidates' subtrees as a unit based at that depth.  A pair's units add their counts and DFS
 // nodes; the sums equal the serial search's, since every subtree is searched exactly once.  A
-// heavy pair so spreads over the wave's 64 lanes instead of holding one lane while the chip
-// idles.  First-match mode never shares: its DFS-node count depends on the search order.
+// heavy pair so spreads over the wave's 64 lanes instead of holding one lane while the others
+// idle.  First-match mode never shares: its DFS-node count depends on the search order.
 
 struct BigJoinParams {
-  const uint64_t* cand;
+  const uint64_t* cand;         // candidate rows (entries of a GMCR), or null with recs
   uint64_t W;
   const uint32_t* qgo;
   const uint32_t* qa;
-  const uint32_t* gidx;
+  const uint32_t* chain;        // per query: bit d = depths d and d - 1 adjacent
+  const uint32_t* gidx;         // query of each entry
   const uint32_t* dgo;
-  const uint64_t* rows;
+  const uint64_t* rows;         // adjacency rows, RW words per node
   uint32_t RW;
-  const uint64_t* list;         // GMCR entries {entry | graph << 32} sorted by class
-  unsigned long long* ctl;      // class counts, class fill counters, class cursors
+  const uint4* runs;            // {graph, entries, first entry (low, high)}
+  uint64_t run_stride;          // class c's runs start at c run_stride, or (0) after the
+                                // earlier classes' runs
+  unsigned long long* ctl;      // [kBigClasses + c] runs of class c, [2 kBigClasses + c] cursors
   uint64_t* stash;
   uint32_t SD;                  // stash depth
   int find_first;
   uint64_t* total;              // nullable
-  uint64_t* per_pair;           // nullable, zeroed: count per GMCR entry (find_first: 0/1)
+  uint64_t* per_pair;           // nullable, zeroed: count per entry (find_first: 0/1)
   const uint8_t* jorder;
   uint64_t* per_pair_nodes;     // nullable, zeroed
-  // entries of the fused pipeline's big-graph screen (recs non-null): class c's entries are
-  // c class_stride + i for i < ctl[c], with their graph in egraph and their domains (NW words per
-  // query node, by local id) at recs + rec_off; list is unused
-  const uint32_t* egraph;
+  // domains of the fused pipeline's big-graph screen (non-null: cand and the stash's domain
+  // half are unused): NW words per depth of the matching order at recs + rec_off[entry]
   const uint64_t* rec_off;
   const uint64_t* recs;
-  uint64_t class_stride;
   // outputs of the fused pipeline (nullable): counts[gid[g] Q + q] (zeroed) and the statistics
   // {matched pairs, embeddings, DFS nodes} added at stats[1..3]
   uint64_t* out_counts;
@@ -608,9 +608,9 @@ struct BigJoinParams {
 };
 
 constexpr int kShareIdle = 16;  // idle lanes of a wave that start a round of work sharing
-// Entries are taken in the order i -> i x (kSpread mod T) mod T, a bijection of [0, T) for every
-// T < 2^32 (kSpread is a prime above 2^32): a claim of consecutive numbers then holds entries of
-// many data graphs rather than a run of one graph's, whose heavy pairs would share one wave.
+// Runs are taken in the order i -> i x (kSpread mod T) mod T, a bijection of [0, T) for every
+// T < 2^32 (kSpread is a prime above 2^32): consecutive claims then reach graphs far apart in
+// the batch rather than neighbours of similar size and shape.
 constexpr uint64_t kSpread = 4294967311ull;
 
 // Position of the k-th (from 0) set bit of m; m has more than k set bits.
@@ -625,57 +625,50 @@ __device__ inline uint32_t nth_bit(uint64_t m, uint32_t k) {
 template <int NW, int BW>  // BW: waves per workgroup
 __global__ __launch_bounds__(64 * BW) void join_big_kernel(BigJoinParams p, int cls) {
   constexpr int NQ = kBigDepth;
-  __shared__ uint8_t F[NQ][64 * BW];  // node matched per depth, per thread of the workgroup
+  __shared__ uint8_t F[NQ][64 * BW];         // node matched per depth, per thread
+  __shared__ uint64_t rows_lds[BW][64 * NW * NW];  // the current run's graph: NW words per node
   const uint32_t lane = lane_id();
   const uint32_t tid = threadIdx.x;
   const uint32_t wbase = tid - lane;  // the wave's first thread
+  uint64_t* const RL = rows_lds[tid / 64];
   unsigned long long* const cursor = p.ctl + 2 * kBigClasses + cls;
-  uint64_t off = 0;
-  if (p.recs)
-    off = uint64_t(cls) * p.class_stride;
+  uint64_t rbase = 0;
+  if (p.run_stride)
+    rbase = uint64_t(cls) * p.run_stride;
   else
-    for (int c = 0; c < cls; ++c) off += p.ctl[c];
-  const uint64_t T = p.ctl[cls];
-  const uint64_t spread = T > 1 ? kSpread % T : 1;
-  const uint64_t* L = p.list + off;
+    for (int c = 0; c < cls; ++c) rbase += p.ctl[kBigClasses + c];
+  const uint64_t NR = p.ctl[kBigClasses + cls];
+  const uint64_t spread = NR > 1 ? kSpread % NR : 1;
+  const uint4* const runs = p.runs + rbase;
   // this lane's stash: the domains [SD][NW], then the pushed R [SD][NW] (SD >= the batch's
   // largest query)
   uint64_t* const DS = p.stash + (uint64_t(blockIdx.x) * 64 * BW + tid) * (2 * p.SD * NW);
   uint64_t* const RSt = DS + p.SD * NW;
+  const uint64_t* dom = DS;  // domains by depth: the stash, or the pair's record
   const bool share = !p.find_first;
-  bool busy = false, exhausted = false;
+  bool busy = false;
   uint64_t t = 0, c = 0, nodes = 0, sum = 0, matched = 0, dfs = 0;
   uint64_t R[NW], X[NW], used[NW];
-  uint32_t s = 0, q = 0, nq = 0, g = 0;
-  uint32_t pend = 0;  // depths below the current one whose pushed R is not empty
+  uint32_t s = 0, e = 0, g = 0, q = 0, nq = 0, chain = 0;
+
+  uint32_t pend = 0;      // depths below the current one whose pushed R is not empty
   int depth = 0, bd = 0;  // current depth, depth the lane's unit starts at
-  uint64_t next = 0, end = 0;
-  constexpr uint64_t kClaim = 256;
   constexpr int kHandout = 32;
   // X of depth d: the domain of depth d + 1 minus `used`, constrained by the nodes matched at
-  // depths < d: their rows are loaded four at a time, except the row of depth d - 1 when the
-  // caller has it (last_row)
+  // depths < d (rows from LDS), the row of depth d - 1 passed in when the caller has it
   auto build_x = [&](int d, const uint64_t* last_row, uint64_t* out) {
     const uint32_t am = p.qa[size_t(q) * 32 + d + 1];
 #pragma unroll
-    for (int k = 0; k < NW; ++k) out[k] = DS[(d + 1) * NW + k] & ~used[k];
+    for (int k = 0; k < NW; ++k) out[k] = dom[(d + 1) * NW + k] & ~used[k];
     const int lim = last_row ? d - 1 : d;
-    for (int i0 = 0; i0 < lim; i0 += 4) {
-      uint64_t r4[4][NW];
+    for (int i = 0; i < lim; ++i) {
+      const uint32_t fi = F[i][tid];
+      const bool a = (am >> i) & 1;
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (i0 + j < lim) {
-          const uint32_t fi = F[i0 + j][tid];
-#pragma unroll
-          for (int k = 0; k < NW; ++k) r4[j][k] = p.rows[uint64_t(s + fi) * p.RW + k];
-        }
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (i0 + j < lim) {
-          const bool a = (am >> (i0 + j)) & 1;
-#pragma unroll
-          for (int k = 0; k < NW; ++k) out[k] &= a ? r4[j][k] : ~r4[j][k];
-        }
+      for (int k = 0; k < NW; ++k) {
+        const uint64_t r = RL[fi * NW + k];
+        out[k] &= a ? r : ~r;
+      }
     }
     if (last_row) {
       const bool a = (am >> (d - 1)) & 1;
@@ -687,7 +680,20 @@ __global__ __launch_bounds__(64 * BW) void join_big_kernel(BigJoinParams p, int 
   // time; at a pair's start also R (depth 0's domain) and X (depth 1's: nq >= 2, single-node
   // queries are never GMCR pairs)
   auto load_domains = [&](bool start) {
-    const uint32_t qb = p.qgo[q], e = p.dgo[g + 1];
+    chain = p.chain[q];
+    if (p.recs) {
+      dom = p.recs + p.rec_off[t];
+      if (start) {
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+          R[k] = dom[k];
+          X[k] = dom[NW + k];
+        }
+      }
+      return;
+    }
+    dom = DS;
+    const uint32_t qb = p.qgo[q];
     for (uint32_t i0 = 0; i0 < nq; i0 += 4) {
       uint64_t d4[4][NW];
 #pragma unroll
@@ -722,212 +728,213 @@ __global__ __launch_bounds__(64 * BW) void join_big_kernel(BigJoinParams p, int 
     }
   };
   while (true) {
-    const uint64_t idle = ballot(!busy);
-    if (popc64(idle) >= kHandout || idle == ~uint64_t(0)) {
-      if (next >= end && !exhausted) {
-        uint64_t base = 0;
-        if (lane == 0) base = atomicAdd(cursor, (unsigned long long)kClaim);
-        base = uniform64(readlane64(base, 0));
-        next = base;
-        end = base >= T ? base : min(base + kClaim, T);
-        if (base + kClaim >= T) exhausted = true;
-      }
-      const uint64_t take = min(end - next, uint64_t(popc64(idle)));
-      const uint64_t rank = uint64_t(popc64(idle & low_mask(lane)));
-      if (!busy && rank < take) {
-        busy = true;
-        const uint64_t pos = (next + rank) * spread % T;
-        if (p.recs) {
-          t = off + pos;
-          g = p.egraph[t];
-        } else {
-          const uint64_t ent = L[pos];
-          t = ent & 0xffffffffu;
-          g = uint32_t(ent >> 32);
+    uint64_t ri = 0;
+    if (lane == 0) ri = atomicAdd(cursor, 1ull);
+    ri = uniform64(readlane64(ri, 0));
+    if (ri >= NR) break;
+    const uint4 run = runs[ri * spread % NR];
+    g = run.x;
+    const uint32_t cnt = run.y;
+    const uint64_t first = uint64_t(run.z) | (uint64_t(run.w) << 32);
+    s = p.dgo[g];
+    e = p.dgo[g + 1];
+    // the run's graph's rows into LDS (the lanes finished the previous run's searches)
+    for (uint32_t w = lane; w < (e - s) * NW; w += 64)
+      RL[w] = p.rows[uint64_t(s + w / NW) * p.RW + w % NW];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    uint32_t next = 0;
+    while (true) {
+      const uint64_t idle = ballot(!busy);
+      if (next < cnt && (popc64(idle) >= kHandout || idle == ~uint64_t(0))) {
+        const uint32_t take = min(cnt - next, uint32_t(popc64(idle)));
+        const uint32_t rank = uint32_t(popc64(idle & low_mask(lane)));
+        if (!busy && rank < take) {
+          busy = true;
+          t = first + next + rank;
+          q = p.gidx[t];
+          nq = p.qgo[q + 1] - p.qgo[q];
+          load_domains(true);
+#pragma unroll
+          for (int k = 0; k < NW; ++k) used[k] = 0;
+          depth = bd = 0;
+          pend = 0;
+          c = 0;
+          nodes = 0;
         }
-        q = p.gidx[t];
-        s = p.dgo[g];
-        nq = p.qgo[q + 1] - p.qgo[q];
-        load_domains(true);
-#pragma unroll
-        for (int k = 0; k < NW; ++k) used[k] = 0;
-        depth = bd = 0;
-        pend = 0;
-        c = 0;
-        nodes = 0;
+        next += take;
       }
-      next += take;
-    }
-    if (share && exhausted && next >= end) {
-      const uint64_t free_lanes = ballot(!busy);
-      if (popc64(free_lanes) >= kShareIdle && free_lanes != ~uint64_t(0)) {
-        // a donor's depth: the shallowest with pushed candidates, else its current one when it
-        // holds two or more
-        int d0 = -1;
-        if (busy) {
-          if (pend) {
-            d0 = __builtin_ctz(pend);
-          } else {
-            uint32_t pc = 0;
-#pragma unroll
-            for (int k = 0; k < NW; ++k) pc += uint32_t(popc64(R[k]));
-            if (pc >= 2) d0 = depth;
-          }
-        }
-        const uint64_t donors = ballot(d0 >= 0);
-        const uint32_t npair = min(uint32_t(popc64(donors)), uint32_t(popc64(free_lanes)));
-        if (npair) {
-          uint64_t give[NW];
-#pragma unroll
-          for (int k = 0; k < NW; ++k) give[k] = 0;
-          if (d0 >= 0 && uint32_t(popc64(donors & low_mask(lane))) < npair) {
-            if (d0 < depth) {
-#pragma unroll
-              for (int k = 0; k < NW; ++k) {
-                give[k] = RSt[d0 * NW + k];
-                RSt[d0 * NW + k] = 0;
-              }
-              pend &= ~(1u << d0);
+      if (share && next >= cnt) {
+        const uint64_t free_lanes = ballot(!busy);
+        if (popc64(free_lanes) >= kShareIdle && free_lanes != ~uint64_t(0)) {
+          // a donor's depth: the shallowest with pushed candidates, else its current one when
+          // it holds two or more
+          int d0 = -1;
+          if (busy) {
+            if (pend) {
+              d0 = __builtin_ctz(pend);
             } else {
-              // keep the lower half of the current candidates, give the upper half
               uint32_t pc = 0;
 #pragma unroll
               for (int k = 0; k < NW; ++k) pc += uint32_t(popc64(R[k]));
-              uint32_t keep = pc - pc / 2;
+              if (pc >= 2) d0 = depth;
+            }
+          }
+          const uint64_t donors = ballot(d0 >= 0);
+          const uint32_t npair = min(uint32_t(popc64(donors)), uint32_t(popc64(free_lanes)));
+          if (npair) {
+            uint64_t give[NW];
 #pragma unroll
-              for (int k = 0; k < NW; ++k) {
-                const uint32_t n = uint32_t(popc64(R[k]));
-                if (keep >= n) {
-                  keep -= n;
-                } else {
-                  const uint64_t m = keep ? low_mask(nth_bit(R[k], keep)) : 0;
-                  give[k] = R[k] & ~m;
-                  R[k] &= m;
-                  keep = 0;
+            for (int k = 0; k < NW; ++k) give[k] = 0;
+            if (d0 >= 0 && uint32_t(popc64(donors & low_mask(lane))) < npair) {
+              if (d0 < depth) {
+#pragma unroll
+                for (int k = 0; k < NW; ++k) {
+                  give[k] = RSt[d0 * NW + k];
+                  RSt[d0 * NW + k] = 0;
+                }
+                pend &= ~(1u << d0);
+              } else {
+                // keep the lower half of the current candidates, give the upper half
+                uint32_t pc = 0;
+#pragma unroll
+                for (int k = 0; k < NW; ++k) pc += uint32_t(popc64(R[k]));
+                uint32_t keep = pc - pc / 2;
+#pragma unroll
+                for (int k = 0; k < NW; ++k) {
+                  const uint32_t n = uint32_t(popc64(R[k]));
+                  if (keep >= n) {
+                    keep -= n;
+                  } else {
+                    const uint64_t m = keep ? low_mask(nth_bit(R[k], keep)) : 0;
+                    give[k] = R[k] & ~m;
+                    R[k] &= m;
+                    keep = 0;
+                  }
                 }
               }
             }
-          }
-          // the k-th idle lane takes the k-th donor's gift
-          const uint32_t rr = uint32_t(popc64(free_lanes & low_mask(lane)));
-          const bool recv = !busy && rr < npair;
-          const int src = recv ? int(nth_bit(donors, rr)) : int(lane);
-          const uint64_t st = __shfl(t, src);
-          const uint32_t sg = __shfl(g, src), sq = __shfl(q, src);
-          const int sd = __shfl(d0, src);
-          uint64_t gv[NW];
+            // the k-th idle lane takes the k-th donor's gift (same run: same graph)
+            const uint32_t rr = uint32_t(popc64(free_lanes & low_mask(lane)));
+            const bool recv = !busy && rr < npair;
+            const int src = recv ? int(nth_bit(donors, rr)) : int(lane);
+            const uint64_t st = __shfl(t, src);
+            const uint32_t sq = __shfl(q, src);
+            const int sd = __shfl(d0, src);
+            uint64_t gv[NW];
 #pragma unroll
-          for (int k = 0; k < NW; ++k) gv[k] = __shfl(give[k], src);
-          if (recv) {
-            t = st;
-            g = sg;
-            q = sq;
-            s = p.dgo[g];
-            nq = p.qgo[q + 1] - p.qgo[q];
+            for (int k = 0; k < NW; ++k) gv[k] = __shfl(give[k], src);
+            if (recv) {
+              t = st;
+              q = sq;
+              nq = p.qgo[q + 1] - p.qgo[q];
 #pragma unroll
-            for (int k = 0; k < NW; ++k) used[k] = 0;
-            for (int i = 0; i < sd; ++i) {
-              const uint32_t f = F[i][wbase + uint32_t(src)];
-              F[i][tid] = uint8_t(f);
-              used[f >> 6] |= uint64_t(1) << (f & 63);
+              for (int k = 0; k < NW; ++k) used[k] = 0;
+              for (int i = 0; i < sd; ++i) {
+                const uint32_t f = F[i][wbase + uint32_t(src)];
+                F[i][tid] = uint8_t(f);
+                used[f >> 6] |= uint64_t(1) << (f & 63);
+              }
+              load_domains(false);
+              build_x(sd, nullptr, X);
+#pragma unroll
+              for (int k = 0; k < NW; ++k) R[k] = gv[k];
+              depth = bd = sd;
+              pend = 0;
+              c = 0;
+              nodes = 0;
+              busy = true;
             }
-            load_domains(false);
-            build_x(sd, nullptr, X);
-#pragma unroll
-            for (int k = 0; k < NW; ++k) R[k] = gv[k];
-            depth = bd = sd;
-            pend = 0;
-            c = 0;
-            nodes = 0;
-            busy = true;
           }
         }
       }
-    }
-    if (!ballot(busy)) {
-      if (exhausted && next >= end) break;
-      continue;
-    }
-    bool finished = false;
-    if (busy) {
-      uint64_t any = 0;
+      if (!ballot(busy)) {
+        if (next >= cnt) break;
+        continue;
+      }
+      bool finished = false;
+      if (busy) {
+        uint64_t any = 0;
 #pragma unroll
-      for (int k = 0; k < NW; ++k) any |= R[k];
-      if (!any) {
-        if (depth == bd) {
-          finished = true;
-        } else {
-          --depth;
+        for (int k = 0; k < NW; ++k) any |= R[k];
+        if (!any) {
+          if (depth == bd) {
+            finished = true;
+          } else {
+            --depth;
 #pragma unroll
-          for (int k = 0; k < NW; ++k) R[k] = RSt[depth * NW + k];
-          pend &= ~(1u << depth);
-          const uint32_t f = F[depth][tid];
-          used[f >> 6] &= ~(uint64_t(1) << (f & 63));
-          build_x(depth, nullptr, X);
-        }
-      } else {
-        // lowest remaining candidate v
-        uint32_t v = 0;
-        bool found = false;
-#pragma unroll
-        for (int k = 0; k < NW; ++k)
-          if (!found && R[k]) {
-            v = uint32_t(64 * k + ctz64(R[k]));
-            R[k] &= R[k] - 1;
-            found = true;
+            for (int k = 0; k < NW; ++k) R[k] = RSt[depth * NW + k];
+            pend &= ~(1u << depth);
+            const uint32_t f = F[depth][tid];
+            used[f >> 6] &= ~(uint64_t(1) << (f & 63));
+            build_x(depth, nullptr, X);
           }
-        ++nodes;
-        uint64_t nv[NW];
+        } else {
+          // lowest remaining candidate v
+          uint32_t v = 0;
+          bool found = false;
 #pragma unroll
-        for (int k = 0; k < NW; ++k) nv[k] = p.rows[uint64_t(s + v) * p.RW + k];
-        const int nd = depth + 1;
-        const bool adj = (p.qa[size_t(q) * 32 + nd] >> depth) & 1;
-        uint64_t Y[NW], ny = 0;
+          for (int k = 0; k < NW; ++k)
+            if (!found && R[k]) {
+              v = uint32_t(64 * k + ctz64(R[k]));
+              R[k] &= R[k] - 1;
+              found = true;
+            }
+          ++nodes;
+          uint64_t nv[NW];
 #pragma unroll
-        for (int k = 0; k < NW; ++k) {
-          Y[k] = X[k] & (adj ? nv[k] : ~nv[k]);
-          if (k == int(v >> 6)) Y[k] &= ~(uint64_t(1) << (v & 63));
-          ny |= Y[k];
-        }
-        if (uint32_t(nd) == nq - 1) {
-#pragma unroll
-          for (int k = 0; k < NW; ++k) c += uint64_t(popc64(Y[k]));
-          if (p.find_first && c) finished = true;
-        } else if (ny) {
-          // descend: push R, then X of the new depth (v is matched at `depth` now)
-          uint64_t rest = 0;
+          for (int k = 0; k < NW; ++k) nv[k] = RL[v * NW + k];
+          const int nd = depth + 1;
+          const bool adj = (chain >> nd) & 1;
+          uint64_t Y[NW], ny = 0;
 #pragma unroll
           for (int k = 0; k < NW; ++k) {
-            RSt[depth * NW + k] = R[k];
-            rest |= R[k];
+            Y[k] = X[k] & (adj ? nv[k] : ~nv[k]);
+            if (k == int(v >> 6)) Y[k] &= ~(uint64_t(1) << (v & 63));
+            ny |= Y[k];
           }
-          if (rest) pend |= 1u << depth;
-          F[depth][tid] = uint8_t(v);
-          used[v >> 6] |= uint64_t(1) << (v & 63);
-          build_x(nd, nv, X);
+          if (uint32_t(nd) == nq - 1) {
 #pragma unroll
-          for (int k = 0; k < NW; ++k) R[k] = Y[k];
-          depth = nd;
+            for (int k = 0; k < NW; ++k) c += uint64_t(popc64(Y[k]));
+            if (p.find_first && c) finished = true;
+          } else if (ny) {
+            // descend: push R, then X of the new depth (v is matched at `depth` now)
+            uint64_t rest = 0;
+#pragma unroll
+            for (int k = 0; k < NW; ++k) {
+              RSt[depth * NW + k] = R[k];
+              rest |= R[k];
+            }
+            if (rest) pend |= 1u << depth;
+            F[depth][tid] = uint8_t(v);
+            used[v >> 6] |= uint64_t(1) << (v & 63);
+            build_x(nd, nv, X);
+#pragma unroll
+            for (int k = 0; k < NW; ++k) R[k] = Y[k];
+            depth = nd;
+          }
         }
       }
-    }
-    if (finished) {
-      const uint64_t cc = p.find_first ? (c ? 1 : 0) : c;
-      if (p.out_counts) {
-        if (cc) {
-          const unsigned long long old = atomicAdd(
-              reinterpret_cast<unsigned long long*>(p.out_counts + uint64_t(p.out_gid[g]) * p.Q + q),
-              (unsigned long long)cc);
-          if (!old) ++matched;
+      if (finished) {
+        const uint64_t cc = p.find_first ? (c ? 1 : 0) : c;
+        if (p.out_counts) {
+          if (cc) {
+            const unsigned long long old = atomicAdd(
+                reinterpret_cast<unsigned long long*>(p.out_counts + uint64_t(p.out_gid[g]) * p.Q + q),
+                (unsigned long long)cc);
+            if (!old) ++matched;
+          }
+        } else if (p.per_pair && cc) {
+          atomicAdd(reinterpret_cast<unsigned long long*>(p.per_pair + t), (unsigned long long)cc);
         }
-      } else if (p.per_pair && cc) {
-        atomicAdd(reinterpret_cast<unsigned long long*>(p.per_pair + t), (unsigned long long)cc);
+        if (p.per_pair_nodes && nodes)
+          atomicAdd(reinterpret_cast<unsigned long long*>(p.per_pair_nodes + t),
+                    (unsigned long long)nodes);
+        sum += cc;
+        dfs += nodes;
+        busy = false;
       }
-      if (p.per_pair_nodes && nodes)
-        atomicAdd(reinterpret_cast<unsigned long long*>(p.per_pair_nodes + t), (unsigned long long)nodes);
-      sum += cc;
-      dfs += nodes;
-      busy = false;
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -984,6 +991,18 @@ __global__ void join_depth_adjacency_kernel(const uint32_t* __restrict__ qgo,
     a |= 1u << j;
   }
   qa[x] = a;
+}
+
+// Per query, bit d set when the query nodes matched at depths d and d - 1 are adjacent (the
+// test of every DFS step), from the depth adjacency qa; one thread per query.
+__global__ void depth_chain_kernel(const uint32_t* __restrict__ qgo, uint32_t Q,
+                                   const uint32_t* __restrict__ qa, uint32_t* __restrict__ chain) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= Q) return;
+  const uint32_t nq = qgo[q + 1] - qgo[q];
+  uint32_t m = 0;
+  for (uint32_t d = 1; d < nq && d < 32; ++d) m |= ((qa[size_t(q) * 32 + d] >> (d - 1)) & 1u) << d;
+  chain[q] = m;
 }
 
 // The staged join for data graphs of up to 64 nodes (JoinCandidates3Optimized,
@@ -1548,11 +1567,14 @@ uint64_t join_scratch_bytes(const bms2_device_csr* query, const bms2_device_csr*
   // depth-adjacency words per query
   if (data->max_graph_nodes <= 64) return 8 * (nrows + 1) + qa;
   // larger graphs: rows of 2 or 4 words per node, the depth adjacency, class counters and
-  // cursors, the GMCR entries sorted by size class (8 bytes each), the lanes' stashes
+  // cursors (padded to 16 bytes), one run of entries per data graph (16 bytes each), the lanes'
+  // stashes
+  (void)T;
   const uint64_t rw = rw_above ? rw_above : data->max_graph_nodes > 128 ? 4 : 2;
   const uint64_t stash = uint64_t(cus) * waves * 64 * 2 * stash_depth(query) * rw * 8;
-  return 8 * rw * nrows + (qa + 7) / 8 * 8 + 8 * 3 * kBigClasses + 8 * std::max<uint64_t>(T, 1) +
-         stash;
+  const uint64_t qac = qa + 4 * uint64_t(query->num_graphs);  // and the depth chains
+  const uint64_t head = (8 * rw * nrows + (qac + 7) / 8 * 8 + 8 * 3 * kBigClasses + 15) / 16 * 16;
+  return head + 16 * std::max<uint64_t>(data->num_graphs, 1) + stash;
 }
 
 }  // namespace
@@ -1636,34 +1658,39 @@ int staged_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_cs
                        (unsigned long long)scratch_bytes);
   }
   auto* qa = reinterpret_cast<uint32_t*>(rows + RW * nrows);
-  auto* ctl = reinterpret_cast<unsigned long long*>(rows + RW * nrows + (nqa + 1) / 2);
-  uint64_t* list = reinterpret_cast<uint64_t*>(ctl + 3 * kBigClasses);
-  uint64_t* stash = list + std::max<uint64_t>(T, 1);
+  uint32_t* chain = qa + nqa;
+  const uint64_t nqac = nqa + query->num_graphs;
+  auto* ctl = reinterpret_cast<unsigned long long*>(rows + RW * nrows + (nqac + 1) / 2);
+  const uint64_t head_words = (RW * nrows + (nqac + 1) / 2 + 3 * kBigClasses + 1) / 2 * 2;
+  auto* runs = reinterpret_cast<uint4*>(rows + head_words);
+  uint64_t* stash = rows + head_words + 2 * std::max<uint64_t>(data->num_graphs, 1);
   const uint32_t SD = stash_depth(query);
   if (nqa)
     hipLaunchKernelGGL(join_depth_adjacency_kernel, dim3(grid_for(nqa, 256)), dim3(256), 0, st,
                        query->graph_offsets, query->row_offsets, query->column_indices,
                        query->num_graphs, d_jorder, qa);
+  if (query->num_graphs)
+    hipLaunchKernelGGL(depth_chain_kernel, dim3(grid_for(query->num_graphs, 256)), dim3(256), 0, st,
+                       query->graph_offsets, query->num_graphs, qa, chain);
   hipLaunchKernelGGL(fill_zero_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<uint64_t*>(ctl),
                      uint64_t(3 * kBigClasses));
   hipLaunchKernelGGL(adj_rows_wide_kernel, dim3(grid_for(data->total_nodes, 256, 1u << 30)),
                      dim3(256), 0, st, data->graph_offsets, data->num_graphs, data->row_offsets,
                      data->column_indices, data->total_nodes, RW, rows);
   for (int pass = 0; pass < 2; ++pass)
-    hipLaunchKernelGGL(big_classify_kernel,
-                       dim3(grid_for(uint64_t(data->num_graphs) * 64, 256, 1u << 30)), dim3(256),
-                       0, st, data->graph_offsets, data->num_graphs, gmcr->offsets, pass,
-                       wide_above, ctl, list);
+    hipLaunchKernelGGL(big_classify_kernel, dim3(grid_for(data->num_graphs, 256, 1u << 30)),
+                       dim3(256), 0, st, data->graph_offsets, data->num_graphs, gmcr->offsets,
+                       pass, wide_above, ctl, runs);
   BMS2_CHECK_LAUNCH("big_classify_kernel");
   // units of one pair add their counts and DFS nodes
   if (d_pair_counts)
     hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(T, 256, 4096)), dim3(256), 0, st, d_pair_counts, T);
   if (d_pair_nodes)
     hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(T, 256, 4096)), dim3(256), 0, st, d_pair_nodes, T);
-  BigJoinParams bp{cand->bits, cand->W, query->graph_offsets, qa, gmcr->indices,
-                   data->graph_offsets, rows, RW, list, ctl, stash, SD, find_first, d_total,
-                   d_pair_counts, d_jorder, d_pair_nodes, nullptr, nullptr, nullptr, 0,
-                   nullptr, nullptr, 0, nullptr};
+  BigJoinParams bp{cand->bits, cand->W, query->graph_offsets, qa, chain, gmcr->indices,
+                   data->graph_offsets, rows, RW, runs, 0, ctl, stash, SD, find_first, d_total,
+                   d_pair_counts, d_jorder, d_pair_nodes, nullptr, nullptr, nullptr, nullptr, 0,
+                   nullptr};
   const dim3 grid(uint32_t(cus) * waves / bw), block(64 * bw);
   auto launch = [&](auto kern, int cls) { hipLaunchKernelGGL(kern, grid, block, 0, st, bp, cls); };
   if (bw == 4) {
@@ -1701,13 +1728,14 @@ uint64_t big_join_scratch_bytes(const bms2_device_csr* query, const bms2_device_
   const uint32_t wide_above = big_wide_above(shape ? *shape : BigJoinShape{});
   const uint64_t RW = data->max_graph_nodes > wide_above ? 4 : 2;
   const uint64_t nrows = std::max<uint64_t>(1, data->total_nodes);
-  const uint64_t qa = (4 * 32 * uint64_t(query->num_graphs) + 7) / 8 * 8;
+  const uint64_t qa = (4 * 33 * uint64_t(query->num_graphs) + 7) / 8 * 8;  // with the chains
   return 8 * RW * nrows + qa + uint64_t(device_cus()) * waves * 64 * 2 * stash_depth(query) * RW * 8;
 }
 
-int big_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* data,
-             const BigEntries& e, const uint64_t count[kBigClasses], int find_first,
-             const uint8_t* d_jorder, uint64_t* d_counts, const uint32_t* d_gid, uint64_t* d_stats,
+int big_join(bms2_queue q, bms2_queue q_wide, const bms2_device_csr* query,
+             const bms2_device_csr* data, const BigEntries& e, const uint64_t count[kBigClasses],
+             const uint64_t runs[kBigClasses], int find_first, const uint8_t* d_jorder,
+             uint64_t* d_counts, const uint32_t* d_gid, uint64_t* d_stats,
              const BigJoinShape* shape) {
   if (!count[0] && !count[1]) return BMS2_OK;
   hipStream_t st = q->stream;
@@ -1722,23 +1750,50 @@ int big_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* 
   uint64_t* rows = nullptr;
   if (int rc = queue_scratch(q, big_join_scratch_bytes(query, data, shape), (void**)&rows)) return rc;
   auto* qa = reinterpret_cast<uint32_t*>(rows + RW * nrows);
-  uint64_t* stash = rows + RW * nrows + (nqa + 1) / 2;
-  if (nqa)
+  uint32_t* chain = qa + nqa;
+  uint64_t* stash = rows + RW * nrows + (nqa + query->num_graphs + 1) / 2;
+  if (nqa) {
     hipLaunchKernelGGL(join_depth_adjacency_kernel, dim3(grid_for(nqa, 256)), dim3(256), 0, st,
                        query->graph_offsets, query->row_offsets, query->column_indices,
                        query->num_graphs, d_jorder, qa);
+    hipLaunchKernelGGL(depth_chain_kernel, dim3(grid_for(query->num_graphs, 256)), dim3(256), 0, st,
+                       query->graph_offsets, query->num_graphs, qa, chain);
+  }
   hipLaunchKernelGGL(adj_rows_wide_kernel, dim3(grid_for(data->total_nodes, 256, 1u << 30)),
                      dim3(256), 0, st, data->graph_offsets, data->num_graphs, data->row_offsets,
                      data->column_indices, data->total_nodes, RW, rows);
   BMS2_CHECK_LAUNCH("adj_rows_wide_kernel");
-  BigJoinParams bp{nullptr, 0, query->graph_offsets, qa, e.gidx, data->graph_offsets, rows, RW,
-                   nullptr, e.ctl, stash, stash_depth(query), find_first, nullptr, nullptr,
-                   d_jorder, nullptr, e.egraph, e.rec_off, e.recs, e.cap_e, d_counts, d_gid,
-                   query->num_graphs, d_stats};
-  const dim3 grid(uint32_t(device_cus()) * waves / bw), block(64 * bw);
+  BigJoinParams bp{nullptr, 0, query->graph_offsets, qa, chain, e.gidx, data->graph_offsets, rows, RW,
+                   e.runs, e.cap_r, e.ctl, stash, stash_depth(query), find_first, nullptr, nullptr,
+                   d_jorder, nullptr, e.rec_off, e.recs, d_counts, d_gid, query->num_graphs,
+                   d_stats};
+  const uint32_t blocks = uint32_t(device_cus()) * waves / bw;
+  const dim3 block(64 * bw);
+  // the 4-word class, when both classes have entries and a second queue is given, runs beside
+  // the 2-word class: its few graphs' long searches overlap the other class's instead of
+  // following them; one workgroup per run at most, its stashes in that queue's scratch
+  const bool beside = q_wide && count[0] && count[1];
+  BigJoinParams wp = bp;
+  uint32_t wide_blocks = blocks;
+  hipEvent_t fork = nullptr, done = nullptr;
+  if (beside) {
+    wide_blocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(blocks, (runs[1] + bw - 1) / bw)));
+    const uint64_t bytes = uint64_t(wide_blocks) * bw * 64 * 2 * stash_depth(query) * 4 * 8;
+    if (int rc = queue_scratch(q_wide, bytes, (void**)&wp.stash)) return rc;
+    BMS2_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    hipError_t erc = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+    if (erc == hipSuccess) erc = hipEventRecord(fork, st);
+    if (erc == hipSuccess) erc = hipStreamWaitEvent(q_wide->stream, fork, 0);
+    if (erc != hipSuccess) {
+      (void)hipEventDestroy(fork);
+      if (done) (void)hipEventDestroy(done);
+      return hip_fail(erc, "fork of the 4-word class", __FILE__, __LINE__);
+    }
+  }
   auto launch = [&](auto k2, auto k4) {
-    if (count[0]) hipLaunchKernelGGL(k2, grid, block, 0, st, bp, 0);
-    if (count[1]) hipLaunchKernelGGL(k4, grid, block, 0, st, bp, 1);
+    if (count[1])
+      hipLaunchKernelGGL(k4, dim3(wide_blocks), block, 0, beside ? q_wide->stream : st, wp, 1);
+    if (count[0]) hipLaunchKernelGGL(k2, dim3(blocks), block, 0, st, bp, 0);
   };
   if (bw == 4)
     launch(join_big_kernel<2, 4>, join_big_kernel<4, 4>);
@@ -1746,7 +1801,14 @@ int big_join(bms2_queue q, const bms2_device_csr* query, const bms2_device_csr* 
     launch(join_big_kernel<2, 2>, join_big_kernel<4, 2>);
   else
     launch(join_big_kernel<2, 1>, join_big_kernel<4, 1>);
-  BMS2_CHECK_LAUNCH("join_big_kernel");
+  hipError_t erc = hipGetLastError();
+  if (beside) {
+    if (erc == hipSuccess) erc = hipEventRecord(done, q_wide->stream);
+    if (erc == hipSuccess) erc = hipStreamWaitEvent(st, done, 0);
+    (void)hipEventDestroy(fork);
+    (void)hipEventDestroy(done);
+  }
+  if (erc != hipSuccess) return hip_fail(erc, "join_big_kernel", __FILE__, __LINE__);
   return BMS2_OK;
 }
 
