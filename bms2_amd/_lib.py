@@ -178,6 +178,8 @@ PROTOTYPES: dict[str, tuple] = {
     "bms2_pipeline_autotune": (i32, [vp, vp, vp, u32, C.POINTER(Tuning)]),
     "bms2_pipeline_set_queries": (i32, [vp, DCP]),
     "bms2_pipeline_query_totals": (i32, [vp, vp, vp, vp]),
+    "bms2_query_static_features": (i32, [HCP, vp]),
+    "bms2_score_orders": (i32, [HCP, vp, vp, vp]),
     "bms2_mlp_num_params": (u64, []),
     "bms2_mlp_create": (i32, [vp, u64, vpp]),
     "bms2_mlp_destroy": (i32, [vp]),

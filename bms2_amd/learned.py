@@ -207,6 +207,28 @@ def _static_features(qi: QueryInfo) -> np.ndarray:
     return X
 
 
+def static_features(queries) -> np.ndarray:
+    """The structure-only columns of features() for every node of a query batch, computed
+    natively (bms2_query_static_features); equal to _static_features row for row."""
+    X = np.zeros((queries.total_nodes, NUM_FEATURES), np.float32)
+    c = queries.to_c()
+    check(_L().bms2_query_static_features(C.byref(c), X.ctypes.data_as(C.c_void_p)))
+    return X
+
+
+def score_orders(queries, scores: np.ndarray, use: np.ndarray) -> np.ndarray:
+    """[Q][32] u8 matching orders: score_order for the queries with use[q], identity for the
+    others (bms2_score_orders)."""
+    Q = queries.num_graphs
+    order = np.zeros((Q, 32), np.uint8)
+    sc = np.ascontiguousarray(scores, np.float32)
+    u = np.ascontiguousarray(use, np.uint8)
+    c = queries.to_c()
+    check(_L().bms2_score_orders(C.byref(c), sc.ctypes.data_as(C.c_void_p),
+                                 u.ctypes.data_as(C.c_void_p), order.ctypes.data_as(C.c_void_p)))
+    return order
+
+
 def features(infos: list[QueryInfo], domsum: np.ndarray, qpairs: np.ndarray) -> np.ndarray:
     """[total query nodes][32] f32 features, rows in global query node order: query size, edges,
     cycle rank, largest degree, diameter, distinct labels, traced GMCR pairs and mean domain;
@@ -219,24 +241,52 @@ def features(infos: list[QueryInfo], domsum: np.ndarray, qpairs: np.ndarray) -> 
         return np.zeros((0, NUM_FEATURES), np.float32)
     X = np.concatenate([_static_features(qi) for qi in infos])
     labels = np.concatenate([qi.labels for qi in infos]).astype(np.int64)
+    sizes = np.array([qi.n for qi in infos], np.int64)
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    nbr = np.concatenate([qi.nbr + st for qi, st in zip(infos, starts)])
+    deg = np.concatenate([np.diff(qi.nbr_ptr) for qi in infos])
+    _batch_columns(X, labels, sizes, nbr, deg, domsum, qpairs)
+    return X
+
+
+def batch_features(queries, domsum: np.ndarray, qpairs: np.ndarray,
+                   static: np.ndarray | None = None) -> np.ndarray:
+    """features() of a whole query batch from its CSR: the structure-only columns natively (or
+    `static`, a copy of them made earlier), the per-batch columns as in features()."""
+    total = queries.total_nodes
+    if total == 0:
+        return np.zeros((0, NUM_FEATURES), np.float32)
+    X = static_features(queries) if static is None else np.array(static, np.float32)
+    labels = np.asarray(queries.node_labels).astype(np.int64)
+    sizes = np.diff(np.asarray(queries.graph_offsets, np.int64))
+    nbr = np.asarray(queries.column_indices, np.int64)
+    deg = np.diff(np.asarray(queries.row_offsets, np.int64))
+    _batch_columns(X, labels, sizes, nbr, deg, domsum, qpairs)
+    return X
+
+
+def _batch_columns(X: np.ndarray, labels: np.ndarray, sizes: np.ndarray, nbr: np.ndarray,
+                   deg: np.ndarray, domsum: np.ndarray, qpairs: np.ndarray) -> None:
+    """Columns of features() that depend on the batch and the trace: label rarity, traced pairs,
+    domain sizes, smallest neighbour domain, domain rank.  nbr / deg: every node's neighbours
+    (global node ids, any order, repeats allowed) and their count."""
+    total = X.shape[0]
     freq = np.bincount(labels, minlength=256).astype(np.float64) / total
     X[:, 27] = -np.log2(np.maximum(freq[labels], 1e-9)) / 8
-    sizes = np.array([qi.n for qi in infos], np.int64)
-    qid = np.repeat(np.arange(len(infos)), sizes)
+    Qn = len(sizes)
+    qid = np.repeat(np.arange(Qn), sizes)
     starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
-    pairs = np.asarray(qpairs, np.float64)[: len(infos)]
+    pairs = np.asarray(qpairs, np.float64)[:Qn]
     dom = np.where(pairs[qid] > 0,
                    np.asarray(domsum[:total], np.float64) / np.maximum(pairs[qid], 1), 0.0)
     nz = sizes > 0
-    meandom = np.zeros(len(infos))
+    meandom = np.zeros(Qn)
     meandom[nz] = np.add.reduceat(dom, starts[nz]) / sizes[nz]
     X[:, 6] = (np.log2(1 + pairs) / 16)[qid]
     X[:, 7] = (np.log2(1 + meandom) / 6)[qid]
     X[:, 11] = np.log2(1 + dom) / 6
     X[:, 12] = np.log2((1 + dom) / (1 + meandom[qid])) / 4
-    # smallest neighbour domain: neighbour lists of all queries in global node ids
-    nbr = np.concatenate([qi.nbr + st for qi, st in zip(infos, starts)])
-    deg = np.concatenate([np.diff(qi.nbr_ptr) for qi in infos])
+    # smallest neighbour domain
     has = deg > 0
     if nbr.size:
         ptr = np.concatenate([[0], np.cumsum(deg)[:-1]])
@@ -249,7 +299,6 @@ def features(infos: list[QueryInfo], domsum: np.ndarray, qpairs: np.ndarray) -> 
     rank[order] = np.arange(total) - starts[qid[order]]
     X[:, 29] = rank / np.maximum(sizes[qid] - 1, 1)
     X[:, 31] = (rank == 0).astype(np.float32)
-    return X
 
 
 # ---------------------------------------------------------------- orders
@@ -425,32 +474,30 @@ class Plan:
 
 
 def plan(pipe: Pipeline, net: OrderNet, queries, sample_graphs: int = 16384,
-         budget_log_ratio: float = -0.02, infos: list[QueryInfo] | None = None) -> Plan:
+         budget_log_ratio: float = -0.02, static: np.ndarray | None = None) -> Plan:
     """Traces a sample of the resident data graphs (identity order), predicts per query node
-    with the network in bf16, sets the pipeline's matching order and returns the plan."""
+    with the network in bf16, sets the pipeline's matching order and returns the plan.  The
+    query structure (features, orders) is handled natively; `static` (static_features of these
+    queries, nullable) skips recomputing it."""
     t0 = time.perf_counter()
-    infos = infos if infos is not None else query_infos(queries)
     pipe.set_order(None)
     tr = pipe.trace(sample_graphs, queries.total_nodes)
-    X = features(infos, tr["domsum"], tr["qpairs"])
+    X = batch_features(queries, tr["domsum"], tr["qpairs"], static)
     Y = net.forward(X, bf16=True)
-    Q = len(infos)
-    order = np.zeros((Q, 32), np.uint8)
-    learned = np.zeros(Q, bool)
-    depth = np.zeros(Q, np.int64)
+    Q = queries.num_graphs
+    sizes = np.diff(np.asarray(queries.graph_offsets, np.int64))
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]])
     ratio = np.zeros(Q, np.float32)
-    r = 0
-    for qq, qi in enumerate(infos):
-        n = qi.n
-        order[qq, :n] = np.arange(n)
-        if n:
-            y = Y[r:r + n]
-            ratio[qq] = float(y[:, 1].mean())
-            depth[qq] = 1 + int(np.argmax(y[:, 2:].mean(axis=0)))
-            if n > 1 and ratio[qq] < budget_log_ratio:
-                order[qq, :n] = score_order(qi, y[:, 0])
-                learned[qq] = True
-        r += n
+    depth = np.zeros(Q, np.int64)
+    nz = sizes > 0
+    if nz.any():
+        # per-query means of the predicted log cost ratio and of the depth scores
+        sums = np.add.reduceat(Y.astype(np.float64), starts[nz], axis=0)
+        means = sums / sizes[nz][:, None]
+        ratio[nz] = means[:, 1]
+        depth[nz] = 1 + np.argmax(means[:, 2:], axis=1)
+    learned = (sizes > 1) & (ratio < budget_log_ratio)
+    order = score_orders(queries, Y[:, 0], learned)
     pipe.set_order(order)
     return Plan(order, learned, depth, ratio, tr["graphs"], time.perf_counter() - t0)
 

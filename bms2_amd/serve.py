@@ -145,12 +145,13 @@ class PipelineExecutor:
         self.counts = DeviceBuffer(q, 8 * self.D * max_queries, zero=False)
         self.valid = DeviceBuffer(q, 8 * self.D * ((max_queries + 63) // 64), zero=False)
         self.stats = DeviceBuffer(q, 64, zero=True)
-        self.infos = None
+        self.static = None  # structure-only feature rows of the pool's query nodes
         self.orders: dict[int, np.ndarray] = {}  # pool index -> planned order row
         self.planned_batches = 0
+        self.pool_go = np.asarray(pool.graph_offsets, np.int64)
         if net is not None:
             from . import learned
-            self.infos = learned.query_infos(pool)
+            self.static = learned.static_features(pool)
         self.keep = []  # device query batches stay alive while bound
 
     def __call__(self, batch: list[Request]) -> None:
@@ -166,8 +167,10 @@ class PipelineExecutor:
             if all(int(i) in self.orders for i in idx):
                 self.pipe.set_order(np.stack([self.orders[int(i)] for i in idx]))
             else:
+                rows = np.concatenate([np.arange(self.pool_go[i], self.pool_go[i + 1])
+                                       for i in idx])
                 pl = learned.plan(self.pipe, self.net, qb, sample_graphs=self.plan_graphs,
-                                  infos=[self.infos[int(i)] for i in idx])
+                                  static=self.static[rows])
                 self.planned_batches += 1
                 for k, i in enumerate(idx):
                     self.orders.setdefault(int(i), pl.order[k].copy())

@@ -440,6 +440,20 @@ typedef struct bms2_adam {
   float lr, beta1, beta2, eps;
 } bms2_adam;
 
+/* Query-side input and output of the network, on the host (queries of at most 32 nodes).
+ * bms2_query_static_features fills, per query node (rows in global node order, BMS2_MLP_IN floats
+ * each), the feature columns that depend on the query's structure only -- 0-5 query size,
+ * edges, cycle rank, largest degree, diameter, distinct labels; 8-10 degree, eccentricity,
+ * closeness; 13-25 leaf neighbours, summed neighbour degrees, ring membership, wildcard and
+ * element one-hot, position; 28 degree rank; 30 largest-degree flag -- and sets the others to 0.
+ * bms2_score_orders writes order[32 g + d] for every query g: with use[g] != 0 the connected order
+ * by descending scores[node] (the best node first, then always the best node adjacent to the
+ * matched set, any remaining node when none is, ties to the lower id), identity otherwise.
+ * No counterpart in the reference. */
+int bms2_query_static_features(const bms2_host_csr* query, float* out);
+int bms2_score_orders(const bms2_host_csr* query, const float* scores, const uint8_t* use,
+                      uint8_t* order);
+
 typedef struct bms2_mlp_s* bms2_mlp;
 uint64_t bms2_mlp_num_params(void);
 /* He-uniform weights from a splitmix64 stream of `seed`, zero biases, zero Adam state. */

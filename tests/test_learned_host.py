@@ -136,3 +136,54 @@ def test_vectorised_features_match_per_node_statement():
     assert np.allclose(got, want, rtol=1e-6, atol=1e-6), np.argwhere(~np.isclose(got, want))[:5]
     # cached static parts give the same result on a second call
     assert np.array_equal(learned.features(infos, domsum, qpairs), got)
+
+
+def _batches_for_native():
+    from bms2_amd import graphs
+    out = [_queries()]
+    for wl, seed in (("uneven-1m", 2), ("molecules-1m", 5)):
+        _, q = graphs.generate(wl, seed=seed, num_data_graphs=50, num_query_graphs=200)
+        out.append(q)
+    # a repeated edge collapses as in the neighbour sets
+    out.append(GraphBatch.from_graphs([([8, 8, 8], [(0, 1), (0, 1), (1, 2)])]))
+    return out
+
+
+def test_native_static_features_equal_python():
+    """bms2_query_static_features gives _static_features' rows bit for bit (ring membership,
+    distances, ranks and ties included) on hand-made, uneven and molecule queries and on a query
+    with a repeated edge."""
+    for q in _batches_for_native():
+        infos = learned.query_infos(q)
+        want = np.concatenate([learned._static_features(qi) for qi in infos])
+        assert np.array_equal(learned.static_features(q), want)
+
+
+def test_native_score_orders_equal_python():
+    rng = np.random.default_rng(3)
+    for q in _batches_for_native():
+        infos = learned.query_infos(q)
+        go = np.asarray(q.graph_offsets)
+        for quant in (None, 0.5):  # continuous scores, then many ties
+            sc = rng.normal(size=q.total_nodes).astype(np.float32)
+            if quant:
+                sc = (np.round(sc / quant) * quant).astype(np.float32)
+            use = rng.random(q.num_graphs) < 0.7
+            got = learned.score_orders(q, sc, use)
+            for g, qi in enumerate(infos):
+                want = learned.score_order(qi, sc[go[g]:go[g + 1]]) if use[g] else list(range(qi.n))
+                assert list(got[g, :qi.n]) == want, (g, quant)
+
+
+def test_batch_features_equal_features():
+    from bms2_amd import graphs
+    _, queries = graphs.generate("uneven-1m", seed=4, num_data_graphs=50, num_query_graphs=80)
+    infos = learned.query_infos(queries)
+    rng = np.random.default_rng(1)
+    domsum = rng.integers(0, 500, size=queries.total_nodes).astype(np.uint64)
+    qpairs = rng.integers(0, 30, size=queries.num_graphs).astype(np.uint64)
+    qpairs[::5] = 0
+    want = learned.features(infos, domsum, qpairs)
+    assert np.array_equal(learned.batch_features(queries, domsum, qpairs), want)
+    static = learned.static_features(queries)
+    assert np.array_equal(learned.batch_features(queries, domsum, qpairs, static), want)
