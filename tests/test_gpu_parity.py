@@ -4,6 +4,7 @@ import pytest
 
 import oracle
 from bms2_amd import _lib, api, graphs
+from bms2_amd.graphs import GraphBatch
 
 pytestmark = pytest.mark.gpu
 
@@ -174,6 +175,49 @@ def test_fused_big_graphs_autotuned(queue, big_graphs):
     c, v = pipe.results(counts, valid)
     assert np.array_equal(c, rc) and np.array_equal(v, rv)
     assert np.array_equal(stats.to_numpy(np.uint64, 8)[:4], rs)
+
+
+def test_fused_big_graphs_buffers_grow(queue):
+    """600 queries of 2-3 carbons against 40 graphs of 65-250 nodes: every graph holds far more
+    GMCR pairs than the big-graph screen's first buffers (128 entries per graph), so the run
+    sizes its buffers from the counters and redoes the screen; results equal the oracle's."""
+    data, _ = graphs.generate("uneven-1m", seed=5, num_data_graphs=40, data_min_nodes=65,
+                              data_lognormal_mu=4.8, data_lognormal_sigma=0.4)
+    queries = GraphBatch.from_graphs([([8, 8], [(0, 1)])] * 300 +
+                                     [([8, 8, 8], [(0, 1), (1, 2)])] * 300)
+    for find_first in (False, True):
+        counts, valid, stats = api.run_pipeline(queue, queries, data, 2, find_first)
+        rc, rv, rs = oracle.pipeline(queries, data, 2, find_first)
+        assert rs[0] > 128 * 40  # GMCR pairs beyond the first sizing
+        assert np.array_equal(valid, rv)
+        assert np.array_equal(counts, rc)
+        assert np.array_equal(stats[:4], rs)
+
+
+def test_fused_big_graphs_new_query_batch(queue, big_graphs):
+    """A pipeline over graphs above 64 nodes bound to a second query batch
+    (bms2_pipeline_set_queries) serves it exactly, twice in a row."""
+    data, queries = big_graphs
+    second = graphs.generate("uneven-1m", seed=9, num_data_graphs=10, num_query_graphs=300)[1]
+    dd = api.DeviceBatchedCSRGraph(queue, data)
+    pipe = api.Pipeline(queue, api.DeviceBatchedCSRGraph(queue, queries), dd, refinement_steps=6)
+    counts, valid, stats = pipe.alloc_outputs()
+    pipe.run(counts, valid, stats)
+    queue.wait()
+    rc, rv, _ = oracle.pipeline(queries, data, 6, False)
+    c, v = pipe.results(counts, valid)
+    assert np.array_equal(c, rc) and np.array_equal(v, rv)
+    dq2 = api.DeviceBatchedCSRGraph(queue, second)
+    pipe.set_queries(dq2)
+    counts, valid, stats = pipe.alloc_outputs()
+    rc, rv, rs = oracle.pipeline(second, data, 6, False)
+    for _ in range(2):
+        stats.zero()
+        pipe.run(counts, valid, stats)
+        queue.wait()
+        c, v = pipe.results(counts, valid)
+        assert np.array_equal(c, rc) and np.array_equal(v, rv)
+        assert np.array_equal(stats.to_numpy(np.uint64, 8)[:4], rs)
 
 
 def test_fused_uneven_order_and_trace(queue, uneven_2k):
