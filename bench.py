@@ -486,14 +486,19 @@ def main() -> int:
     k_avg_s = float(np.mean(kernel_ms)) / 1000.0
     achieved = abytes / k_avg_s / 1e9
     traffic = load_traffic()
+    # the measured traffic belongs to one configuration: the default molecules-1m learned run
+    measured_here = (traffic is not None and args.workload == "molecules-1m" and args.seed == 0
+                     and K == 6 and not args.find_first and not args.data_graphs
+                     and order == traffic.get("order", "learned"))
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": (traffic or {}).get("bytes_per_launch"),
-        "traffic_source": None if not traffic else (
+        "traffic": traffic.get("bytes_per_launch") if measured_here else None,
+        "traffic_source": (
             "profiles/pipeline_molecules-1m_traffic.json: rocprofv3 FETCH_SIZE + WRITE_SIZE of one "
             f"{traffic.get('order', 'learned')}-order run, tree {traffic.get('tree', 'unrecorded')}, "
-            f"{traffic.get('measured', 'date unrecorded')} (not this run)"),
+            f"{traffic.get('measured', 'date unrecorded')} (not this run)") if measured_here else
+            "not measured for this configuration",
         "kernel": f"screen_kernel<{K}> + join_kernel (device span per run, {order} order)",
         "kernel_ms_avg": round(float(np.mean(kernel_ms)), 3),
         "algorithmic_bytes_per_launch": abytes,
