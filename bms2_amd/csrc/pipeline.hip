@@ -1068,16 +1068,27 @@ struct bms2_pipeline_s {
   // beside its 2-word class), forked after the run's query records and joined before its end
   bms2_queue side_q = nullptr, side_q2 = nullptr;
   hipEvent_t big_fork = nullptr, big_done = nullptr;
+  hipEvent_t big_screened = nullptr;  // the chunk's screen is done: the big-graph joins may start
   bool big_pending = false;
   uint64_t big_pairs = 0;           // GMCR pairs of the pending big-graph path
   // the single-chunk kernels captured in a hipGraph, valid for these outputs and K
   hipGraphExec_t graph = nullptr;
+  // with graphs above 64 nodes the chunk is two graphs, the screen's and the joins', so the
+  // big-graph path can fork between them
+  hipGraphExec_t graph_joins = nullptr;
   uint64_t* graph_counts = nullptr;
   uint64_t* graph_valid = nullptr;
   uint32_t graph_packs = 0;
 };
 
 namespace {
+
+// Destroys the captured chunk graphs (the next run captures them again).
+void drop_graphs(bms2_pipeline_s* p) {
+  if (p->graph) (void)hipGraphExecDestroy(p->graph);
+  if (p->graph_joins) (void)hipGraphExecDestroy(p->graph_joins);
+  p->graph = p->graph_joins = nullptr;
+}
 
 // Releases the big-graph screen's entry and record buffers (the control words stay).
 void free_big_entries(bms2_pipeline_s* p) {
@@ -1129,7 +1140,7 @@ void pipeline_free(bms2_pipeline p) {
   (void)hipFree(p->d_segs);
   (void)hipFree(p->d_recs_l);
   (void)hipFree(p->d_ctl);
-  if (p->graph) (void)hipGraphExecDestroy(p->graph);
+  drop_graphs(p);
   (void)hipFree(p->d_big_gid);
   (void)hipFree(p->d_big_list);
   (void)hipFree(p->d_big_lev);
@@ -1142,6 +1153,7 @@ void pipeline_free(bms2_pipeline p) {
   if (p->big_cyc) bms2_cycle_signature_destroy(p->big_cyc);
   if (p->big.graph_offsets) bms2_device_csr_destroy(p->q, &p->big);
   if (p->big_fork) (void)hipEventDestroy(p->big_fork);
+  if (p->big_screened) (void)hipEventDestroy(p->big_screened);
   if (p->big_done) (void)hipEventDestroy(p->big_done);
   if (p->side_q2) bms2_queue_destroy(p->side_q2);
   if (p->side_q) bms2_queue_destroy(p->side_q);
@@ -1212,6 +1224,7 @@ int build_big_batch(bms2_pipeline_s* p, const std::vector<uint32_t>& go) {
   if (int rc = bms2_queue_create(p->q->device, &p->side_q)) return rc;
   if (int rc = bms2_queue_create(p->q->device, &p->side_q2)) return rc;
   BMS2_HIP(hipEventCreateWithFlags(&p->big_fork, hipEventDisableTiming));
+  BMS2_HIP(hipEventCreateWithFlags(&p->big_screened, hipEventDisableTiming));
   BMS2_HIP(hipEventCreateWithFlags(&p->big_done, hipEventDisableTiming));
   BMS2_HIP(hipMemcpy(p->d_big_gid, gid.data(), 4ull * gid.size(), hipMemcpyHostToDevice));
   BMS2_HIP(hipMemcpy(p->d_big_list, list.data(), 4ull * list.size(), hipMemcpyHostToDevice));
@@ -1219,8 +1232,9 @@ int build_big_batch(bms2_pipeline_s* p, const std::vector<uint32_t>& go) {
 }
 
 // Enqueues the screen and joins of the graphs above 64 nodes with an original id below g_limit
-// on the side queue, after the main stream's work up to p->big_fork (this run's query records);
-// they write those graphs' rows of the dense outputs.  The side queue's stream is synchronised
+// on the side queue: the screen after the main stream's work up to p->big_fork (this run's query
+// records), the joins after p->big_screened (the first chunk's screen, so they run beside the
+// chunk's joins); they write those graphs' rows of the dense outputs.  The side queue's stream is synchronised
 // once, to size the screen's buffers; big_finish joins it.  The signatures are the staged
 // kernels' (path signatures; ring counts, which stay zero on graphs above 64 nodes as in
 // signature.hpp:614) and the shell levels of every refinement step; the screen replaces the
@@ -1285,6 +1299,7 @@ int big_start(bms2_pipeline_s* p, uint64_t* d_counts, uint64_t* d_valid, uint32_
   const uint64_t count[kBigClasses] = {ctl[0], ctl[1]};
   const uint64_t runs[kBigClasses] = {ctl[kBigClasses], ctl[kBigClasses + 1]};
   hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_big_stats, 8u);
+  BMS2_HIP(hipStreamWaitEvent(s, p->big_screened, 0));  // beside the chunk's joins
   if (int rc = big_join(q, p->side_q2, dq, db, p->be, count, runs, int(p->opt.find_first),
                         p->d_jorder, d_counts, p->d_big_gid, p->d_big_stats, &p->big_shape))
     return rc;
@@ -1384,8 +1399,7 @@ void free_query_side(bms2_pipeline_s* p) {
   if (p->big_cyc) bms2_cycle_signature_destroy(p->big_cyc);
   p->big_path = nullptr;
   p->big_cyc = nullptr;
-  if (p->graph) (void)hipGraphExecDestroy(p->graph);
-  p->graph = nullptr;
+  drop_graphs(p);
   p->dens_s = p->dens_seg = p->dens_l = 0.0;
 }
 
@@ -1667,8 +1681,8 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
     BMS2_CHECK_LAUNCH("query_pre_kernel");
     if (p->num_big) BMS2_HIP(hipEventRecord(p->big_fork, s));  // the big-graph path's inputs
     if (int krc = ks.begin()) return krc;
-    // kernels of one chunk of packs [p0, p1)
-    auto launch_chunk = [&](uint32_t p0, uint32_t p1) -> int {
+    // kernels of one chunk of packs [p0, p1): the screen, then the joins
+    auto launch_screen = [&](uint32_t p0, uint32_t p1) -> int {
       hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_ctl, uint32_t(kCtlWords));
       sp.pack_begin = p0;
       if (p->hist_compact)
@@ -1676,6 +1690,9 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
       else
         hipLaunchKernelGGL((screen_kernel<K, 4>), dim3(p1 - p0), dim3(64), stage_bytes, s, sp);
       BMS2_CHECK_LAUNCH("screen_kernel");
+      return BMS2_OK;
+    };
+    auto launch_joins = [&]() -> int {
       const dim3 gs(uint32_t(p->cus) * p->small_waves), gl(uint32_t(p->cus) * p->large_waves);
       if (p->slots == 8)
         hipLaunchKernelGGL(join_small_kernel<8>, gs, dim3(64), 0, s, js);
@@ -1689,6 +1706,18 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
           hipLaunchKernelGGL(join_kernel<32>, gl, dim3(64), 0, s, jp);
         BMS2_CHECK_LAUNCH("join_kernel");
       }
+      return BMS2_OK;
+    };
+    auto capture = [&](auto&& launch, hipGraphExec_t* out) -> int {
+      hipGraph_t g = nullptr;
+      BMS2_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      const int lrc = launch();
+      const hipError_t erc = hipStreamEndCapture(s, &g);
+      if (lrc) return lrc;
+      BMS2_HIP(erc);
+      const hipError_t irc = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      BMS2_HIP(irc);
       return BMS2_OK;
     };
     const bool tracing = tr.domsum || tr.qnodes || tr.pair_nodes;
@@ -1707,31 +1736,35 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
       // a measured batch that fits one chunk replays its captured kernels
       const bool graphable = !p->opt.no_graph && !tracing && p0 == 0 && p1 == pack_end &&
                              p->dens_s + p->dens_l > 0;
+      // the graphs above 64 nodes are screened beside the first chunk's screen and joined beside
+      // its joins (their rows of the outputs are disjoint from the chunk's)
+      const bool fork = !big_started && p->num_big;
       if (graphable && !(p->graph && p->graph_counts == d_counts && p->graph_valid == d_valid &&
                          p->graph_packs == pack_end)) {
-        if (p->graph) (void)hipGraphExecDestroy(p->graph);
-        p->graph = nullptr;
-        hipGraph_t g = nullptr;
-        BMS2_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        const int lrc = launch_chunk(p0, p1);
-        const hipError_t erc = hipStreamEndCapture(s, &g);
-        if (lrc) return lrc;
-        BMS2_HIP(erc);
-        const hipError_t irc = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        BMS2_HIP(irc);
+        drop_graphs(p);
+        if (p->num_big) {
+          if (int crc = capture([&] { return launch_screen(p0, p1); }, &p->graph)) return crc;
+          if (int crc = capture([&] { return launch_joins(); }, &p->graph_joins)) return crc;
+        } else if (int crc = capture([&] {
+                     const int rc = launch_screen(p0, p1);
+                     return rc ? rc : launch_joins();
+                   }, &p->graph)) {
+          return crc;
+        }
         p->graph_counts = d_counts;
         p->graph_valid = d_valid;
         p->graph_packs = pack_end;
       }
       if (graphable) {
         BMS2_HIP(hipGraphLaunch(p->graph, s));
-      } else if (int lrc = launch_chunk(p0, p1)) {
-        return lrc;
+        if (fork) BMS2_HIP(hipEventRecord(p->big_screened, s));
+        if (p->graph_joins) BMS2_HIP(hipGraphLaunch(p->graph_joins, s));
+      } else {
+        if (int lrc = launch_screen(p0, p1)) return lrc;
+        if (fork) BMS2_HIP(hipEventRecord(p->big_screened, s));
+        if (int lrc = launch_joins()) return lrc;
       }
       if (!big_started) {
-        // the graphs above 64 nodes, beside the first chunk (its rows of the outputs are
-        // disjoint from theirs)
         big_started = true;
         if (int brc = big_start(p, d_counts, d_valid, g_limit)) return brc;
       }
@@ -1752,10 +1785,7 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
         p->dens_s = std::max(p->dens_s * 2.0, ds);
         p->dens_seg = std::max(p->dens_seg * 2.0, dg);
         p->dens_l = std::max(p->dens_l * 2.0, dl);
-        if (graphable) {
-          (void)hipGraphExecDestroy(p->graph);
-          p->graph = nullptr;
-        }
+        if (graphable) drop_graphs(p);
         continue;
       }
       p->dens_s = std::max(p->dens_s, ds);
@@ -1764,8 +1794,10 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
       for (int k = 0; k < 4; ++k) stats[k] += ctl[kCtlPairs + k];
       p0 = p1;
     }
-    if (!big_started)
+    if (!big_started) {
+      if (p->num_big) BMS2_HIP(hipEventRecord(p->big_screened, s));
       if (int brc = big_start(p, d_counts, d_valid, g_limit)) return brc;
+    }
     if (int brc = big_finish(p, stats)) return brc;
     return ks.end();
   });
@@ -1851,10 +1883,7 @@ extern "C" int bms2_pipeline_autotune(bms2_pipeline p, uint64_t* d_counts, uint6
     return c.small_waves == p->small_waves && c.large_waves == p->large_waves && c.slots == p->slots;
   };
   auto apply = [&](const Cand& c) {
-    if (!same_join_shape(c)) {
-      if (p->graph) (void)hipGraphExecDestroy(p->graph);
-      p->graph = nullptr;  // grid shapes are baked into the graph
-    }
+    if (!same_join_shape(c)) drop_graphs(p);  // grid shapes are baked into the graphs
     p->small_waves = c.small_waves;
     p->large_waves = c.large_waves;
     p->slots = c.slots;
