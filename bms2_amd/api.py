@@ -559,20 +559,21 @@ class Pipeline(_Handle):
             raise ValueError(f"order must have shape ({self.Q}, 32), got {o.shape}")
         check(_L().bms2_pipeline_set_order(self.h, o.ctypes.data_as(C.c_void_p)))
 
-    def trace(self, max_graphs: int, query_nodes: int, pair_nodes: bool = False) -> dict:
+    def trace(self, max_graphs: int, query_nodes: int, pair_nodes: bool = False,
+              counts: bool = True) -> dict:
         """Join traces over the leading data graphs (at least max_graphs of them): summed
         domain size per query node, GMCR pairs and DFS nodes per query, optionally DFS nodes per
-        (graph, query) pair, plus the counts of the traced graphs."""
+        (graph, query) pair, plus (counts=True) the counts of the traced graphs."""
         W = PACK_WINDOW  # whole pack windows: max_graphs rounded up to the window
         G = min((int(max_graphs) + W - 1) // W * W, self.D)
-        counts = DeviceBuffer(self.q, 8 * max(G * self.Q, 1), zero=False)
+        dcounts = DeviceBuffer(self.q, 8 * max(G * self.Q, 1), zero=False)
         valid = DeviceBuffer(self.q, 8 * max(G * self.VW, 1), zero=False)
         domsum = DeviceBuffer(self.q, 8 * max(query_nodes, 1), zero=True)
         qpairs = DeviceBuffer(self.q, 8 * max(self.Q, 1), zero=True)
         qnodes = DeviceBuffer(self.q, 8 * max(self.Q, 1), zero=True)
         pn = DeviceBuffer(self.q, 4 * max(G * self.Q, 1), zero=True) if pair_nodes else None
         got = C.c_uint32(0)
-        check(_L().bms2_pipeline_trace(self.h, int(max_graphs), C.c_void_p(counts.ptr),
+        check(_L().bms2_pipeline_trace(self.h, int(max_graphs), C.c_void_p(dcounts.ptr),
                                        C.c_void_p(valid.ptr), C.c_void_p(domsum.ptr),
                                        C.c_void_p(qpairs.ptr), C.c_void_p(qnodes.ptr),
                                        C.c_void_p(pn.ptr) if pn else None, C.byref(got)))
@@ -580,8 +581,9 @@ class Pipeline(_Handle):
         out = dict(graphs=g,
                    domsum=domsum.to_numpy(np.uint64, query_nodes),
                    qpairs=qpairs.to_numpy(np.uint64, self.Q),
-                   qnodes=qnodes.to_numpy(np.uint64, self.Q),
-                   counts=counts.to_numpy(np.uint64, g * self.Q).reshape(g, self.Q))
+                   qnodes=qnodes.to_numpy(np.uint64, self.Q))
+        if counts:
+            out["counts"] = dcounts.to_numpy(np.uint64, g * self.Q).reshape(g, self.Q)
         if pn is not None:
             out["pair_nodes"] = pn.to_numpy(np.uint32, g * self.Q).reshape(g, self.Q)
         return out
