@@ -560,28 +560,33 @@ class Pipeline(_Handle):
         check(_L().bms2_pipeline_set_order(self.h, o.ctypes.data_as(C.c_void_p)))
 
     def trace(self, max_graphs: int, query_nodes: int, pair_nodes: bool = False,
-              counts: bool = True) -> dict:
+              counts: bool = True, joins: bool = True) -> dict:
         """Join traces over the leading data graphs (at least max_graphs of them): summed
         domain size per query node, GMCR pairs and DFS nodes per query, optionally DFS nodes per
-        (graph, query) pair, plus (counts=True) the counts of the traced graphs."""
+        (graph, query) pair, plus (counts=True) the counts of the traced graphs.  joins=False
+        runs the screens only: domain sums and GMCR pairs, no DFS nodes or counts."""
+        if not joins and (pair_nodes or counts):
+            raise ValueError("a trace without joins has no DFS nodes or counts")
         W = PACK_WINDOW  # whole pack windows: max_graphs rounded up to the window
         G = min((int(max_graphs) + W - 1) // W * W, self.D)
         dcounts = DeviceBuffer(self.q, 8 * max(G * self.Q, 1), zero=False)
         valid = DeviceBuffer(self.q, 8 * max(G * self.VW, 1), zero=False)
         domsum = DeviceBuffer(self.q, 8 * max(query_nodes, 1), zero=True)
         qpairs = DeviceBuffer(self.q, 8 * max(self.Q, 1), zero=True)
-        qnodes = DeviceBuffer(self.q, 8 * max(self.Q, 1), zero=True)
+        qnodes = DeviceBuffer(self.q, 8 * max(self.Q, 1), zero=True) if joins else None
         pn = DeviceBuffer(self.q, 4 * max(G * self.Q, 1), zero=True) if pair_nodes else None
         got = C.c_uint32(0)
         check(_L().bms2_pipeline_trace(self.h, int(max_graphs), C.c_void_p(dcounts.ptr),
                                        C.c_void_p(valid.ptr), C.c_void_p(domsum.ptr),
-                                       C.c_void_p(qpairs.ptr), C.c_void_p(qnodes.ptr),
+                                       C.c_void_p(qpairs.ptr),
+                                       C.c_void_p(qnodes.ptr) if qnodes else None,
                                        C.c_void_p(pn.ptr) if pn else None, C.byref(got)))
         g = int(got.value)
         out = dict(graphs=g,
                    domsum=domsum.to_numpy(np.uint64, query_nodes),
-                   qpairs=qpairs.to_numpy(np.uint64, self.Q),
-                   qnodes=qnodes.to_numpy(np.uint64, self.Q))
+                   qpairs=qpairs.to_numpy(np.uint64, self.Q))
+        if qnodes is not None:
+            out["qnodes"] = qnodes.to_numpy(np.uint64, self.Q)
         if counts:
             out["counts"] = dcounts.to_numpy(np.uint64, g * self.Q).reshape(g, self.Q)
         if pn is not None:

@@ -1721,7 +1721,9 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
       return BMS2_OK;
     };
     const bool tracing = tr.domsum || tr.qnodes || tr.pair_nodes;
-    bool big_started = false;
+    // a trace of domains alone runs the screens only
+    const bool screen_only = tr.domsum && !tr.qnodes && !tr.pair_nodes;
+    bool big_started = screen_only;
     uint32_t p0 = 0;
     while (p0 < pack_end) {
       // chunk of packs sized so the expected records of both classes and the segments fit
@@ -1762,7 +1764,8 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
       } else {
         if (int lrc = launch_screen(p0, p1)) return lrc;
         if (fork) BMS2_HIP(hipEventRecord(p->big_screened, s));
-        if (int lrc = launch_joins()) return lrc;
+        if (!screen_only)
+          if (int lrc = launch_joins()) return lrc;
       }
       if (!big_started) {
         big_started = true;
@@ -1835,7 +1838,7 @@ extern "C" int bms2_pipeline_trace(bms2_pipeline p, uint32_t max_graphs, uint64_
                                    uint32_t* graphs_out) {
   if (!p) return set_error(BMS2_ERR_INVALID, "pipeline is null");
   if (int rc = use_device(p->q)) return rc;
-  if (!d_counts || !d_valid || !d_domsum || !d_qpairs || !d_qnodes)
+  if (!d_counts || !d_valid || !d_domsum || !d_qpairs)
     return set_error(BMS2_ERR_INVALID, "null trace buffer");
   // leading pack windows covering at least max_graphs graphs (all of them when max_graphs is)
   size_t w = 0;

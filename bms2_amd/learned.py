@@ -481,7 +481,7 @@ def plan(pipe: Pipeline, net: OrderNet, queries, sample_graphs: int = 16384,
     queries, nullable) skips recomputing it."""
     t0 = time.perf_counter()
     pipe.set_order(None)
-    tr = pipe.trace(sample_graphs, queries.total_nodes, counts=False)
+    tr = pipe.trace(sample_graphs, queries.total_nodes, counts=False, joins=False)
     X = batch_features(queries, tr["domsum"], tr["qpairs"], static)
     Y = net.forward(X, bf16=True)
     Q = queries.num_graphs

@@ -415,7 +415,10 @@ int bms2_pipeline_set_order(bms2_pipeline p, const uint8_t* order);
  * per query the number of GMCR pairs (d_qpairs u64[Q]) and DFS nodes expanded (d_qnodes u64[Q]),
  * and per pair the DFS nodes (d_pair_nodes u32[graphs][Q], saturating, nullable).  The caller
  * zeroes the accumulators.  d_counts / d_valid are sized for the traced graphs as in
- * bms2_pipeline_run.  No counterpart in the reference: it feeds the learned matching order. */
+ * bms2_pipeline_run.  With d_qnodes and d_pair_nodes both null only the screen runs (the domain
+ * sums and GMCR pairs need nothing else): d_valid is complete, d_counts holds only the zeros of
+ * the pairs outside the GMCR.  No counterpart in the reference: it feeds the learned matching
+ * order. */
 int bms2_pipeline_trace(bms2_pipeline p, uint32_t max_graphs, uint64_t* d_counts,
                         uint64_t* d_valid, uint64_t* d_domsum, uint64_t* d_qpairs,
                         uint64_t* d_qnodes, uint32_t* d_pair_nodes, uint32_t* graphs_out);

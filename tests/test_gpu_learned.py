@@ -106,6 +106,26 @@ def test_trace_prefix(queue, molecules_2k):
     assert tr["graphs"] == data.num_graphs
 
 
+def test_screen_only_trace_equals_full_trace(queue, molecules_2k):
+    """The planner's trace runs the screens only (null DFS-node buffers): its domain sums and
+    GMCR pairs equal the full trace's, and it reports no DFS nodes or counts."""
+    data, queries = molecules_2k
+    pipe = api.Pipeline(queue, api.DeviceBatchedCSRGraph(queue, queries),
+                        api.DeviceBatchedCSRGraph(queue, data), refinement_steps=6)
+    full = pipe.trace(data.num_graphs, queries.total_nodes)
+    screen = pipe.trace(data.num_graphs, queries.total_nodes, counts=False, joins=False)
+    assert screen["graphs"] == full["graphs"]
+    assert np.array_equal(screen["domsum"], full["domsum"])
+    assert np.array_equal(screen["qpairs"], full["qpairs"])
+    assert "qnodes" not in screen and "counts" not in screen
+    # a full run afterwards is unaffected
+    rc, _, _ = oracle.pipeline(queries, data, 6, False)
+    counts, valid, stats = pipe.alloc_outputs()
+    pipe.run(counts, valid, stats)
+    queue.wait()
+    assert np.array_equal(pipe.results(counts, valid)[0], rc)
+
+
 def _rand_batch(rows, rng):
     x = rng.normal(size=(rows, omlp.IN)).astype(np.float32)
     t = np.stack([rng.normal(size=rows), rng.normal(size=rows),
