@@ -75,15 +75,19 @@ struct ScreenParams {
   uint64_t* qpairs;         // trace (nullable): [Q] GMCR pairs
 };
 
-// Control words of one chunk (u64).
-constexpr int kCtlSmall = 0;      // small records reserved (low 40 bits), segments (high 24)
-constexpr int kCtlOverflow = 1;   // a record or segment buffer overflowed
-constexpr int kCtlSegCursor = 2;  // small join: segments claimed
-constexpr int kCtlPairs = 3;      // stats: GMCR pairs, matched pairs, embeddings, DFS nodes
-constexpr int kCtlLarge = 7;      // large records reserved
-constexpr int kCtlLargeCursor = 8;
+// Control words of one chunk (u64).  Each counter that many waves update at once (the screen's
+// record reservations of each class and its GMCR pair count, the joins' claim cursors) has a
+// 128-byte line of its own: atomics on different words of one line wait for each other (sharing
+// one line cost the screen 5 ms per run on 1M molecules x 1,024 queries).
+constexpr int kCtlLine = 16;                  // u64 words per 128-byte line
+constexpr int kCtlSmall = 0;                  // small records reserved (low 40 bits), segments (high 24)
+constexpr int kCtlOverflow = 1;               // a record or segment buffer overflowed
+constexpr int kCtlLarge = kCtlLine;           // large records reserved
+constexpr int kCtlPairs = 2 * kCtlLine;       // stats: GMCR pairs, matched pairs, embeddings, DFS nodes
+constexpr int kCtlSegCursor = 3 * kCtlLine;   // small join: segments claimed
+constexpr int kCtlLargeCursor = 4 * kCtlLine; // large join: records claimed
 constexpr int kSegShift = 40;
-constexpr int kCtlWords = 16;
+constexpr int kCtlWords = 5 * kCtlLine;
 
 struct JoinParams {
   const uint64_t* recs;     // large-query records
@@ -358,6 +362,9 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
   }
   __syncthreads();
   const uint32_t myslotbit = lane < nn ? 1u << myslot : 0u;  // this lane's graph in a gpass mask
+  // the node test's masks held in VGPRs (as scalar constants they are rebuilt per test)
+  uint32_t k_sh_bits = 0x10101010u, k_ring_bits = 0x80008000u;
+  asm volatile("" : "+v"(k_sh_bits), "+v"(k_ring_bits));
 
   for (uint32_t qb = 0; qb < p.Q; qb += 64) {
     // 64 queries at once, one per lane, against every graph of the pack: every node's (label,
@@ -382,6 +389,10 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
         }
       }
     }
+    // the prefilter's record loads are complete here on every path (a lane whose query fails
+    // the prefilter never reads them): without this the first node test of every query waits on
+    // the vector-memory counter, and with it on the matching-order load issued ahead of the test
+    __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
     uint64_t cand = ballot(gpass != 0);
     while (cand) {
       const uint32_t li = uint32_t(ctz64(cand));
@@ -398,10 +409,9 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
       uint64_t tops = lanes & T;
       // domains by local query node id go to qdom (written by lane 0)
       // nq >= 2 here (the prefilter passes only queries with more than one node); the loop
-      // ends after the last node or as soon as no graph is left (then `left` is set to 1 on the
-      // rare path where a graph dies, keeping the common path to one counter test)
+      // ends after the last node or as soon as no graph is left
       const uint32_t* r = p.qrec + size_t(rb) * kRecDwords;
-      uint32_t left = nq;
+      const uint32_t* rend = r + size_t(nq) * kRecDwords;
       do {
         // the whole record in one batch of scalar loads, then a branch-free test
         constexpr int A = 12 + HW * K;  // key span, key mask, domain offset
@@ -414,12 +424,14 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
         // label equal (any label for the wildcard) and degree at least the query node's in one
         // range compare on the masked key: (key & mask) - lo <= span
         uint64_t m = ballot(((d_key & w[A + 1]) - w[0]) <= w[A]) & lanes;
-        if (m) {
+        // (the prefilter leaves a key match in every live graph for labels below 16 and
+        // degrees below 8, so m is almost never 0 here: the test runs unconditionally)
+        {
           uint32_t acc = 0;
 #pragma unroll
           for (int x = 0; x < 8; ++x) acc |= w[1 + x] & d_pnot[x];
           const uint32_t c0 = d_cyc0 - w[9], c1 = d_cyc1 - w[10];
-          uint32_t fail = ((c0 & c1) & 0x80008000u) ^ 0x80008000u;  // a ring field below the query's
+          uint32_t fail = ((c0 & c1) & k_ring_bits) ^ k_ring_bits;  // a ring field below the query's
           // every shell level at once (an empty query level adds 0 and passes every data node)
           uint64_t sh = 0;
 #pragma unroll
@@ -428,7 +440,7 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
             for (int j = 0; j < HL; ++j)
               sh |= (uint64_t(w[12 + HW * k + 2 * j]) | (uint64_t(w[13 + HW * k + 2 * j]) << 32)) +
                     d_hist[k][j];
-          fail |= uint32_t(sh | (sh >> 32)) & 0x10101010u;
+          fail |= uint32_t(sh | (sh >> 32)) & k_sh_bits;
           m &= ballot((acc | fail) == 0);
         }
         if (lane == 0) *reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(qdom) + w[A + 2]) = m;
@@ -438,12 +450,12 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
         const uint64_t u = (m | T) - S;
         const uint64_t alive = (u | m) & T;
         if (alive != tops) {
-          tops = alive;
-          if (!tops) left = 1;
-          lanes = ballot(mytop < 64 && ((tops >> mytop) & 1)) & node_mask;
+          if (!alive) rend = r + kRecDwords;  // no graph left: this node is the last
+          lanes = ballot(mytop < 64 && ((alive >> mytop) & 1)) & node_mask;
         }
+        tops = alive;
         r += kRecDwords;
-      } while (--left);
+      } while (r != rend);
       if (!tops) continue;
       if (lane < ng && slot_top < 64 && ((tops >> slot_top) & 1)) vbits |= uint64_t(1) << (qu & 63);
       __syncthreads();  // lane 0's qdom writes visible to every lane (one wave: no waiting)
