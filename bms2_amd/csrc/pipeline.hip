@@ -40,7 +40,8 @@ constexpr int kSmallWords = 2 + kSmallNq;
 // degree, or the degree alone for the wildcard), [1..8] path Bloom, [9..10] ring counts, [11]
 // degree | label << 8, [12 + HW k .. 11 + HW (k + 1)] shell level k < K (HW = 2 compact or 4
 // full words), then at A = 12 + HW K: [A] 255 - degree (the range's span), [A + 1] the data key
-// bits compared (0xffff, 0xff for the wildcard), [A + 2] local id * 8 (its domain's byte offset).  Shell words hold the even / odd nibbles of the histogram, one count
+// bits compared (0xffff, 0xff for the wildcard), [A + 2] the node's depth in the matching order * 8
+// (its domain's byte offset).  Shell words hold the even / odd nibbles of the histogram, one count
 // per byte; the screen keeps the data node's counts as 15 - d per byte, so one 64-bit add over
 // eight labels gives q + 15 - d per byte (at most 30: no carry between bytes), with bit 4 set
 // exactly when q > d.
@@ -113,6 +114,7 @@ struct QueryPrepParams {
   uint32_t* qrec;
   uint32_t hist_map;        // 8 labels (4 bits each) of the compact shell histograms
   uint32_t compact;         // 1: shell levels as 2 words over the hist_map labels, 0: 4 words
+  const uint8_t* jorder;    // [Q][32] local query node matched at each DFS depth
 };
 
 // Shell histogram restricted to the 8 labels of `map` (nibble i = count of label map[i]).  A
@@ -181,7 +183,14 @@ __global__ __launch_bounds__(64) void query_prep_kernel(QueryPrepParams p) {
   }
   r[aux + 0] = 255u - qd;                 // span of the key range
   r[aux + 1] = wild ? 0xffu : 0xffffu;    // key bits compared: degree only for the wildcard
-  r[aux + 2] = local * 8u;                // byte offset of the node's domain in qdom
+  // byte offset of the node's domain in the screen's qdom: its depth in the matching order, so
+  // the domains arrive in the order the pair records carry them
+  const uint8_t* jo = p.jorder + size_t(g) * 32;
+  const uint32_t nqg = p.graph_offsets[g + 1] - p.graph_offsets[g];
+  uint32_t depth = 0;
+  for (uint32_t d = 0; d < nqg; ++d)
+    if (jo[d] == local) depth = d;
+  r[aux + 2] = depth * 8u;
   for (uint32_t w = aux + 3; w < uint32_t(kRecDwords); ++w) r[w] = 0;
 }
 
@@ -401,13 +410,12 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
       const uint32_t gm = __builtin_amdgcn_readlane(gpass, li);
       const uint32_t rb = __builtin_amdgcn_readlane(q_rb, li);
       const uint32_t nq = __builtin_amdgcn_readlane(q_nq, li);
-      // matching order and query adjacency, loaded ahead of the node loop
-      const uint32_t jl = lane < nq ? uint32_t(p.jorder[size_t(qu) * 32 + lane]) : 0u;
+      // query adjacency by depth, loaded ahead of the node loop
       const uint64_t qa8 =
           nq <= uint32_t(kSmallNq) ? ((const BMS2_CONST uint64_t*)(p.qadj8))[qu] : 0;
       uint64_t lanes = ballot((gm & myslotbit) != 0);
       uint64_t tops = lanes & T;
-      // domains by local query node id go to qdom (written by lane 0)
+      // domains by depth in the matching order go to qdom (written by lane 0)
       // nq >= 2 here (the prefilter passes only queries with more than one node); the loop
       // ends after the last node or as soon as no graph is left
       const uint32_t* r = p.qrec + size_t(rb) * kRecDwords;
@@ -459,9 +467,9 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
       if (!tops) continue;
       if (lane < ng && slot_top < 64 && ((tops >> slot_top) & 1)) vbits |= uint64_t(1) << (qu & 63);
       __syncthreads();  // lane 0's qdom writes visible to every lane (one wave: no waiting)
-      // domains in matching order: lane d takes the domain of the node matched at depth d
-      const uint64_t jdom = lane < nq ? qdom[jl] : 0;
-      const uint64_t own = lane < nq ? qdom[lane] : 0;  // lane = local node id
+      // lane d takes the domain of the node matched at depth d (no vector load on this path: a
+      // wait on one would also wait for every store still in flight)
+      const uint64_t jdom = lane < nq ? qdom[lane] : 0;
       // GMCR pairs -> staged records {q | g << 32, rows | nq << 32, domains in matching order}
       uint64_t at = tops;
       while (at) {
@@ -480,8 +488,9 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
         if (lane < nq) rec[2 + lane] = jdom & seg;
         if (p.domsum) {
           if (lane < nq)
-            atomicAdd(reinterpret_cast<unsigned long long*>(p.domsum + rb + lane),
-                      (unsigned long long)popc64(own & seg));
+            atomicAdd(reinterpret_cast<unsigned long long*>(
+                          p.domsum + rb + p.jorder[size_t(qu) * 32 + lane]),
+                      (unsigned long long)popc64(jdom & seg));
           if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(p.qpairs + qu), 1ull);
         }
         if (small)
@@ -1679,7 +1688,7 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
   js.pair_nodes = tr.pair_nodes;
   QueryPrepParams qp{p->query.graph_offsets, p->query.row_offsets, p->query.column_indices,
                      p->query.node_labels, p->d_qpacks, p->d_qpos, p->d_qrec, p->hist_map,
-                     p->hist_compact};
+                     p->hist_compact, p->d_jorder};
   sp.hist_map = p->hist_map;
   const size_t stage_bytes = std::max<size_t>(
       (size_t(kStageSmall) * kSmallWords + size_t(kStageLarge) * p->rec_words_l) * 8,
