@@ -467,38 +467,43 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
       if (!tops) continue;
       if (lane < ng && slot_top < 64 && ((tops >> slot_top) & 1)) vbits |= uint64_t(1) << (qu & 63);
       __syncthreads();  // lane 0's qdom writes visible to every lane (one wave: no waiting)
-      // lane d takes the domain of the node matched at depth d (no vector load on this path: a
-      // wait on one would also wait for every store still in flight)
-      const uint64_t jdom = lane < nq ? qdom[lane] : 0;
-      // GMCR pairs -> staged records {q | g << 32, rows | nq << 32, domains in matching order}
+      // GMCR pairs -> staged records {q | g << 32, header, domains in matching order}: lane w
+      // writes word w of each record.  The header is the query adjacency by depth (its byte 0 is
+      // always 0) and nq for small records, the pack's first adjacency row and nq for large
+      // ones; lane 2 + d holds the domain of the node matched at depth d (read from qdom, no
+      // vector load on this path: a wait on one would also wait for every store in flight).
+      // Lanes past the query's words write 0 (the record's unused domain words).
+      const bool small = nq <= uint32_t(kSmallNq);
+      const uint32_t rw = small ? uint32_t(kSmallWords) : RWL;
+      uint64_t* const stage_q = small ? stage_s : stage_l;
+      uint32_t nrec = small ? ns : nl;
+      const uint32_t d = lane - 2;  // depth of a domain lane (wraps for lanes 0 and 1)
+      uint64_t word = d < nq ? qdom[d] : 0;
+      if (lane == 0) word = uint64_t(qu);
+      if (lane == 1) word = small ? (qa8 | nq) : (uint64_t(rows) | (uint64_t(nq) << 32));
+      const uint64_t open = lane < 2 ? ~uint64_t(0) : 0;  // header words are not masked
       uint64_t at = tops;
       while (at) {
         const int tb = ctz64(at);
         at &= at - 1;
         const uint64_t seg = ballot(mytop == uint32_t(tb));
-        const uint32_t g = __builtin_amdgcn_readlane(mygraph, tb);
-        const bool small = nq <= uint32_t(kSmallNq);
-        uint64_t* rec = small ? stage_s + ns * kSmallWords : stage_l + nl * RWL;
-        if (lane == 0) {
-          rec[0] = uint64_t(qu) | (uint64_t(g) << 32);
-          // small records carry the query adjacency by depth (its byte 0 is always 0) and nq;
-          // large records the pack's first adjacency row and nq
-          rec[1] = small ? (qa8 | nq) : (uint64_t(rows) | (uint64_t(nq) << 32));
-        }
-        if (lane < nq) rec[2 + lane] = jdom & seg;
+        const uint64_t gw = uint64_t(__builtin_amdgcn_readlane(mygraph, tb)) << 32;
+        const uint64_t v = (word & (seg | open)) | (lane == 0 ? gw : 0);
+        if (lane < rw) stage_q[nrec * rw + lane] = v;
         if (p.domsum) {
-          if (lane < nq)
+          if (d < nq)
             atomicAdd(reinterpret_cast<unsigned long long*>(
-                          p.domsum + rb + p.jorder[size_t(qu) * 32 + lane]),
-                      (unsigned long long)popc64(jdom & seg));
+                          p.domsum + rb + p.jorder[size_t(qu) * 32 + d]),
+                      (unsigned long long)popc64(v));
           if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(p.qpairs + qu), 1ull);
         }
-        if (small)
-          ++ns;
-        else
-          ++nl;
+        ++nrec;
         ++pairs;
       }
+      if (small)
+        ns = nrec;
+      else
+        nl = nrec;
       if (ns > uint32_t(kStageSmall - kPackGraphs) || nl > uint32_t(kStageLarge - kPackGraphs)) flush();
     }
     // GMCR bit rows of the block, and count 0 for every pair of the block outside the GMCR
