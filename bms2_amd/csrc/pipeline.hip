@@ -40,8 +40,7 @@ constexpr int kSmallWords = 2 + kSmallNq;
 // degree, or the degree alone for the wildcard), [1..8] path Bloom, [9..10] ring counts, [11]
 // degree | label << 8, [12 + HW k .. 11 + HW (k + 1)] shell level k < K (HW = 2 compact or 4
 // full words), then at A = 12 + HW K: [A] 255 - degree (the range's span), [A + 1] the data key
-// bits compared (0xffff, 0xff for the wildcard), [A + 2] the node's depth in the matching order * 8
-// (its domain's byte offset).  Shell words hold the even / odd nibbles of the histogram, one count
+// bits compared (0xffff, 0xff for the wildcard), [A + 2] the node's depth in the matching order.  Shell words hold the even / odd nibbles of the histogram, one count
 // per byte; the screen keeps the data node's counts as 15 - d per byte, so one 64-bit add over
 // eight labels gives q + 15 - d per byte (at most 30: no carry between bytes), with bit 4 set
 // exactly when q > d.
@@ -183,14 +182,14 @@ __global__ __launch_bounds__(64) void query_prep_kernel(QueryPrepParams p) {
   }
   r[aux + 0] = 255u - qd;                 // span of the key range
   r[aux + 1] = wild ? 0xffu : 0xffffu;    // key bits compared: degree only for the wildcard
-  // byte offset of the node's domain in the screen's qdom: its depth in the matching order, so
-  // the domains arrive in the order the pair records carry them
+  // the node's depth in the matching order: the lane of the screen's domain registers that
+  // takes its domain, so the domains arrive in the order the pair records carry them
   const uint8_t* jo = p.jorder + size_t(g) * 32;
   const uint32_t nqg = p.graph_offsets[g + 1] - p.graph_offsets[g];
   uint32_t depth = 0;
   for (uint32_t d = 0; d < nqg; ++d)
     if (jo[d] == local) depth = d;
-  r[aux + 2] = depth * 8u;
+  r[aux + 2] = depth;
   for (uint32_t w = aux + 3; w < uint32_t(kRecDwords); ++w) r[w] = 0;
 }
 
@@ -229,7 +228,6 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
   __shared__ uint64_t adj[64];
   __shared__ uint8_t lbl[64];
   __shared__ __align__(16) uint32_t gsum_lds[kPackGraphs * kSumDwords];
-  __shared__ uint64_t qdom[BMS2_MAX_QUERY_NODES];  // domain of each query node (local id)
   extern __shared__ __align__(16) uint64_t stage[];  // staged records; Bloom scratch first
   const uint32_t lane = lane_id();
   const uint32_t pack = p.pack_begin + blockIdx.x;
@@ -415,7 +413,8 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
           nq <= uint32_t(kSmallNq) ? ((const BMS2_CONST uint64_t*)(p.qadj8))[qu] : 0;
       uint64_t lanes = ballot((gm & myslotbit) != 0);
       uint64_t tops = lanes & T;
-      // domains by depth in the matching order go to qdom (written by lane 0)
+      // the domain of the node matched at depth d goes to lane d of dom_lo / dom_hi
+      uint32_t dom_lo = 0, dom_hi = 0;
       // nq >= 2 here (the prefilter passes only queries with more than one node); the loop
       // ends after the last node or as soon as no graph is left
       const uint32_t* r = p.qrec + size_t(rb) * kRecDwords;
@@ -451,7 +450,18 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
           fail |= uint32_t(sh | (sh >> 32)) & k_sh_bits;
           m &= ballot((acc | fail) == 0);
         }
-        if (lane == 0) *reinterpret_cast<uint64_t*>(reinterpret_cast<char*>(qdom) + w[A + 2]) = m;
+        // v_writelane ignores the exec mask; its lane select comes from m0 (with the data in an
+        // SGPR too, one operand must be m0).  The compiler keeps nothing in m0 in this kernel (no
+        // LDS DMA, no relative register moves: every m0 reference in its code is this block's), so
+        // clobbering the reserved register is safe here.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+        asm volatile(
+            "s_mov_b32 m0, %4\n\ts_nop 0\n\tv_writelane_b32 %0, %2, m0\n\tv_writelane_b32 %1, %3, m0"
+            : "+v"(dom_lo), "+v"(dom_hi)
+            : "s"(uint32_t(m)), "s"(uint32_t(m >> 32)), "s"(w[A + 2])
+            : "m0");
+#pragma clang diagnostic pop
         // Graphs left with a candidate: per segment, (m | top) - first has its top bit set
         // unless every non-top bit of m is clear, in which case m's own top bit decides.
         // m has bits only in lanes of live graphs, so a dead graph's top stays clear.
@@ -466,19 +476,21 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
       } while (r != rend);
       if (!tops) continue;
       if (lane < ng && slot_top < 64 && ((tops >> slot_top) & 1)) vbits |= uint64_t(1) << (qu & 63);
-      __syncthreads();  // lane 0's qdom writes visible to every lane (one wave: no waiting)
       // GMCR pairs -> staged records {q | g << 32, header, domains in matching order}: lane w
       // writes word w of each record.  The header is the query adjacency by depth (its byte 0 is
       // always 0) and nq for small records, the pack's first adjacency row and nq for large
-      // ones; lane 2 + d holds the domain of the node matched at depth d (read from qdom, no
-      // vector load on this path: a wait on one would also wait for every store in flight).
+      // ones; lane 2 + d holds the domain of the node matched at depth d (permuted from the
+      // domain registers, no vector load on this path: a wait on one would also wait for every
+      // store in flight).
       // Lanes past the query's words write 0 (the record's unused domain words).
       const bool small = nq <= uint32_t(kSmallNq);
       const uint32_t rw = small ? uint32_t(kSmallWords) : RWL;
       uint64_t* const stage_q = small ? stage_s : stage_l;
       uint32_t nrec = small ? ns : nl;
       const uint32_t d = lane - 2;  // depth of a domain lane (wraps for lanes 0 and 1)
-      uint64_t word = d < nq ? qdom[d] : 0;
+      const uint32_t wlo = uint32_t(__builtin_amdgcn_ds_bpermute(int(d * 4), int(dom_lo)));
+      const uint32_t whi = uint32_t(__builtin_amdgcn_ds_bpermute(int(d * 4), int(dom_hi)));
+      uint64_t word = d < nq ? (uint64_t(whi) << 32) | wlo : 0;
       if (lane == 0) word = uint64_t(qu);
       if (lane == 1) word = small ? (qa8 | nq) : (uint64_t(rows) | (uint64_t(nq) << 32));
       const uint64_t open = lane < 2 ? ~uint64_t(0) : 0;  // header words are not masked
