@@ -351,6 +351,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="time the plain leg only")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--no-dedup", action="store_true",
+                    help="screen every query (default: one query per isomorphism class, the "
+                         "others copy its results)")
     ap.add_argument("--skip-plain", action="store_true",
                     help="time the learned leg only (profiling runs)")
     ap.add_argument("--no-staged", action="store_true",
@@ -392,27 +395,37 @@ def main() -> int:
     dq = api.DeviceBatchedCSRGraph(q, queries)
     dd = api.DeviceBatchedCSRGraph(q, data)
     pipe = api.Pipeline(q, dq, dd, refinement_steps=K, find_first=args.find_first)
+    # isomorphism classes of the query batch (host, computed when the batch is bound; timed here
+    # on the same function)
+    t0 = time.perf_counter()
+    reps = api.query_classes(queries)
+    classes_ms = 1000 * (time.perf_counter() - t0)
+    ndup = pipe.set_dedup(not args.no_dedup)
+    dedup_info = {"on": not args.no_dedup, "queries_copied": int((reps != np.arange(len(reps))).sum()),
+                  "classes": int((reps == np.arange(len(reps))).sum()),
+                  "classes_ms": round(classes_ms, 2)}
     counts, valid, stats = pipe.alloc_outputs()
     q.wait()
     print("[bench] resident batch uploaded; warming up", file=sys.stderr, flush=True)
     D, Q = data.num_graphs, queries.num_graphs
     pairs = D * Q
 
-    def leg(name):
+    def leg(name, steps=None):
+        steps = steps or args.steps
         for _ in range(args.warmup):
             pipe.run(counts, valid, stats)
         q.wait()
         evs = []
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             _, kev = pipe.run(counts, valid, stats, kernel_event=True)
             evs.append(kev)
         q.wait()
         t = time.perf_counter() - t0
         st = stats.to_numpy(np.uint64, 8)
         kms = [e.elapsed_ms() for e in evs]
-        print(f"[bench] {name}: {args.steps} steps, {1000 * t / args.steps:.2f} ms/step, "
-              f"{pairs * args.steps / t:.4g} pairs/s, device span {np.mean(kms):.2f} ms, DFS nodes "
+        print(f"[bench] {name}: {steps} steps, {1000 * t / steps:.2f} ms/step, "
+              f"{pairs * steps / t:.4g} pairs/s, device span {np.mean(kms):.2f} ms, DFS nodes "
               f"{int(st[3])}", file=sys.stderr, flush=True)
         return t, kms, st
 
@@ -425,10 +438,31 @@ def main() -> int:
     order = "identity"
     S = min(args.cpu_sample_graphs or D, D)
     plain_sample = None
+    nodedup = None
     if not (args.skip_plain and not args.no_learned):
+        if ndup and not args.no_dedup:
+            # the plain pipeline screening every query (no class copies), for reference
+            pipe.set_dedup(False)
+            nsteps = min(args.steps, 3)
+            t_nd, kms_nd, st_nd = leg("plain (identity order), every query screened", nsteps)
+            nd_sample = pipe.results(counts, valid, rows=S)
+            pipe.set_dedup(True)
+            nodedup = {"value": round(pairs * nsteps / t_nd, 1),
+                       "ms_per_step": round(1000 * t_nd / nsteps, 3),
+                       "kernel_ms_avg": round(float(np.mean(kms_nd)), 3),
+                       "dfs_nodes": int(st_nd[3]), "steps": nsteps}
         t_plain, kms_plain, st_plain = leg("plain (identity order)")
         # plain counts of the leading graphs; the learned leg must reproduce them (and the totals)
         plain_sample = pipe.results(counts, valid, rows=S)
+        if nodedup is not None:
+            same_nd = (np.array_equal(nd_sample[0], plain_sample[0]) and
+                       np.array_equal(nd_sample[1], plain_sample[1]) and
+                       all(int(st_nd[k]) == int(st_plain[k]) for k in range(3)))
+            del nd_sample
+            nodedup["counts_equal"] = bool(same_nd)
+            if not same_nd:
+                print("[bench] runs with and without class copies differ", file=sys.stderr, flush=True)
+                return 1
     plan_info = None
     if not args.no_learned:
         from bms2_amd import learned
@@ -551,13 +585,17 @@ def main() -> int:
             "value": round(plain_value, 1), "ms_per_step": round(1000 * t_plain / args.steps, 3),
             "kernel_ms_avg": round(float(np.mean(kms_plain)), 3), "dfs_nodes": int(st_plain[3])},
         "learned_vs_plain": None if plain_value is None else round(value / plain_value, 4),
+        # isomorphic queries of the batch: one per class is screened and joined, the others copy
+        "dedup": dedup_info,
+        "plain_every_query": nodedup,
+        "learned_vs_plain_every_query": None if nodedup is None else round(value / nodedup["value"], 4),
         # one pass over the batch with the planning included (the plan is made once per query
         # batch and reused by every later run, which `value` measures)
         "learned_single_pass": None if plan_info is None else {
-            "value": round(pairs / ((plan_info["plan_ms"] + ms_per_step) / 1e3), 1),
-            "ms": round(plan_info["plan_ms"] + ms_per_step, 3),
+            "value": round(pairs / ((plan_info["plan_ms"] + classes_ms + ms_per_step) / 1e3), 1),
+            "ms": round(plan_info["plan_ms"] + classes_ms + ms_per_step, 3),
             "vs_plain": None if plain_value is None else round(
-                (pairs / ((plan_info["plan_ms"] + ms_per_step) / 1e3)) / plain_value, 4),
+                (pairs / ((plan_info["plan_ms"] + classes_ms + ms_per_step) / 1e3)) / plain_value, 4),
             "break_even_runs": None if plain_value is None or value <= plain_value else round(
                 plan_info["plan_ms"] / (1000 * t_plain / args.steps - ms_per_step), 1)},
         "learned_plan": plan_info,

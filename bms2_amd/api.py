@@ -548,6 +548,13 @@ class Pipeline(_Handle):
                     big_row_words=t.big_row_words, candidates=t.candidates,
                     best_ms=round(float(t.best_ms), 3))
 
+    def set_dedup(self, on: bool) -> int:
+        """Screen one query per isomorphism class and copy its outputs to the others (on, the
+        default) or screen every query (off); returns the number of queries that copy."""
+        n = C.c_uint32(0)
+        check(_L().bms2_pipeline_set_dedup(self.h, int(bool(on)), C.byref(n)))
+        return int(n.value)
+
     def set_order(self, order: np.ndarray | None) -> None:
         """Join matching order: order[q, d] = local query node matched at depth d (u8 [Q, 32]);
         None restores identity."""
@@ -650,11 +657,22 @@ def run_staged(q: Queue, query: GraphBatch, data: GraphBatch, K: int, find_first
     return out
 
 
-def run_pipeline(q: Queue, query: GraphBatch, data: GraphBatch, K: int, find_first: bool = False):
+def query_classes(query: GraphBatch) -> np.ndarray:
+    """rep[g] = lowest-indexed query of the batch isomorphic to query g (bms2_query_classes)."""
+    rep = np.zeros(max(query.num_graphs, 1), np.uint32)
+    n = C.c_uint32(0)
+    check(_L().bms2_query_classes(C.byref(query.to_c()), rep.ctypes.data_as(C.c_void_p),
+                                  C.byref(n)))
+    return rep[:query.num_graphs]
+
+
+def run_pipeline(q: Queue, query: GraphBatch, data: GraphBatch, K: int, find_first: bool = False,
+                 dedup: bool = True):
     """Fused pipeline: (counts [D][Q] u64, valid bit rows [D][ceil(Q/64)], stats[8])."""
     dq = DeviceBatchedCSRGraph(q, query)
     dd = DeviceBatchedCSRGraph(q, data)
     p = Pipeline(q, dq, dd, K, find_first)
+    p.set_dedup(dedup)
     counts, valid, stats = p.alloc_outputs()
     p.run(counts, valid, stats)
     q.wait()
@@ -666,5 +684,6 @@ __all__ = [
     "Bms2Error", "Queue", "DeviceBuffer", "DeviceBatchedCSRGraph", "create_device_csr_graph",
     "Signature", "PathSignature", "CycleSignature", "Candidates", "filter_candidates",
     "refine_candidates", "GMCR", "join_candidates", "Pipeline", "run_staged", "run_pipeline",
+    "query_classes",
     "SCOPE_DATA", "SCOPE_QUERY",
 ]

@@ -33,4 +33,9 @@ struct CsrView {
 // message set.  `max_nodes_out` receives the largest graph size.
 int validate_host_csr(const bms2_host_csr* g, uint32_t max_graph_nodes, uint32_t* max_nodes_out);
 
+// Isomorphism classes of a query batch (labels and adjacency; edge labels ignored): rep[g] = the
+// lowest-indexed query isomorphic to query g, proven by an explicit isomorphism.  Returns the
+// number of queries with rep[g] != g.
+uint32_t query_classes(const bms2_host_csr* q, uint32_t* rep);
+
 }  // namespace bms2

@@ -195,9 +195,11 @@ __global__ __launch_bounds__(64) void query_prep_kernel(QueryPrepParams p) {
 
 // Per-query prefilter record (kPreDwords): [0..3] presence bits every node needs (label < 16 at
 // its degree threshold, capped at 7), [4..11] union of the nodes' path Bloom filters, [12..15]
-// the largest of each ring-count field over the nodes, [16] nq > 1, [17] first record, [18] nq.
+// the largest of each ring-count field over the nodes, [16] screened (nq > 1, and with qrep the
+// query is its class's first), [17] first record, [18] nq.
 __global__ void query_pre_kernel(const uint32_t* __restrict__ qhead, const uint32_t* __restrict__ qrec,
-                                 uint32_t Q, uint32_t* __restrict__ qpre) {
+                                 uint32_t Q, const uint32_t* __restrict__ qrep,
+                                 uint32_t* __restrict__ qpre) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= Q) return;
   const uint32_t rb = qhead[4 * q], nq = qhead[4 * q + 1];
@@ -215,7 +217,7 @@ __global__ void query_pre_kernel(const uint32_t* __restrict__ qhead, const uint3
     out[14] = max(out[14], r[10] & 0xffffu);
     out[15] = max(out[15], r[10] >> 16);
   }
-  out[16] = nq > 1 ? 1u : 0u;
+  out[16] = nq > 1 && (!qrep || qrep[q] == q) ? 1u : 0u;
   out[17] = rb;
   out[18] = nq;
   for (int w = 0; w < kPreDwords; ++w) qpre[size_t(q) * kPreDwords + w] = out[w];
@@ -926,6 +928,48 @@ __global__ void zero_kernel(uint64_t* p, uint32_t n) {
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
+// Outputs of the queries that were not screened because an isomorphic query earlier in the batch
+// was: each copies its class representative's count and GMCR bit in every data graph.  One
+// thread per (data graph, output word holding such queries); words[j] is the word, pairs
+// [begin[j], begin[j + 1]) its {query, representative} pairs.  The screen left these queries'
+// counts 0 and their GMCR bits clear.  stats[0..2] += GMCR pairs, matched pairs, embeddings.
+__global__ __launch_bounds__(256) void copy_classes_kernel(
+    uint64_t* __restrict__ counts, uint64_t* __restrict__ valid, uint32_t Q, uint32_t VW,
+    uint64_t D, const uint32_t* __restrict__ words, const uint32_t* __restrict__ begin,
+    const uint2* __restrict__ pairs, uint32_t nw, uint64_t* __restrict__ stats) {
+  const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint64_t gp = 0, mp = 0, emb = 0;
+  if (t < D * nw) {
+    const uint64_t g = t / nw;
+    const uint32_t j = uint32_t(t - g * nw);
+    const uint64_t* vrow = valid + g * VW;
+    uint64_t* crow = counts + g * Q;
+    uint64_t bits = 0;
+    for (uint32_t i = begin[j], e = begin[j + 1]; i < e; ++i) {
+      const uint2 pr = pairs[i];  // x: query, y: its representative
+      const uint64_t c = crow[pr.y];
+      crow[pr.x] = c;
+      const uint64_t vb = (vrow[pr.y >> 6] >> (pr.y & 63)) & 1u;
+      bits |= vb << (pr.x & 63);
+      gp += vb;
+      mp += c != 0;
+      emb += c;
+    }
+    if (bits) valid[g * VW + words[j]] |= bits;
+  }
+  // one atomic per wave and statistic
+  for (int o = 32; o > 0; o >>= 1) {
+    gp += __shfl_xor(gp, o);
+    mp += __shfl_xor(mp, o);
+    emb += __shfl_xor(emb, o);
+  }
+  if (lane_id() == 0 && (gp | mp | emb)) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(stats + 0), (unsigned long long)gp);
+    atomicAdd(reinterpret_cast<unsigned long long*>(stats + 1), (unsigned long long)mp);
+    atomicAdd(reinterpret_cast<unsigned long long*>(stats + 2), (unsigned long long)emb);
+  }
+}
+
 // ------------------------------------------------------------------ host side
 
 // Greedy packing of consecutive graphs: at most 64 nodes and kPackGraphs graphs per pack.  A
@@ -1088,6 +1132,15 @@ struct bms2_pipeline_s {
   uint64_t* d_ctl = nullptr;
   // host copy of the query batch (matching orders are derived from it)
   std::vector<uint32_t> qgo, qro, qci;
+  // isomorphism classes of the query batch: only each class's first query is screened and
+  // joined, the others copy its outputs (copy_classes_kernel) unless dedup is off
+  bool dedup = true;
+  uint32_t num_dups = 0, dup_words = 0;
+  uint32_t* d_qrep = nullptr;       // [Q] first query of each query's class
+  uint32_t* d_dup_words = nullptr;  // output words holding copied queries
+  uint32_t* d_dup_begin = nullptr;  // [dup_words + 1] ranges of d_dup_pairs
+  uint2* d_dup_pairs = nullptr;     // {query, representative}, ascending query
+  uint64_t* d_dup_stats = nullptr;  // [4] statistics of the copied pairs
   // data graphs above 64 nodes: their own compact batch through the staged kernels
   uint32_t num_big = 0;
   bms2_device_csr big{};
@@ -1178,6 +1231,11 @@ void pipeline_free(bms2_pipeline p) {
   (void)hipFree(p->d_segs);
   (void)hipFree(p->d_recs_l);
   (void)hipFree(p->d_ctl);
+  (void)hipFree(p->d_qrep);
+  (void)hipFree(p->d_dup_words);
+  (void)hipFree(p->d_dup_begin);
+  (void)hipFree(p->d_dup_pairs);
+  (void)hipFree(p->d_dup_stats);
   drop_graphs(p);
   (void)hipFree(p->d_big_gid);
   (void)hipFree(p->d_big_list);
@@ -1426,8 +1484,12 @@ int check_queries(const bms2_device_csr* query) {
 void free_query_side(bms2_pipeline_s* p) {
   for (void* b : {(void*)p->d_qpacks, (void*)p->d_qhead, (void*)p->d_qpre, (void*)p->d_qrec,
                   (void*)p->d_qpos, (void*)p->d_jorder, (void*)p->d_qadj, (void*)p->d_qadj8,
-                  (void*)p->d_big_qlev, (void*)p->d_big_qrec})
+                  (void*)p->d_big_qlev, (void*)p->d_big_qrec, (void*)p->d_qrep,
+                  (void*)p->d_dup_words, (void*)p->d_dup_begin, (void*)p->d_dup_pairs})
     (void)hipFree(b);
+  p->d_qrep = p->d_dup_words = p->d_dup_begin = nullptr;
+  p->d_dup_pairs = nullptr;
+  p->num_dups = p->dup_words = 0;
   p->d_qpacks = p->d_qhead = p->d_qpre = p->d_qrec = p->d_qpos = p->d_qadj = nullptr;
   p->d_jorder = nullptr;
   p->d_qadj8 = nullptr;
@@ -1515,6 +1577,45 @@ int bind_queries(bms2_pipeline_s* p, const bms2_device_csr* query) {
   BMS2_HIP(hipMemcpyAsync(p->d_qpacks, qpacks.data(), 4ull * qpacks.size(), hipMemcpyHostToDevice, s));
   BMS2_HIP(hipMemcpyAsync(p->d_qpos, qpos.data(), 4ull * qpos.size(), hipMemcpyHostToDevice, s));
   if (Q) BMS2_HIP(hipMemcpyAsync(p->d_qhead, qhead.data(), 4ull * qhead.size(), hipMemcpyHostToDevice, s));
+  // isomorphism classes: per output word of the GMCR rows, the queries that copy their class's
+  // first query (ascending), uploaded for copy_classes_kernel
+  std::vector<uint32_t> qrep(std::max<uint32_t>(Q, 1)), dup_words, dup_begin;
+  std::vector<uint2> dup_pairs;
+  {
+    bms2_host_csr hv{};
+    hv.num_graphs = Q;
+    hv.total_nodes = Qn;
+    hv.total_edges = qci.size();
+    hv.graph_offsets = qgo.data();
+    hv.row_offsets = qro.data();
+    hv.column_indices = qci.data();
+    hv.node_labels = qlab.data();
+    p->num_dups = query_classes(&hv, qrep.data());
+    for (uint32_t g = 0; g < Q; ++g) {
+      if (qrep[g] == g) continue;
+      if (dup_words.empty() || dup_words.back() != g / 64) {
+        dup_words.push_back(g / 64);
+        dup_begin.push_back(uint32_t(dup_pairs.size()));
+      }
+      dup_pairs.push_back(make_uint2(g, qrep[g]));
+    }
+    dup_begin.push_back(uint32_t(dup_pairs.size()));
+    p->dup_words = uint32_t(dup_words.size());
+  }
+  if (p->num_dups) {
+    if ((rc = device_alloc((void**)&p->d_qrep, 4ull * qrep.size())) ||
+        (rc = device_alloc((void**)&p->d_dup_words, 4ull * dup_words.size())) ||
+        (rc = device_alloc((void**)&p->d_dup_begin, 4ull * dup_begin.size())) ||
+        (rc = device_alloc((void**)&p->d_dup_pairs, sizeof(uint2) * dup_pairs.size())))
+      return rc;
+    BMS2_HIP(hipMemcpyAsync(p->d_qrep, qrep.data(), 4ull * qrep.size(), hipMemcpyHostToDevice, s));
+    BMS2_HIP(hipMemcpyAsync(p->d_dup_words, dup_words.data(), 4ull * dup_words.size(),
+                            hipMemcpyHostToDevice, s));
+    BMS2_HIP(hipMemcpyAsync(p->d_dup_begin, dup_begin.data(), 4ull * dup_begin.size(),
+                            hipMemcpyHostToDevice, s));
+    BMS2_HIP(hipMemcpyAsync(p->d_dup_pairs, dup_pairs.data(), sizeof(uint2) * dup_pairs.size(),
+                            hipMemcpyHostToDevice, s));
+  }
   p->qgo = std::move(qgo);
   p->qro = std::move(qro);
   p->qci = std::move(qci);
@@ -1596,7 +1697,8 @@ extern "C" int bms2_pipeline_create(bms2_queue q, const bms2_device_csr* query,
   if ((rc = device_alloc((void**)&p->d_packs, 4ull * dp.begins.size())) ||
       (rc = device_alloc((void**)&p->d_pack_gids, 4ull * std::max<size_t>(dp.gids.size(), 1))) ||
       (rc = device_alloc((void**)&p->d_adj, 8ull * 64 * std::max<uint32_t>(p->num_packs, 1))) ||
-      (rc = device_alloc((void**)&p->d_ctl, 8 * kCtlWords)))
+      (rc = device_alloc((void**)&p->d_ctl, 8 * kCtlWords)) ||
+      (rc = device_alloc((void**)&p->d_dup_stats, 8 * 4)))
     return fail(rc);
   BMS2_HIP(hipMemcpyAsync(p->d_packs, dp.begins.data(), 4ull * dp.begins.size(),
                           hipMemcpyHostToDevice, s));
@@ -1706,6 +1808,9 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
   QueryPrepParams qp{p->query.graph_offsets, p->query.row_offsets, p->query.column_indices,
                      p->query.node_labels, p->d_qpacks, p->d_qpos, p->d_qrec, p->hist_map,
                      p->hist_compact, p->d_jorder};
+  // a run screens one query per isomorphism class; traces screen every query (their per-query
+  // accumulators are the planner's input)
+  const bool dedup = p->dedup && p->num_dups && !tr.domsum && !tr.qnodes && !tr.pair_nodes;
   sp.hist_map = p->hist_map;
   const size_t stage_bytes = std::max<size_t>(
       (size_t(kStageSmall) * kSmallWords + size_t(kStageLarge) * p->rec_words_l) * 8,
@@ -1715,7 +1820,7 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
     hipLaunchKernelGGL(query_prep_kernel<K>, dim3(p->num_qpacks), dim3(64), 0, s, qp);
     BMS2_CHECK_LAUNCH("query_prep_kernel");
     hipLaunchKernelGGL(query_pre_kernel, dim3((Q + 255) / 256), dim3(256), 0, s, p->d_qhead,
-                       p->d_qrec, Q, p->d_qpre);
+                       p->d_qrec, Q, dedup ? p->d_qrep : nullptr, p->d_qpre);
     BMS2_CHECK_LAUNCH("query_pre_kernel");
     if (p->num_big) BMS2_HIP(hipEventRecord(p->big_fork, s));  // the big-graph path's inputs
     if (int krc = ks.begin()) return krc;
@@ -1840,6 +1945,21 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
       if (int brc = big_start(p, d_counts, d_valid, g_limit)) return brc;
     }
     if (int brc = big_finish(p, stats)) return brc;
+    if (dedup) {
+      // every screen and join of the run is done on this stream: copy the classes' outputs
+      const uint64_t D = std::min<uint64_t>(g_limit, p->data.num_graphs);
+      const uint64_t threads = D * p->dup_words;
+      hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_dup_stats, 4u);
+      if (threads)
+        hipLaunchKernelGGL(copy_classes_kernel, dim3(uint32_t((threads + 255) / 256)), dim3(256), 0,
+                           s, d_counts, d_valid, Q, (Q + 63) / 64, D, p->d_dup_words,
+                           p->d_dup_begin, p->d_dup_pairs, p->dup_words, p->d_dup_stats);
+      BMS2_CHECK_LAUNCH("copy_classes_kernel");
+      uint64_t dst[4];
+      BMS2_HIP(hipMemcpyAsync(dst, p->d_dup_stats, sizeof(dst), hipMemcpyDeviceToHost, s));
+      BMS2_HIP(hipStreamSynchronize(s));
+      for (int k = 0; k < 3; ++k) stats[k] += dst[k];
+    }
     return ks.end();
   });
 }
@@ -1861,6 +1981,13 @@ extern "C" int bms2_pipeline_run(bms2_pipeline p, uint64_t* d_counts, uint64_t* 
   if (d_stats) BMS2_HIP(hipMemcpyAsync(d_stats, stats, sizeof(stats), hipMemcpyHostToDevice, s));
   if (d_stats) BMS2_HIP(hipStreamSynchronize(s));
   return es.end();
+}
+
+extern "C" int bms2_pipeline_set_dedup(bms2_pipeline p, int on, uint32_t* num_dups) {
+  if (!p) return set_error(BMS2_ERR_INVALID, "pipeline is null");
+  p->dedup = on != 0;
+  if (num_dups) *num_dups = p->num_dups;
+  return BMS2_OK;
 }
 
 extern "C" int bms2_pipeline_set_order(bms2_pipeline p, const uint8_t* order) {

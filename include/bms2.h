@@ -405,6 +405,24 @@ int bms2_pipeline_autotune(bms2_pipeline p, uint64_t* d_counts, uint64_t* d_vali
  * unused alternative. */
 int bms2_pipeline_set_order(bms2_pipeline p, const uint8_t* order);
 
+/* Isomorphic queries of one batch have equal results against every data graph: each node's
+ * filter and refinement fields (label, degree, path Bloom filter, ring counts, shell histograms,
+ * isomorphism.hpp:71-97,142-196) depend on labels and adjacency only, so an isomorphism carries
+ * domains to domains and embeddings to embeddings.  bms2_pipeline_run therefore screens and joins
+ * the lowest-indexed query of each isomorphism class (bms2_query_classes, computed when the batch
+ * is bound) and, after the joins, copies its count and GMCR bit to the class's other queries in
+ * every data graph (inside the timed kernel span).  d_counts, d_valid and stats[0..2] are those of
+ * screening every query; stats[3] counts the DFS nodes actually expanded.  on = 0 screens every
+ * query (traces always do).  *num_dups (nullable) = queries that copy their class's results.  No
+ * counterpart in the reference, which runs every query of a batch. */
+int bms2_pipeline_set_dedup(bms2_pipeline p, int on, uint32_t* num_dups);
+
+/* Host: rep[g] = lowest-indexed query of the batch isomorphic to query g (labels and induced
+ * adjacency preserved; edge labels ignored, as the join ignores them), each class proven by an
+ * explicit isomorphism; queries with parallel edges, self-loops or above 32 nodes, and searches
+ * over budget, stay alone.  *num_dups (nullable) = queries with rep[g] != g. */
+int bms2_query_classes(const bms2_host_csr* q, uint32_t* rep, uint32_t* num_dups);
+
 /* Data packs are planned in windows of BMS2_PACK_WINDOW consecutive data graphs. */
 #define BMS2_PACK_WINDOW 1024u
 

@@ -25,3 +25,24 @@ def molecules_1m_shard():
     """First 3,000 graphs of the 1M-molecule workload with its 1,024 queries (seed 0)."""
     from bms2_amd import graphs
     return graphs.generate("molecules-1m", seed=0, num_data_graphs=3000)
+
+
+def expected_stats(queries, data, K, find_first, rs, dedup=True):
+    """The statistics bms2_pipeline_run reports next to the oracle's `rs` over every query: GMCR
+    pairs, matched pairs and embeddings are those of every query; DFS nodes are those of the
+    queries the run screens, one per isomorphism class when dedup is on (the others copy their
+    class's results), i.e. the oracle's DFS nodes over the classes' first queries."""
+    import numpy as np
+
+    import oracle
+    from bms2_amd import api, graphs
+    if not dedup:
+        return rs
+    rep = api.query_classes(queries)
+    keep = np.flatnonzero(rep == np.arange(len(rep)))
+    if len(keep) == queries.num_graphs:
+        return rs
+    sub = graphs.GraphBatch.from_graphs([queries.graph(int(i)) for i in keep])
+    out = np.array(rs, copy=True)
+    out[3] = oracle.pipeline(sub, data, K, find_first)[2][3]
+    return out

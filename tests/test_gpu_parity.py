@@ -5,6 +5,7 @@ import pytest
 import oracle
 from bms2_amd import _lib, api, graphs
 from bms2_amd.graphs import GraphBatch
+from conftest import expected_stats
 
 pytestmark = pytest.mark.gpu
 
@@ -69,8 +70,8 @@ def test_fused_pipeline_matches_oracle(queue, molecules_2k, K, find_first):
     assert int(stats[0]) == int(rs[0])  # GMCR pairs
     assert int(stats[1]) == int(rs[1])  # matched pairs
     assert int(stats[2]) == int(rs[2])  # embeddings
-    if not find_first:
-        assert int(stats[3]) == int(rs[3])  # DFS nodes (identity order)
+    if not find_first:  # DFS nodes (identity order) of the queries screened
+        assert int(stats[3]) == int(expected_stats(queries, data, K, find_first, rs)[3])
 
 
 def test_fused_equals_staged(queue, molecules_2k):
@@ -89,7 +90,7 @@ def test_fused_1m_shard(queue, molecules_1m_shard):
     rc, rv, rs = oracle.pipeline(queries, data, 6, False)
     assert np.array_equal(valid, rv)
     assert np.array_equal(counts, rc)
-    assert np.array_equal(stats[:4], rs)
+    assert np.array_equal(stats[:4], expected_stats(queries, data, 6, False, rs))
 
 
 @pytest.mark.parametrize("which", ["small", "large"])
@@ -108,7 +109,7 @@ def test_fused_query_size_classes(queue, molecules_1m_shard, which, find_first):
     assert np.array_equal(counts, rc)
     assert int(stats[0]) == int(rs[0]) and int(stats[1]) == int(rs[1])
     if not find_first:
-        assert np.array_equal(stats[:4], rs)
+        assert np.array_equal(stats[:4], expected_stats(qs, sub, 6, False, rs))
 
 
 @pytest.fixture(scope="module")
@@ -131,7 +132,7 @@ def test_fused_uneven_sizes(queue, uneven_2k, K, find_first):
     rc, rv, rs = oracle.pipeline(queries, data, K, find_first)
     assert np.array_equal(valid, rv)
     assert np.array_equal(counts, rc)
-    assert np.array_equal(stats[:4], rs)
+    assert np.array_equal(stats[:4], expected_stats(queries, data, K, find_first, rs))
 
 
 @pytest.fixture(scope="module")
@@ -154,7 +155,7 @@ def test_fused_big_graphs(queue, big_graphs, K, find_first):
     assert rs[0] > 10000  # GMCR pairs: the screen's entries span several query blocks per graph
     assert np.array_equal(valid, rv)
     assert np.array_equal(counts, rc)
-    assert np.array_equal(stats[:4], rs)
+    assert np.array_equal(stats[:4], expected_stats(queries, data, K, find_first, rs))
 
 
 def test_fused_big_graphs_autotuned(queue, big_graphs):
@@ -174,19 +175,20 @@ def test_fused_big_graphs_autotuned(queue, big_graphs):
     queue.wait()
     c, v = pipe.results(counts, valid)
     assert np.array_equal(c, rc) and np.array_equal(v, rv)
-    assert np.array_equal(stats.to_numpy(np.uint64, 8)[:4], rs)
+    assert np.array_equal(stats.to_numpy(np.uint64, 8)[:4], expected_stats(queries, data, 6, False, rs))
 
 
 def test_fused_big_graphs_buffers_grow(queue):
     """600 queries of 2-3 carbons against 40 graphs of 65-250 nodes: every graph holds far more
     GMCR pairs than the big-graph screen's first buffers (128 entries per graph), so the run
-    sizes its buffers from the counters and redoes the screen; results equal the oracle's."""
+    sizes its buffers from the counters and redoes the screen; results equal the oracle's.  The
+    queries are copies of two graphs, so every query is screened (dedup off)."""
     data, _ = graphs.generate("uneven-1m", seed=5, num_data_graphs=40, data_min_nodes=65,
                               data_lognormal_mu=4.8, data_lognormal_sigma=0.4)
     queries = GraphBatch.from_graphs([([8, 8], [(0, 1)])] * 300 +
                                      [([8, 8, 8], [(0, 1), (1, 2)])] * 300)
     for find_first in (False, True):
-        counts, valid, stats = api.run_pipeline(queue, queries, data, 2, find_first)
+        counts, valid, stats = api.run_pipeline(queue, queries, data, 2, find_first, dedup=False)
         rc, rv, rs = oracle.pipeline(queries, data, 2, find_first)
         assert rs[0] > 128 * 40  # GMCR pairs beyond the first sizing
         assert np.array_equal(valid, rv)
@@ -211,6 +213,7 @@ def test_fused_big_graphs_new_query_batch(queue, big_graphs):
     pipe.set_queries(dq2)
     counts, valid, stats = pipe.alloc_outputs()
     rc, rv, rs = oracle.pipeline(second, data, 6, False)
+    rs = expected_stats(second, data, 6, False, rs)
     for _ in range(2):
         stats.zero()
         pipe.run(counts, valid, stats)
@@ -254,6 +257,7 @@ def test_graph_replays_match_eager(queue, molecules_1m_shard, uneven_2k, workloa
     the pipeline that never captures."""
     data, queries = molecules_1m_shard if workload == "molecules" else uneven_2k
     rc, rv, rs = oracle.pipeline(queries, data, 6, False)
+    rs = expected_stats(queries, data, 6, False, rs)
     dq = api.DeviceBatchedCSRGraph(queue, queries)
     dd = api.DeviceBatchedCSRGraph(queue, data)
     eager = api.Pipeline(queue, dq, dd, refinement_steps=6, use_graph=False)
@@ -280,6 +284,7 @@ def test_graph_replays_match_eager(queue, molecules_1m_shard, uneven_2k, workloa
 def test_autotune_keeps_results(queue, uneven_2k):
     data, queries = uneven_2k
     rc, rv, rs = oracle.pipeline(queries, data, 2, False)
+    rs = expected_stats(queries, data, 2, False, rs)
     pipe = api.Pipeline(queue, api.DeviceBatchedCSRGraph(queue, queries),
                         api.DeviceBatchedCSRGraph(queue, data), refinement_steps=2)
     counts, valid, stats = pipe.alloc_outputs()
@@ -335,7 +340,7 @@ def test_shell_histogram_label_sets(queue, qlabels):
         counts, valid, stats = api.run_pipeline(queue, queries, data, K, False)
         rc, rv, rs = oracle.pipeline(queries, data, K, False)
         assert np.array_equal(valid, rv) and np.array_equal(counts, rc)
-        assert np.array_equal(stats[:4], rs)
+        assert np.array_equal(stats[:4], expected_stats(queries, data, K, False, rs))
 
 
 @pytest.fixture(scope="module")
