@@ -666,6 +666,14 @@ def query_classes(query: GraphBatch) -> np.ndarray:
     return rep[:query.num_graphs]
 
 
+def query_orbits(query: GraphBatch) -> np.ndarray:
+    """orbit[v] = lowest local node of node v's automorphism orbit in its query
+    (bms2_query_orbits), one u8 per global query node."""
+    orb = np.zeros(max(query.total_nodes, 1), np.uint8)
+    check(_L().bms2_query_orbits(C.byref(query.to_c()), orb.ctypes.data_as(C.c_void_p)))
+    return orb[:query.total_nodes]
+
+
 def run_pipeline(q: Queue, query: GraphBatch, data: GraphBatch, K: int, find_first: bool = False,
                  dedup: bool = True):
     """Fused pipeline: (counts [D][Q] u64, valid bit rows [D][ceil(Q/64)], stats[8])."""
@@ -684,6 +692,6 @@ __all__ = [
     "Bms2Error", "Queue", "DeviceBuffer", "DeviceBatchedCSRGraph", "create_device_csr_graph",
     "Signature", "PathSignature", "CycleSignature", "Candidates", "filter_candidates",
     "refine_candidates", "GMCR", "join_candidates", "Pipeline", "run_staged", "run_pipeline",
-    "query_classes",
+    "query_classes", "query_orbits",
     "SCOPE_DATA", "SCOPE_QUERY",
 ]

@@ -38,4 +38,8 @@ int validate_host_csr(const bms2_host_csr* g, uint32_t max_graph_nodes, uint32_t
 // number of queries with rep[g] != g.
 uint32_t query_classes(const bms2_host_csr* q, uint32_t* rep);
 
+// orbit[u] (u local to query g) = lowest local node of u's orbit under g's label-preserving
+// automorphisms, each proven by an explicit automorphism (u alone when unproven).
+void query_orbits(const bms2_host_csr* q, uint32_t g, uint8_t* orbit);
+
 }  // namespace bms2

@@ -196,13 +196,16 @@ __global__ __launch_bounds__(64) void query_prep_kernel(QueryPrepParams p) {
 // Per-query prefilter record (kPreDwords): [0..3] presence bits every node needs (label < 16 at
 // its degree threshold, capped at 7), [4..11] union of the nodes' path Bloom filters, [12..15]
 // the largest of each ring-count field over the nodes, [16] screened (nq > 1, and with qrep the
-// query is its class's first), [17] first record, [18] nq.
+// query is its class's first), [17] first record, [18] nq, [19] records the screen tests << 24 |
+// for queries of up to kSmallNq nodes the depth whose domain each depth takes, 3 bits per depth
+// (a node automorphic to an earlier-screened one has its domain: its record is not tested).
 __global__ void query_pre_kernel(const uint32_t* __restrict__ qhead, const uint32_t* __restrict__ qrec,
                                  uint32_t Q, const uint32_t* __restrict__ qrep,
+                                 const uint8_t* __restrict__ jorder, const uint8_t* __restrict__ qorb,
                                  uint32_t* __restrict__ qpre) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= Q) return;
-  const uint32_t rb = qhead[4 * q], nq = qhead[4 * q + 1];
+  const uint32_t rb = qhead[4 * q], nq = qhead[4 * q + 1], ntest = qhead[4 * q + 3];
   uint32_t out[kPreDwords] = {};
   for (uint32_t i = 0; i < nq; ++i) {
     const uint32_t* r = qrec + size_t(rb + i) * kRecDwords;
@@ -220,6 +223,14 @@ __global__ void query_pre_kernel(const uint32_t* __restrict__ qhead, const uint3
   out[16] = nq > 1 && (!qrep || qrep[q] == q) ? 1u : 0u;
   out[17] = rb;
   out[18] = nq;
+  uint32_t map = 0;
+  if (nq <= uint32_t(kSmallNq)) {
+    const uint8_t* jo = jorder + size_t(q) * 32;
+    uint32_t inv[kSmallNq];
+    for (uint32_t d = 0; d < nq; ++d) inv[jo[d]] = d;
+    for (uint32_t d = 0; d < nq; ++d) map |= inv[qorb[rb + jo[d]]] << (3 * d);
+  }
+  out[19] = (ntest << 24) | map;
   for (int w = 0; w < kPreDwords; ++w) qpre[size_t(q) * kPreDwords + w] = out[w];
 }
 
@@ -380,12 +391,13 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
     // degree) present in the graph, every node's path bits inside the graph's union and every
     // ring field within the graph's largest (all necessary for a candidate of each node).
     const uint32_t qq = qb + lane;
-    uint32_t gpass = 0, q_rb = 0, q_nq = 0;
+    uint32_t gpass = 0, q_rb = 0, q_nq = 0, q_map = 0;
     if (qq < p.Q) {
       const uint4* pr = reinterpret_cast<const uint4*>(p.qpre + size_t(qq) * kPreDwords);
       const uint4 a = pr[0], b = pr[1], c = pr[2], rg = pr[3], e = pr[4];
       q_rb = e.y;
       q_nq = e.z;
+      q_map = e.w;
       if (e.x) {
         for (uint32_t j = 0; j < ng; ++j) {
           const uint4* gs = reinterpret_cast<const uint4*>(gsum + j * kSumDwords);
@@ -410,6 +422,7 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
       const uint32_t gm = __builtin_amdgcn_readlane(gpass, li);
       const uint32_t rb = __builtin_amdgcn_readlane(q_rb, li);
       const uint32_t nq = __builtin_amdgcn_readlane(q_nq, li);
+      const uint32_t qmap = __builtin_amdgcn_readlane(q_map, li);
       // query adjacency by depth, loaded ahead of the node loop
       const uint64_t qa8 =
           nq <= uint32_t(kSmallNq) ? ((const BMS2_CONST uint64_t*)(p.qadj8))[qu] : 0;
@@ -417,10 +430,11 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
       uint64_t tops = lanes & T;
       // the domain of the node matched at depth d goes to lane d of dom_lo / dom_hi
       uint32_t dom_lo = 0, dom_hi = 0;
-      // nq >= 2 here (the prefilter passes only queries with more than one node); the loop
-      // ends after the last node or as soon as no graph is left
+      // the query's nodes up to the last one not automorphic to an earlier one (at least one:
+      // the prefilter passes only queries with more than one node); the loop ends there or as
+      // soon as no graph is left
       const uint32_t* r = p.qrec + size_t(rb) * kRecDwords;
-      const uint32_t* rend = r + size_t(nq) * kRecDwords;
+      const uint32_t* rend = r + size_t(qmap >> 24) * kRecDwords;
       do {
         // the whole record in one batch of scalar loads, then a branch-free test
         constexpr int A = 12 + HW * K;  // key span, key mask, domain offset
@@ -490,8 +504,10 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
       uint64_t* const stage_q = small ? stage_s : stage_l;
       uint32_t nrec = small ? ns : nl;
       const uint32_t d = lane - 2;  // depth of a domain lane (wraps for lanes 0 and 1)
-      const uint32_t wlo = uint32_t(__builtin_amdgcn_ds_bpermute(int(d * 4), int(dom_lo)));
-      const uint32_t whi = uint32_t(__builtin_amdgcn_ds_bpermute(int(d * 4), int(dom_hi)));
+      // a depth whose node was not tested takes the domain of the automorphic node's depth
+      const uint32_t src = small ? (qmap >> (3 * (d & 7))) & 7u : d;
+      const uint32_t wlo = uint32_t(__builtin_amdgcn_ds_bpermute(int(src * 4), int(dom_lo)));
+      const uint32_t whi = uint32_t(__builtin_amdgcn_ds_bpermute(int(src * 4), int(dom_hi)));
       uint64_t word = d < nq ? (uint64_t(whi) << 32) | wlo : 0;
       if (lane == 0) word = uint64_t(qu);
       if (lane == 1) word = small ? (qa8 | nq) : (uint64_t(rows) | (uint64_t(nq) << 32));
@@ -928,34 +944,69 @@ __global__ void zero_kernel(uint64_t* p, uint32_t n) {
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
+// Tables of the queries that copy their isomorphism class's results (built on the host).
+struct ClassCopy {
+  const uint32_t* reps;   // [nr] queries whose results others copy, ascending
+  const uint2* dups;      // [nd] {copying query, index of its representative in reps}, by index
+  const uint32_t* chunk;  // [ceil(nr / 64) + 1] ranges of dups whose representative index is in
+                          // [64 c, 64 c + 64)
+  uint32_t nr;
+  const uint32_t* words;  // [nw] GMCR-row words holding copying queries
+  const uint32_t* wbegin; // [nw + 1] ranges of wpairs
+  const uint2* wpairs;    // {copying query, representative}, ascending query
+  uint32_t nw;
+};
+
 // Outputs of the queries that were not screened because an isomorphic query earlier in the batch
-// was: each copies its class representative's count and GMCR bit in every data graph.  One
-// thread per (data graph, output word holding such queries); words[j] is the word, pairs
-// [begin[j], begin[j + 1]) its {query, representative} pairs.  The screen left these queries'
-// counts 0 and their GMCR bits clear.  stats[0..2] += GMCR pairs, matched pairs, embeddings.
+// was: each copies its class representative's count and GMCR bit in every data graph.  The screen
+// left their counts 0 and their GMCR bits clear, so only pairs whose representative is in the
+// GMCR are written.  One wave per data graph (grid-stride): lanes load the counts of 64
+// representatives of the graph's row at a time (GMCR ones only), every copying query takes its
+// representative's count from that lane, and the GMCR-row words holding copying queries get
+// their bits.  stats[0..2] += GMCR pairs, matched pairs, embeddings of the copied pairs.
 __global__ __launch_bounds__(256) void copy_classes_kernel(
     uint64_t* __restrict__ counts, uint64_t* __restrict__ valid, uint32_t Q, uint32_t VW,
-    uint64_t D, const uint32_t* __restrict__ words, const uint32_t* __restrict__ begin,
-    const uint2* __restrict__ pairs, uint32_t nw, uint64_t* __restrict__ stats) {
-  const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint64_t D, ClassCopy cc, uint64_t* __restrict__ stats) {
+  const uint32_t lane = lane_id();
+  const uint64_t waves = uint64_t(gridDim.x) * (blockDim.x / 64);
   uint64_t gp = 0, mp = 0, emb = 0;
-  if (t < D * nw) {
-    const uint64_t g = t / nw;
-    const uint32_t j = uint32_t(t - g * nw);
+  for (uint64_t g = uint64_t(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64; g < D; g += waves) {
     const uint64_t* vrow = valid + g * VW;
     uint64_t* crow = counts + g * Q;
-    uint64_t bits = 0;
-    for (uint32_t i = begin[j], e = begin[j + 1]; i < e; ++i) {
-      const uint2 pr = pairs[i];  // x: query, y: its representative
-      const uint64_t c = crow[pr.y];
-      crow[pr.x] = c;
-      const uint64_t vb = (vrow[pr.y >> 6] >> (pr.y & 63)) & 1u;
-      bits |= vb << (pr.x & 63);
-      gp += vb;
-      mp += c != 0;
-      emb += c;
+    for (uint32_t c0 = 0; c0 < cc.nr; c0 += 64) {
+      const uint32_t r = c0 + lane;
+      uint32_t in = 0;
+      uint64_t cnt = 0;
+      if (r < cc.nr) {
+        const uint32_t q = cc.reps[r];
+        in = uint32_t(vrow[q >> 6] >> (q & 63)) & 1u;
+        if (in) cnt = crow[q];
+      }
+      const uint32_t lo = uint32_t(cnt), hi = uint32_t(cnt >> 32);
+      const uint32_t b = cc.chunk[c0 / 64], e = cc.chunk[c0 / 64 + 1];
+      for (uint32_t i0 = b; i0 < e; i0 += 64) {  // uniform trip count: every lane permutes
+        const uint32_t i = i0 + lane;
+        const uint2 pr = i < e ? cc.dups[i] : make_uint2(0, c0);
+        const int src = int(pr.y - c0) * 4;
+        const uint32_t vin = uint32_t(__builtin_amdgcn_ds_bpermute(src, int(in)));
+        const uint64_t v = uint64_t(uint32_t(__builtin_amdgcn_ds_bpermute(src, int(lo)))) |
+                           (uint64_t(uint32_t(__builtin_amdgcn_ds_bpermute(src, int(hi)))) << 32);
+        if (i < e && vin) {
+          crow[pr.x] = v;
+          gp += 1;
+          mp += v != 0;
+          emb += v;
+        }
+      }
     }
-    if (bits) valid[g * VW + words[j]] |= bits;
+    for (uint32_t j = lane; j < cc.nw; j += 64) {
+      uint64_t bits = 0;
+      for (uint32_t i = cc.wbegin[j], e = cc.wbegin[j + 1]; i < e; ++i) {
+        const uint2 pr = cc.wpairs[i];
+        bits |= ((vrow[pr.y >> 6] >> (pr.y & 63)) & 1u) << (pr.x & 63);
+      }
+      if (bits) valid[g * VW + cc.words[j]] |= bits;
+    }
   }
   // one atomic per wave and statistic
   for (int o = 32; o > 0; o >>= 1) {
@@ -963,7 +1014,7 @@ __global__ __launch_bounds__(256) void copy_classes_kernel(
     mp += __shfl_xor(mp, o);
     emb += __shfl_xor(emb, o);
   }
-  if (lane_id() == 0 && (gp | mp | emb)) {
+  if (lane == 0 && (gp | mp | emb)) {
     atomicAdd(reinterpret_cast<unsigned long long*>(stats + 0), (unsigned long long)gp);
     atomicAdd(reinterpret_cast<unsigned long long*>(stats + 1), (unsigned long long)mp);
     atomicAdd(reinterpret_cast<unsigned long long*>(stats + 2), (unsigned long long)emb);
@@ -1135,11 +1186,13 @@ struct bms2_pipeline_s {
   // isomorphism classes of the query batch: only each class's first query is screened and
   // joined, the others copy its outputs (copy_classes_kernel) unless dedup is off
   bool dedup = true;
-  uint32_t num_dups = 0, dup_words = 0;
+  uint32_t num_dups = 0;
   uint32_t* d_qrep = nullptr;       // [Q] first query of each query's class
-  uint32_t* d_dup_words = nullptr;  // output words holding copied queries
-  uint32_t* d_dup_begin = nullptr;  // [dup_words + 1] ranges of d_dup_pairs
-  uint2* d_dup_pairs = nullptr;     // {query, representative}, ascending query
+  uint8_t* d_qorb = nullptr;        // [Qn] lowest local node of each query node's automorphism
+                                    // orbit (queries of up to kSmallNq nodes; itself otherwise)
+  uint32_t* d_copy_u32 = nullptr;   // ClassCopy's u32 tables: reps, chunk, words, wbegin
+  uint2* d_copy_pairs = nullptr;    // ClassCopy's dups, then wpairs
+  ClassCopy copy{};
   uint64_t* d_dup_stats = nullptr;  // [4] statistics of the copied pairs
   // data graphs above 64 nodes: their own compact batch through the staged kernels
   uint32_t num_big = 0;
@@ -1232,9 +1285,9 @@ void pipeline_free(bms2_pipeline p) {
   (void)hipFree(p->d_recs_l);
   (void)hipFree(p->d_ctl);
   (void)hipFree(p->d_qrep);
-  (void)hipFree(p->d_dup_words);
-  (void)hipFree(p->d_dup_begin);
-  (void)hipFree(p->d_dup_pairs);
+  (void)hipFree(p->d_qorb);
+  (void)hipFree(p->d_copy_u32);
+  (void)hipFree(p->d_copy_pairs);
   (void)hipFree(p->d_dup_stats);
   drop_graphs(p);
   (void)hipFree(p->d_big_gid);
@@ -1485,11 +1538,13 @@ void free_query_side(bms2_pipeline_s* p) {
   for (void* b : {(void*)p->d_qpacks, (void*)p->d_qhead, (void*)p->d_qpre, (void*)p->d_qrec,
                   (void*)p->d_qpos, (void*)p->d_jorder, (void*)p->d_qadj, (void*)p->d_qadj8,
                   (void*)p->d_big_qlev, (void*)p->d_big_qrec, (void*)p->d_qrep,
-                  (void*)p->d_dup_words, (void*)p->d_dup_begin, (void*)p->d_dup_pairs})
+                  (void*)p->d_copy_u32, (void*)p->d_copy_pairs, (void*)p->d_qorb})
     (void)hipFree(b);
-  p->d_qrep = p->d_dup_words = p->d_dup_begin = nullptr;
-  p->d_dup_pairs = nullptr;
-  p->num_dups = p->dup_words = 0;
+  p->d_qorb = nullptr;
+  p->d_qrep = p->d_copy_u32 = nullptr;
+  p->d_copy_pairs = nullptr;
+  p->copy = ClassCopy{};
+  p->num_dups = 0;
   p->d_qpacks = p->d_qhead = p->d_qpre = p->d_qrec = p->d_qpos = p->d_qadj = nullptr;
   p->d_jorder = nullptr;
   p->d_qadj8 = nullptr;
@@ -1546,16 +1601,40 @@ int bind_queries(bms2_pipeline_s* p, const bms2_device_csr* query) {
       for (uint32_t i = 0; i < 8; ++i)
         p->hist_map |= (i < used.size() ? used[i] : (used.empty() ? 0 : used[0])) << (4 * i);
   }
+  bms2_host_csr hv{};  // host view of the query batch
+  hv.num_graphs = Q;
+  hv.total_nodes = Qn;
+  hv.total_edges = qci.size();
+  hv.graph_offsets = qgo.data();
+  hv.row_offsets = qro.data();
+  hv.column_indices = qci.data();
+  hv.node_labels = qlab.data();
+  // automorphism orbits of the small queries: a node automorphic to an earlier one has its
+  // domain in every data graph, so the screen tests one node per orbit
+  std::vector<uint8_t> qorb(std::max<uint64_t>(Qn, 1), 0);
+  for (uint32_t g = 0; g < Q; ++g) {
+    const uint32_t b = qgo[g], n = qgo[g + 1] - qgo[g];
+    if (n <= uint32_t(kSmallNq))
+      query_orbits(&hv, g, qorb.data() + b);
+    else
+      for (uint32_t u = 0; u < n; ++u) qorb[b + u] = uint8_t(u);
+  }
   std::vector<uint32_t> qhead(size_t(Q) * 4, 0), qpos(std::max<uint64_t>(Qn, 1), 0);
   for (uint32_t g = 0; g < Q; ++g) {
     const uint32_t b = qgo[g], n = qgo[g + 1] - qgo[g];
     qhead[4 * g + 0] = b;
     qhead[4 * g + 1] = n;
     qhead[4 * g + 2] = n > 1 ? 1 : 0;
-    // screening order: rarest label first, then highest degree (affects speed only)
+    uint32_t ntest = 0;
+    for (uint32_t u = 0; u < n; ++u) ntest += qorb[b + u] == u;
+    qhead[4 * g + 3] = ntest;
+    // screening order: one node per orbit first (the others are not tested), then rarest label
+    // first, then highest degree (affects speed only)
     std::vector<uint32_t> ord(n);
     std::iota(ord.begin(), ord.end(), 0u);
     std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t c) {
+      const bool ca = qorb[b + a] != a, cc = qorb[b + c] != c;
+      if (ca != cc) return cc;
       const uint32_t fa = qlab[b + a] == 255 ? ~0u : label_freq[qlab[b + a]];
       const uint32_t fc = qlab[b + c] == 255 ? ~0u : label_freq[qlab[b + c]];
       if (fa != fc) return fa < fc;
@@ -1572,49 +1651,67 @@ int bind_queries(bms2_pipeline_s* p, const bms2_device_csr* query) {
       (rc = device_alloc((void**)&p->d_qpos, 4ull * qpos.size())) ||
       (rc = device_alloc((void**)&p->d_jorder, std::max<size_t>(size_t(Q) * 32, 32))) ||
       (rc = device_alloc((void**)&p->d_qadj, 4ull * std::max<size_t>(size_t(Q) * 32, 32))) ||
-      (rc = device_alloc((void**)&p->d_qadj8, 8ull * std::max<uint32_t>(Q, 1))))
+      (rc = device_alloc((void**)&p->d_qadj8, 8ull * std::max<uint32_t>(Q, 1))) ||
+      (rc = device_alloc((void**)&p->d_qorb, qorb.size())))
     return rc;
+  BMS2_HIP(hipMemcpyAsync(p->d_qorb, qorb.data(), qorb.size(), hipMemcpyHostToDevice, s));
   BMS2_HIP(hipMemcpyAsync(p->d_qpacks, qpacks.data(), 4ull * qpacks.size(), hipMemcpyHostToDevice, s));
   BMS2_HIP(hipMemcpyAsync(p->d_qpos, qpos.data(), 4ull * qpos.size(), hipMemcpyHostToDevice, s));
   if (Q) BMS2_HIP(hipMemcpyAsync(p->d_qhead, qhead.data(), 4ull * qhead.size(), hipMemcpyHostToDevice, s));
-  // isomorphism classes: per output word of the GMCR rows, the queries that copy their class's
-  // first query (ascending), uploaded for copy_classes_kernel
-  std::vector<uint32_t> qrep(std::max<uint32_t>(Q, 1)), dup_words, dup_begin;
-  std::vector<uint2> dup_pairs;
-  {
-    bms2_host_csr hv{};
-    hv.num_graphs = Q;
-    hv.total_nodes = Qn;
-    hv.total_edges = qci.size();
-    hv.graph_offsets = qgo.data();
-    hv.row_offsets = qro.data();
-    hv.column_indices = qci.data();
-    hv.node_labels = qlab.data();
-    p->num_dups = query_classes(&hv, qrep.data());
+  // isomorphism classes and the tables copy_classes_kernel reads (ClassCopy)
+  std::vector<uint32_t> qrep(std::max<uint32_t>(Q, 1)), cu32;
+  std::vector<uint2> cpairs;
+  p->num_dups = query_classes(&hv, qrep.data());
+  if (p->num_dups) {
+    std::vector<uint32_t> reps, rindex(Q, ~0u), chunk, words, wbegin;
+    std::vector<uint2> dups, wpairs;
+    for (uint32_t g = 0; g < Q; ++g)
+      if (qrep[g] != g && rindex[qrep[g]] == ~0u) rindex[qrep[g]] = 0;
+    for (uint32_t g = 0; g < Q; ++g)
+      if (rindex[g] == 0) {
+        rindex[g] = uint32_t(reps.size());
+        reps.push_back(g);
+      }
     for (uint32_t g = 0; g < Q; ++g) {
       if (qrep[g] == g) continue;
-      if (dup_words.empty() || dup_words.back() != g / 64) {
-        dup_words.push_back(g / 64);
-        dup_begin.push_back(uint32_t(dup_pairs.size()));
+      dups.push_back(make_uint2(g, rindex[qrep[g]]));
+      if (words.empty() || words.back() != g / 64) {
+        words.push_back(g / 64);
+        wbegin.push_back(uint32_t(wpairs.size()));
       }
-      dup_pairs.push_back(make_uint2(g, qrep[g]));
+      wpairs.push_back(make_uint2(g, qrep[g]));
     }
-    dup_begin.push_back(uint32_t(dup_pairs.size()));
-    p->dup_words = uint32_t(dup_words.size());
-  }
-  if (p->num_dups) {
+    wbegin.push_back(uint32_t(wpairs.size()));
+    std::stable_sort(dups.begin(), dups.end(), [](uint2 a, uint2 b) { return a.y < b.y; });
+    const uint32_t nchunks = uint32_t((reps.size() + 63) / 64);
+    for (uint32_t c = 0, i = 0; c <= nchunks; ++c) {
+      while (i < dups.size() && dups[i].y < 64 * c) ++i;
+      chunk.push_back(i);
+    }
+    const size_t o_chunk = reps.size(), o_words = o_chunk + chunk.size(),
+                 o_wbegin = o_words + words.size();
+    cu32 = reps;
+    cu32.insert(cu32.end(), chunk.begin(), chunk.end());
+    cu32.insert(cu32.end(), words.begin(), words.end());
+    cu32.insert(cu32.end(), wbegin.begin(), wbegin.end());
+    cpairs = dups;
+    cpairs.insert(cpairs.end(), wpairs.begin(), wpairs.end());
     if ((rc = device_alloc((void**)&p->d_qrep, 4ull * qrep.size())) ||
-        (rc = device_alloc((void**)&p->d_dup_words, 4ull * dup_words.size())) ||
-        (rc = device_alloc((void**)&p->d_dup_begin, 4ull * dup_begin.size())) ||
-        (rc = device_alloc((void**)&p->d_dup_pairs, sizeof(uint2) * dup_pairs.size())))
+        (rc = device_alloc((void**)&p->d_copy_u32, 4ull * cu32.size())) ||
+        (rc = device_alloc((void**)&p->d_copy_pairs, sizeof(uint2) * cpairs.size())))
       return rc;
     BMS2_HIP(hipMemcpyAsync(p->d_qrep, qrep.data(), 4ull * qrep.size(), hipMemcpyHostToDevice, s));
-    BMS2_HIP(hipMemcpyAsync(p->d_dup_words, dup_words.data(), 4ull * dup_words.size(),
+    BMS2_HIP(hipMemcpyAsync(p->d_copy_u32, cu32.data(), 4ull * cu32.size(), hipMemcpyHostToDevice, s));
+    BMS2_HIP(hipMemcpyAsync(p->d_copy_pairs, cpairs.data(), sizeof(uint2) * cpairs.size(),
                             hipMemcpyHostToDevice, s));
-    BMS2_HIP(hipMemcpyAsync(p->d_dup_begin, dup_begin.data(), 4ull * dup_begin.size(),
-                            hipMemcpyHostToDevice, s));
-    BMS2_HIP(hipMemcpyAsync(p->d_dup_pairs, dup_pairs.data(), sizeof(uint2) * dup_pairs.size(),
-                            hipMemcpyHostToDevice, s));
+    p->copy.reps = p->d_copy_u32;
+    p->copy.chunk = p->d_copy_u32 + o_chunk;
+    p->copy.words = p->d_copy_u32 + o_words;
+    p->copy.wbegin = p->d_copy_u32 + o_wbegin;
+    p->copy.dups = p->d_copy_pairs;
+    p->copy.wpairs = p->d_copy_pairs + dups.size();
+    p->copy.nr = uint32_t(reps.size());
+    p->copy.nw = uint32_t(words.size());
   }
   p->qgo = std::move(qgo);
   p->qro = std::move(qro);
@@ -1820,7 +1917,8 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
     hipLaunchKernelGGL(query_prep_kernel<K>, dim3(p->num_qpacks), dim3(64), 0, s, qp);
     BMS2_CHECK_LAUNCH("query_prep_kernel");
     hipLaunchKernelGGL(query_pre_kernel, dim3((Q + 255) / 256), dim3(256), 0, s, p->d_qhead,
-                       p->d_qrec, Q, dedup ? p->d_qrep : nullptr, p->d_qpre);
+                       p->d_qrec, Q, dedup ? p->d_qrep : nullptr, p->d_jorder, p->d_qorb,
+                       p->d_qpre);
     BMS2_CHECK_LAUNCH("query_pre_kernel");
     if (p->num_big) BMS2_HIP(hipEventRecord(p->big_fork, s));  // the big-graph path's inputs
     if (int krc = ks.begin()) return krc;
@@ -1948,12 +2046,12 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
     if (dedup) {
       // every screen and join of the run is done on this stream: copy the classes' outputs
       const uint64_t D = std::min<uint64_t>(g_limit, p->data.num_graphs);
-      const uint64_t threads = D * p->dup_words;
       hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_dup_stats, 4u);
-      if (threads)
-        hipLaunchKernelGGL(copy_classes_kernel, dim3(uint32_t((threads + 255) / 256)), dim3(256), 0,
-                           s, d_counts, d_valid, Q, (Q + 63) / 64, D, p->d_dup_words,
-                           p->d_dup_begin, p->d_dup_pairs, p->dup_words, p->d_dup_stats);
+      if (D) {
+        const uint32_t blocks = uint32_t(std::min<uint64_t>((D + 3) / 4, uint64_t(p->cus) * 16));
+        hipLaunchKernelGGL(copy_classes_kernel, dim3(blocks), dim3(256), 0, s, d_counts, d_valid,
+                           Q, (Q + 63) / 64, D, p->copy, p->d_dup_stats);
+      }
       BMS2_CHECK_LAUNCH("copy_classes_kernel");
       uint64_t dst[4];
       BMS2_HIP(hipMemcpyAsync(dst, p->d_dup_stats, sizeof(dst), hipMemcpyDeviceToHost, s));

@@ -90,14 +90,16 @@ QueryGraph load(const bms2_host_csr* q, uint32_t g) {
 }
 
 // Backtracking search for a label-, colour- and adjacency-preserving bijection A -> B (edges and
-// non-edges both preserved, as induced matching needs).  False when none exists or the budget
-// runs out.
-bool isomorphic(const QueryGraph& A, const QueryGraph& B) {
+// non-edges both preserved, as induced matching needs).  With fix_a >= 0 the bijection must map
+// node fix_a of A to node fix_b of B.  On success map[u] is the image of A's node u.  False when
+// none exists or the budget runs out.
+bool find_isomorphism(const QueryGraph& A, const QueryGraph& B, int fix_a, int fix_b,
+                      uint32_t* map) {
   const uint32_t n = A.n;
   if (n != B.n || A.m != B.m || A.key != B.key) return false;
   if (n == 0) return true;
-  // A's nodes in search order: rarest colour first, then always a node with the most neighbours
-  // already placed (ties: rarer colour, lower id)
+  // A's nodes in search order: fix_a (else the rarest colour) first, then always a node with the
+  // most neighbours already placed (ties: rarer colour, lower id)
   auto multiplicity = [&](uint64_t c) {
     uint32_t k = 0;
     for (uint32_t u = 0; u < n; ++u) k += A.colour[u] == c;
@@ -107,7 +109,11 @@ bool isomorphic(const QueryGraph& A, const QueryGraph& B) {
   for (uint32_t i = 0; i < n; ++i) {
     uint32_t best = 0, best_links = 0, best_mult = ~0u;
     bool have = false;
-    for (uint32_t u = 0; u < n; ++u) {
+    if (i == 0 && fix_a >= 0) {
+      best = uint32_t(fix_a);
+      have = true;
+    }
+    for (uint32_t u = 0; u < n && !(i == 0 && fix_a >= 0); ++u) {
       if ((placed >> u) & 1u) continue;
       const uint32_t links = __builtin_popcount(A.adj[u] & placed), mult = multiplicity(A.colour[u]);
       if (!have || links > best_links || (links == best_links && mult < best_mult)) {
@@ -125,12 +131,16 @@ bool isomorphic(const QueryGraph& A, const QueryGraph& B) {
   uint32_t used = 0;
   uint64_t steps = 0;
   int i = 0;
-  next_try[0] = 0;
+  next_try[0] = fix_a >= 0 ? uint32_t(fix_b) : 0;
   while (i >= 0) {
-    if (i == int(n)) return true;
+    if (i == int(n)) {
+      for (uint32_t j = 0; j < n; ++j) map[order[j]] = image[j];
+      return true;
+    }
     const uint32_t a = order[i];
+    const uint32_t last = (i == 0 && fix_a >= 0) ? uint32_t(fix_b) + 1 : n;
     bool advanced = false;
-    for (uint32_t b = next_try[i]; b < n; ++b) {
+    for (uint32_t b = next_try[i]; b < last; ++b) {
       if (++steps > kSearchBudget) return false;
       if (((used >> b) & 1u) || B.label[b] != A.label[a] || B.colour[b] != A.colour[a]) continue;
       bool ok = true;
@@ -151,6 +161,11 @@ bool isomorphic(const QueryGraph& A, const QueryGraph& B) {
     }
   }
   return false;
+}
+
+bool isomorphic(const QueryGraph& A, const QueryGraph& B) {
+  uint32_t map[32];
+  return find_isomorphism(A, B, -1, -1, map);
 }
 
 }  // namespace
@@ -179,6 +194,37 @@ uint32_t query_classes(const bms2_host_csr* q, uint32_t* rep) {
   return dups;
 }
 
+// orbit[u] = the lowest local node of u's orbit under the label-preserving automorphisms of query
+// g (every node its own orbit for a query that is not simple or has more than 32 nodes, or when a
+// search runs over budget).  Automorphic nodes have equal filter and refinement fields, hence
+// equal domains in every data graph.
+void query_orbits(const bms2_host_csr* q, uint32_t g, uint8_t* orbit) {
+  const QueryGraph G = load(q, g);
+  const uint32_t n = q->graph_offsets[g + 1] - q->graph_offsets[g];
+  for (uint32_t u = 0; u < n; ++u) orbit[u] = uint8_t(u);
+  if (!G.simple || n < 2) return;
+  uint32_t parent[32];
+  for (uint32_t u = 0; u < n; ++u) parent[u] = u;
+  auto root = [&](uint32_t u) {
+    while (parent[u] != u) u = parent[u] = parent[parent[u]];
+    return u;
+  };
+  uint32_t map[32];
+  for (uint32_t u = 0; u < n; ++u)
+    for (uint32_t v = u + 1; v < n; ++v) {
+      if (G.colour[u] != G.colour[v] || G.label[u] != G.label[v] || root(u) == root(v)) continue;
+      if (!find_isomorphism(G, G, int(u), int(v), map)) continue;
+      for (uint32_t w = 0; w < n; ++w) {  // every pair an automorphism relates shares an orbit
+        const uint32_t a = root(w), b = root(map[w]);
+        if (a != b) parent[std::max(a, b)] = std::min(a, b);
+      }
+    }
+  for (uint32_t u = 0; u < n; ++u) {
+    uint32_t r = root(u);
+    orbit[u] = uint8_t(r);
+  }
+}
+
 }  // namespace bms2
 
 extern "C" int bms2_query_classes(const bms2_host_csr* q, uint32_t* rep, uint32_t* num_dups) {
@@ -186,5 +232,13 @@ extern "C" int bms2_query_classes(const bms2_host_csr* q, uint32_t* rep, uint32_
     return bms2::set_error(BMS2_ERR_INVALID, "null query batch or output");
   const uint32_t d = bms2::query_classes(q, rep);
   if (num_dups) *num_dups = d;
+  return BMS2_OK;
+}
+
+extern "C" int bms2_query_orbits(const bms2_host_csr* q, uint8_t* orbit) {
+  if (!q || !orbit || (q->num_graphs && (!q->graph_offsets || !q->row_offsets || !q->node_labels)))
+    return bms2::set_error(BMS2_ERR_INVALID, "null query batch or output");
+  for (uint32_t g = 0; g < q->num_graphs; ++g)
+    bms2::query_orbits(q, g, orbit + q->graph_offsets[g]);
   return BMS2_OK;
 }

@@ -423,6 +423,13 @@ int bms2_pipeline_set_dedup(bms2_pipeline p, int on, uint32_t* num_dups);
  * over budget, stay alone.  *num_dups (nullable) = queries with rep[g] != g. */
 int bms2_query_classes(const bms2_host_csr* q, uint32_t* rep, uint32_t* num_dups);
 
+/* Host: orbit[v] (v a global node id of the batch) = the lowest local node of v's orbit under its
+ * query's label-preserving automorphisms, each orbit proven by explicit automorphisms (a node
+ * stays alone when a search runs over budget or its query has parallel edges or self-loops).
+ * Automorphic nodes have equal filter and refinement fields, hence equal domains: the fused
+ * screen tests one node per orbit of a query of up to 8 nodes and copies its domain to the rest. */
+int bms2_query_orbits(const bms2_host_csr* q, uint8_t* orbit);
+
 /* Data packs are planned in windows of BMS2_PACK_WINDOW consecutive data graphs. */
 #define BMS2_PACK_WINDOW 1024u
 

@@ -92,3 +92,39 @@ def test_colour_refinement_ties_are_searched():
 def test_single_nodes_and_empty_batch_stay_alone():
     batch = GraphBatch.from_graphs([([8], []), ([8], []), ([8, 8], [(0, 1)]), ([8, 8], [(0, 1)])])
     assert api.query_classes(batch).tolist() == [0, 1, 2, 2]
+
+
+def _networkx_orbits(batch: GraphBatch) -> np.ndarray:
+    out = []
+    for g in range(batch.num_graphs):
+        G = _nx(*batch.graph(g))
+        n = G.number_of_nodes()
+        orb = list(range(n))
+        for s in iso.GraphMatcher(G, G, node_match=lambda a, b: a["l"] == b["l"]).isomorphisms_iter():
+            for u, v in s.items():
+                orb[u] = min(orb[u], v)
+        out.extend(orb)
+    return np.asarray(out, np.uint8)
+
+
+@pytest.mark.parametrize("workload", ["molecules-1m", "uneven-1m"])
+def test_orbits_equal_networkx(workload):
+    """Automorphism orbits (lowest local node of each orbit) equal networkx's, over every
+    automorphism VF2 enumerates."""
+    _, queries = graphs.generate(workload, seed=1, num_data_graphs=200)
+    small = [g for g in range(queries.num_graphs)
+             if queries.graph_offsets[g + 1] - queries.graph_offsets[g] <= 12]
+    batch = GraphBatch.from_graphs([queries.graph(g) for g in small])
+    orb = api.query_orbits(batch)
+    local = np.concatenate([np.arange(n) for n in np.diff(batch.graph_offsets.astype(np.int64))])
+    assert np.array_equal(orb, _networkx_orbits(batch))
+    assert (orb != local).sum() > 50
+
+
+def test_orbits_of_symmetric_shapes():
+    C, O = 8, 7
+    star = ([C, C, C, C, O], [(0, 1), (0, 2), (0, 3), (0, 4)])  # three equal arms, one oxygen
+    ring = ([C] * 6, [(i, (i + 1) % 6) for i in range(6)])
+    chain = ([O, C, C, O], [(0, 1), (1, 2), (2, 3)])
+    batch = GraphBatch.from_graphs([star, ring, chain])
+    assert api.query_orbits(batch).tolist() == [0, 1, 1, 1, 4] + [0] * 6 + [0, 1, 1, 0]
