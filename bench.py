@@ -533,7 +533,7 @@ def main() -> int:
             f"{traffic.get('order', 'learned')}-order run, tree {traffic.get('tree', 'unrecorded')}, "
             f"{traffic.get('measured', 'date unrecorded')} (not this run)") if measured_here else
             "not measured for this configuration",
-        "kernel": f"screen_kernel<{K}> + join_kernel (device span per run, {order} order)",
+        "kernel": f"screen_kernel<{K}> + joins + class copies (device span per run, {order} order)",
         "kernel_ms_avg": round(float(np.mean(kernel_ms)), 3),
         "algorithmic_bytes_per_launch": abytes,
         "bytes_per_pair": round(abytes / pairs, 4),

@@ -948,65 +948,77 @@ __global__ void zero_kernel(uint64_t* p, uint32_t n) {
 struct ClassCopy {
   const uint32_t* reps;   // [nr] queries whose results others copy, ascending
   const uint2* dups;      // [nd] {copying query, index of its representative in reps}, by index
+  const uint32_t* dpos;   // [nd] position of the copying query's GMCR-row word in words
   const uint32_t* chunk;  // [ceil(nr / 64) + 1] ranges of dups whose representative index is in
                           // [64 c, 64 c + 64)
   uint32_t nr;
   const uint32_t* words;  // [nw] GMCR-row words holding copying queries
-  const uint32_t* wbegin; // [nw + 1] ranges of wpairs
-  const uint2* wpairs;    // {copying query, representative}, ascending query
   uint32_t nw;
 };
 
 // Outputs of the queries that were not screened because an isomorphic query earlier in the batch
 // was: each copies its class representative's count and GMCR bit in every data graph.  The screen
 // left their counts 0 and their GMCR bits clear, so only pairs whose representative is in the
-// GMCR are written.  One wave per data graph (grid-stride): lanes load the counts of 64
-// representatives of the graph's row at a time (GMCR ones only), every copying query takes its
-// representative's count from that lane, and the GMCR-row words holding copying queries get
-// their bits.  stats[0..2] += GMCR pairs, matched pairs, embeddings of the copied pairs.
+// GMCR are written.  One wave per data graph (grid-stride): lanes load the GMCR bits and counts of
+// 64 representatives at a time, every copying query takes its representative's from that lane,
+// stores the count and ORs its bit into the wave's LDS copy of the words holding copying
+// queries, which then go to the GMCR row.  stats[0..2] += GMCR pairs, matched pairs, embeddings
+// of the copied pairs.  A wave takes 4 graphs per step, their loads in flight together.  Dynamic
+// LDS: 4 nw u64 per wave.
 __global__ __launch_bounds__(256) void copy_classes_kernel(
     uint64_t* __restrict__ counts, uint64_t* __restrict__ valid, uint32_t Q, uint32_t VW,
     uint64_t D, ClassCopy cc, uint64_t* __restrict__ stats) {
+  constexpr int G = 4;  // graphs per wave and step: their loads are in flight together
+  extern __shared__ uint64_t wbits_all[];
   const uint32_t lane = lane_id();
+  uint64_t* wbits = wbits_all + (threadIdx.x / 64) * cc.nw * G;
+  for (uint32_t j = lane; j < cc.nw * G; j += 64) wbits[j] = 0;
   const uint64_t waves = uint64_t(gridDim.x) * (blockDim.x / 64);
   uint64_t gp = 0, mp = 0, emb = 0;
-  for (uint64_t g = uint64_t(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64; g < D; g += waves) {
-    const uint64_t* vrow = valid + g * VW;
-    uint64_t* crow = counts + g * Q;
+  for (uint64_t g0 = (uint64_t(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64) * G; g0 < D;
+       g0 += waves * G) {
     for (uint32_t c0 = 0; c0 < cc.nr; c0 += 64) {
       const uint32_t r = c0 + lane;
-      uint32_t in = 0;
-      uint64_t cnt = 0;
-      if (r < cc.nr) {
-        const uint32_t q = cc.reps[r];
-        in = uint32_t(vrow[q >> 6] >> (q & 63)) & 1u;
-        if (in) cnt = crow[q];
-      }
-      const uint32_t lo = uint32_t(cnt), hi = uint32_t(cnt >> 32);
+      const uint32_t q = r < cc.nr ? cc.reps[r] : 0;
+      uint32_t in[G];
+      uint64_t cnt[G];
+#pragma unroll
+      for (int k = 0; k < G; ++k)
+        in[k] = (r < cc.nr && g0 + k < D) ? uint32_t(valid[(g0 + k) * VW + (q >> 6)] >> (q & 63)) & 1u : 0u;
+#pragma unroll
+      for (int k = 0; k < G; ++k) cnt[k] = in[k] ? counts[(g0 + k) * Q + q] : 0;
       const uint32_t b = cc.chunk[c0 / 64], e = cc.chunk[c0 / 64 + 1];
       for (uint32_t i0 = b; i0 < e; i0 += 64) {  // uniform trip count: every lane permutes
         const uint32_t i = i0 + lane;
         const uint2 pr = i < e ? cc.dups[i] : make_uint2(0, c0);
+        const uint32_t wp = i < e ? cc.dpos[i] : 0;
         const int src = int(pr.y - c0) * 4;
-        const uint32_t vin = uint32_t(__builtin_amdgcn_ds_bpermute(src, int(in)));
-        const uint64_t v = uint64_t(uint32_t(__builtin_amdgcn_ds_bpermute(src, int(lo)))) |
-                           (uint64_t(uint32_t(__builtin_amdgcn_ds_bpermute(src, int(hi)))) << 32);
-        if (i < e && vin) {
-          crow[pr.x] = v;
-          gp += 1;
-          mp += v != 0;
-          emb += v;
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+          const uint32_t vin = uint32_t(__builtin_amdgcn_ds_bpermute(src, int(in[k])));
+          const uint64_t v = uint64_t(uint32_t(__builtin_amdgcn_ds_bpermute(src, int(uint32_t(cnt[k]))))) |
+                             (uint64_t(uint32_t(__builtin_amdgcn_ds_bpermute(src, int(uint32_t(cnt[k] >> 32))))) << 32);
+          if (i < e && vin) {
+            counts[(g0 + k) * Q + pr.x] = v;
+            atomicOr(reinterpret_cast<unsigned long long*>(wbits + k * cc.nw + wp),
+                     (unsigned long long)(uint64_t(1) << (pr.x & 63)));
+            gp += 1;
+            mp += v != 0;
+            emb += v;
+          }
         }
       }
     }
-    for (uint32_t j = lane; j < cc.nw; j += 64) {
-      uint64_t bits = 0;
-      for (uint32_t i = cc.wbegin[j], e = cc.wbegin[j + 1]; i < e; ++i) {
-        const uint2 pr = cc.wpairs[i];
-        bits |= ((vrow[pr.y >> 6] >> (pr.y & 63)) & 1u) << (pr.x & 63);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (uint32_t j = lane; j < cc.nw * G; j += 64) {
+      const uint32_t k = j / cc.nw, w = j - k * cc.nw;
+      const uint64_t bits = wbits[j];
+      if (bits) {
+        valid[(g0 + k) * VW + cc.words[w]] |= bits;
+        wbits[j] = 0;
       }
-      if (bits) valid[g * VW + cc.words[j]] |= bits;
     }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
   // one atomic per wave and statistic
   for (int o = 32; o > 0; o >>= 1) {
@@ -1662,9 +1674,10 @@ int bind_queries(bms2_pipeline_s* p, const bms2_device_csr* query) {
   std::vector<uint32_t> qrep(std::max<uint32_t>(Q, 1)), cu32;
   std::vector<uint2> cpairs;
   p->num_dups = query_classes(&hv, qrep.data());
+  std::vector<uint32_t> reps, chunk, words, dpos;
+  std::vector<uint2> dups;
   if (p->num_dups) {
-    std::vector<uint32_t> reps, rindex(Q, ~0u), chunk, words, wbegin;
-    std::vector<uint2> dups, wpairs;
+    std::vector<uint32_t> rindex(Q, ~0u), wpos(Q, 0);
     for (uint32_t g = 0; g < Q; ++g)
       if (qrep[g] != g && rindex[qrep[g]] == ~0u) rindex[qrep[g]] = 0;
     for (uint32_t g = 0; g < Q; ++g)
@@ -1675,27 +1688,27 @@ int bind_queries(bms2_pipeline_s* p, const bms2_device_csr* query) {
     for (uint32_t g = 0; g < Q; ++g) {
       if (qrep[g] == g) continue;
       dups.push_back(make_uint2(g, rindex[qrep[g]]));
-      if (words.empty() || words.back() != g / 64) {
-        words.push_back(g / 64);
-        wbegin.push_back(uint32_t(wpairs.size()));
-      }
-      wpairs.push_back(make_uint2(g, qrep[g]));
+      if (words.empty() || words.back() != g / 64) words.push_back(g / 64);
+      wpos[g] = uint32_t(words.size() - 1);
     }
-    wbegin.push_back(uint32_t(wpairs.size()));
     std::stable_sort(dups.begin(), dups.end(), [](uint2 a, uint2 b) { return a.y < b.y; });
+    for (const uint2& d : dups) dpos.push_back(wpos[d.x]);
+    // copy_classes_kernel keeps the copied words of 4 graphs in each of 4 waves in 64 KB of LDS
+    if (words.size() > 512) p->num_dups = 0;
+  }
+  if (p->num_dups) {
     const uint32_t nchunks = uint32_t((reps.size() + 63) / 64);
     for (uint32_t c = 0, i = 0; c <= nchunks; ++c) {
       while (i < dups.size() && dups[i].y < 64 * c) ++i;
       chunk.push_back(i);
     }
     const size_t o_chunk = reps.size(), o_words = o_chunk + chunk.size(),
-                 o_wbegin = o_words + words.size();
+                 o_dpos = o_words + words.size();
     cu32 = reps;
     cu32.insert(cu32.end(), chunk.begin(), chunk.end());
     cu32.insert(cu32.end(), words.begin(), words.end());
-    cu32.insert(cu32.end(), wbegin.begin(), wbegin.end());
+    cu32.insert(cu32.end(), dpos.begin(), dpos.end());
     cpairs = dups;
-    cpairs.insert(cpairs.end(), wpairs.begin(), wpairs.end());
     if ((rc = device_alloc((void**)&p->d_qrep, 4ull * qrep.size())) ||
         (rc = device_alloc((void**)&p->d_copy_u32, 4ull * cu32.size())) ||
         (rc = device_alloc((void**)&p->d_copy_pairs, sizeof(uint2) * cpairs.size())))
@@ -1707,9 +1720,8 @@ int bind_queries(bms2_pipeline_s* p, const bms2_device_csr* query) {
     p->copy.reps = p->d_copy_u32;
     p->copy.chunk = p->d_copy_u32 + o_chunk;
     p->copy.words = p->d_copy_u32 + o_words;
-    p->copy.wbegin = p->d_copy_u32 + o_wbegin;
+    p->copy.dpos = p->d_copy_u32 + o_dpos;
     p->copy.dups = p->d_copy_pairs;
-    p->copy.wpairs = p->d_copy_pairs + dups.size();
     p->copy.nr = uint32_t(reps.size());
     p->copy.nw = uint32_t(words.size());
   }
@@ -2048,9 +2060,9 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
       const uint64_t D = std::min<uint64_t>(g_limit, p->data.num_graphs);
       hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, s, p->d_dup_stats, 4u);
       if (D) {
-        const uint32_t blocks = uint32_t(std::min<uint64_t>((D + 3) / 4, uint64_t(p->cus) * 16));
-        hipLaunchKernelGGL(copy_classes_kernel, dim3(blocks), dim3(256), 0, s, d_counts, d_valid,
-                           Q, (Q + 63) / 64, D, p->copy, p->d_dup_stats);
+        const uint32_t blocks = uint32_t(std::min<uint64_t>((D + 15) / 16, uint64_t(p->cus) * 16));
+        hipLaunchKernelGGL(copy_classes_kernel, dim3(blocks), dim3(256), 4 * 4 * 8 * p->copy.nw, s,
+                           d_counts, d_valid, Q, (Q + 63) / 64, D, p->copy, p->d_dup_stats);
       }
       BMS2_CHECK_LAUNCH("copy_classes_kernel");
       uint64_t dst[4];
