@@ -385,11 +385,14 @@ def _trace(pipe: Pipeline, graphs: int, qn: int, order=None, pair_nodes=False) -
 
 
 def collect_seed(q: Queue, seed: int, num_data_graphs: int = 10000, K: int = 6,
-                 workload: str = "molecules-1m", depth_labels: bool = True) -> SeedTraces:
-    """Plain runs on one seed's workload: candidate orders and depths, as training rows."""
+                 workload: str = "molecules-1m", depth_labels: bool = True,
+                 num_query_graphs: int = 0) -> SeedTraces:
+    """Plain runs on one seed's workload: candidate orders and depths, as training rows.
+    num_query_graphs overrides the workload's query count (0 keeps it)."""
     from . import graphs as G
     from .api import DeviceBatchedCSRGraph
-    data, queries = G.generate(workload, seed=seed, num_data_graphs=num_data_graphs)
+    over = {"num_query_graphs": num_query_graphs} if num_query_graphs else {}
+    data, queries = G.generate(workload, seed=seed, num_data_graphs=num_data_graphs, **over)
     dq, dd = DeviceBatchedCSRGraph(q, queries), DeviceBatchedCSRGraph(q, data)
     qn = queries.total_nodes
     infos = query_infos(queries)
@@ -514,6 +517,8 @@ def _main(argv=None) -> int:
     t = sub.add_parser("train", help="collect traces on seeds and train the order network")
     t.add_argument("--seeds", default="1-20")
     t.add_argument("--graphs-per-seed", type=int, default=10000)
+    t.add_argument("--queries-per-seed", type=int, default=1400,
+                   help="queries generated per seed (1,400 x 20 seeds: about 200,000 query-node rows)")
     t.add_argument("--epochs", type=int, default=40)
     t.add_argument("--out", default=str(DEFAULT_WEIGHTS))
     t.add_argument("--no-depth", action="store_true")
@@ -527,11 +532,12 @@ def _main(argv=None) -> int:
     if args.cmd == "train":
         a, b = (int(x) for x in args.seeds.split("-"))
         Xs, Ts, meta = [], [], dict(seeds=args.seeds, graphs_per_seed=args.graphs_per_seed,
-                                    per_seed=[])
+                                    queries_per_seed=args.queries_per_seed, per_seed=[])
         traced = 0
         for s in range(a, b + 1):
             t0 = time.perf_counter()
-            st = collect_seed(q, s, args.graphs_per_seed, depth_labels=not args.no_depth)
+            st = collect_seed(q, s, args.graphs_per_seed, depth_labels=not args.no_depth,
+                              num_query_graphs=args.queries_per_seed)
             Xs.append(st.X)
             Ts.append(st.T)
             traced += st.pairs_traced
