@@ -178,6 +178,7 @@ PROTOTYPES: dict[str, tuple] = {
     "bms2_pipeline_set_dedup": (i32, [vp, i32, C.POINTER(C.c_uint32)]),
     "bms2_query_classes": (i32, [HCP, vp, C.POINTER(C.c_uint32)]),
     "bms2_query_orbits": (i32, [HCP, vp]),
+    "bms2_query_symmetry": (i32, [HCP, vp, vp, vp]),
     "bms2_pipeline_autotune": (i32, [vp, vp, vp, u32, C.POINTER(Tuning)]),
     "bms2_pipeline_set_queries": (i32, [vp, DCP]),
     "bms2_pipeline_query_totals": (i32, [vp, vp, vp, vp]),

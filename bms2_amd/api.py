@@ -674,6 +674,19 @@ def query_orbits(query: GraphBatch) -> np.ndarray:
     return orb[:query.total_nodes]
 
 
+def query_symmetry(query: GraphBatch, order: np.ndarray | None = None) -> tuple[np.ndarray, np.ndarray]:
+    """(du u8[Q], vmask u32[Q]): the symmetry breaking of bms2_query_symmetry under a matching
+    order (u8 [Q, 32], None = identity)."""
+    Q = query.num_graphs
+    du = np.zeros(max(Q, 1), np.uint8)
+    vm = np.zeros(max(Q, 1), np.uint32)
+    o = None if order is None else np.ascontiguousarray(order, np.uint8)
+    check(_L().bms2_query_symmetry(C.byref(query.to_c()),
+                                   None if o is None else o.ctypes.data_as(C.c_void_p),
+                                   du.ctypes.data_as(C.c_void_p), vm.ctypes.data_as(C.c_void_p)))
+    return du[:Q], vm[:Q]
+
+
 def run_pipeline(q: Queue, query: GraphBatch, data: GraphBatch, K: int, find_first: bool = False,
                  dedup: bool = True):
     """Fused pipeline: (counts [D][Q] u64, valid bit rows [D][ceil(Q/64)], stats[8])."""
@@ -692,6 +705,6 @@ __all__ = [
     "Bms2Error", "Queue", "DeviceBuffer", "DeviceBatchedCSRGraph", "create_device_csr_graph",
     "Signature", "PathSignature", "CycleSignature", "Candidates", "filter_candidates",
     "refine_candidates", "GMCR", "join_candidates", "Pipeline", "run_staged", "run_pipeline",
-    "query_classes", "query_orbits",
+    "query_classes", "query_orbits", "query_symmetry",
     "SCOPE_DATA", "SCOPE_QUERY",
 ]

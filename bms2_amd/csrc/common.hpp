@@ -42,4 +42,10 @@ uint32_t query_classes(const bms2_host_csr* q, uint32_t* rep);
 // automorphisms, each proven by an explicit automorphism (u alone when unproven).
 void query_orbits(const bms2_host_csr* q, uint32_t g, uint8_t* orbit);
 
+// Symmetry breaking of query g under a matching order (order[d] = local node at depth d, null =
+// identity) from its orbits: the candidates at the depths of vmask must exceed the node matched
+// at depth du, and the count is multiplied by popcount(vmask) + 1 (du = 0xff: none).
+void query_symmetry(const bms2_host_csr* q, uint32_t g, const uint8_t* order, const uint8_t* orbit,
+                    uint8_t* du, uint32_t* vmask);
+
 }  // namespace bms2

@@ -95,10 +95,13 @@ struct JoinParams {
   uint32_t rec_words;
   uint32_t Q;
   const uint64_t* adj;      // [Dn]
-  const uint32_t* qadj;     // [Q][32]: bit i of word d = depths i and d adjacent
+  const uint64_t* qadj;     // [Q][32]: bit i < 32 of word d = depths i and d adjacent; with sym,
+                            // bit 37 = depth d's candidates must exceed the node at depth bits
+                            // 32-36, and bits 40-45 of word 0 = the count's multiplier
   uint64_t* counts;
   uint64_t* ctl;
   uint32_t find_first;
+  uint32_t sym;             // apply the symmetry breaking of qadj
   uint64_t* qnodes;         // trace (nullable): [Q] DFS nodes
   uint32_t* pair_nodes;     // trace (nullable): [D][Q] DFS nodes per pair, saturating
 };
@@ -569,7 +572,7 @@ __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
   const uint32_t RW = p.rec_words;
   bool busy = false, exhausted = false;
   uint64_t rec = 0, used = 0, cnt = 0, nodes = 0;
-  uint32_t q = 0, g = 0, n0 = 0, nq = 0;
+  uint32_t q = 0, g = 0, n0 = 0, nq = 0, mult = 1;
   int depth = 0;
   uint64_t st_matched = 0, st_emb = 0, st_nodes = 0;
   const uint64_t* recs = p.recs;
@@ -604,6 +607,7 @@ __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
         cnt = 0;
         nodes = 0;
         R[0][lane] = recs[rec * RW + 2];
+        mult = p.sym ? max(uint32_t(p.qadj[size_t(q) * 32] >> 40) & 63u, 1u) : 1u;
       }
       local_next += take;
     }
@@ -631,11 +635,17 @@ __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
         const int nd = depth + 1;
         const uint64_t vbit = uint64_t(1) << v;
         uint64_t Rn = recs[rec * RW + 2 + nd] & ~(used | vbit);
-        const uint32_t am = p.qadj[size_t(q) * 32 + nd];
+        const uint64_t aw = p.qadj[size_t(q) * 32 + nd];
+        const uint32_t am = uint32_t(aw);
         Rn &= ((am >> depth) & 1) ? nv : ~nv;
         for (int i = 0; i < depth; ++i) {
           const uint64_t a = NF[i][lane];
           Rn &= ((am >> i) & 1) ? a : ~a;
+        }
+        if (p.sym && ((aw >> 37) & 1)) {
+          // symmetry breaking: only candidates above the node matched at the orbit's first depth
+          const uint32_t fu = F[(aw >> 32) & 31][lane];
+          Rn &= ~((uint64_t(2) << fu) - 1);
         }
         if (uint32_t(nd) == nq - 1) {
           cnt += uint64_t(popc64(Rn));
@@ -648,7 +658,7 @@ __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
       }
     }
     if (finished) {
-      const uint64_t c = p.find_first ? (cnt ? 1 : 0) : cnt;
+      const uint64_t c = p.find_first ? (cnt ? 1 : 0) : cnt * mult;
       p.counts[uint64_t(g) * p.Q + q] = c;
       if (p.qnodes) atomicAdd(reinterpret_cast<unsigned long long*>(p.qnodes + q), (unsigned long long)nodes);
       if (p.pair_nodes) p.pair_nodes[uint64_t(g) * p.Q + q] = uint32_t(min(nodes, uint64_t(0xffffffffu)));
@@ -1186,15 +1196,18 @@ struct bms2_pipeline_s {
   uint32_t* d_qrec = nullptr;
   uint32_t* d_qpos = nullptr;
   uint8_t* d_jorder = nullptr;
-  uint32_t* d_qadj = nullptr;
+  uint64_t* d_qadj = nullptr;
   uint64_t* d_adj = nullptr;
   uint64_t* d_qadj8 = nullptr;
   uint64_t* d_recs_s = nullptr;
   uint4* d_segs = nullptr;
   uint64_t* d_recs_l = nullptr;
   uint64_t* d_ctl = nullptr;
-  // host copy of the query batch (matching orders are derived from it)
+  // host copy of the query batch (matching orders are derived from it), a view of it, and the
+  // automorphism orbits of every query (local lowest node per orbit)
   std::vector<uint32_t> qgo, qro, qci;
+  std::vector<uint8_t> qlab, orbit;
+  bms2_host_csr qhv{};
   // isomorphism classes of the query batch: only each class's first query is screened and
   // joined, the others copy its outputs (copy_classes_kernel) unless dedup is off
   bool dedup = true;
@@ -1489,7 +1502,7 @@ int upload_order(bms2_pipeline_s* p, const uint8_t* order) {
   const uint32_t Q = p->query.num_graphs;
   if (!Q) return BMS2_OK;
   std::vector<uint8_t> jorder(size_t(Q) * 32, 0);
-  std::vector<uint32_t> qadj(size_t(Q) * 32, 0);
+  std::vector<uint64_t> qadj(size_t(Q) * 32, 0);
   std::vector<uint64_t> qadj8(Q, 0);
   for (uint32_t g = 0; g < Q; ++g) {
     const uint32_t b = p->qgo[g], n = p->qgo[g + 1] - b;
@@ -1521,10 +1534,21 @@ int upload_order(bms2_pipeline_s* p, const uint8_t* order) {
       qadj[size_t(g) * 32 + d] = w;
       if (n <= uint32_t(kSmallNq)) qadj8[g] |= uint64_t(w) << (8 * d);
     }
+    if (n > uint32_t(kSmallNq)) {
+      // symmetry breaking of the large-query join under this order
+      uint8_t du = 0xff;
+      uint32_t vmask = 0;
+      query_symmetry(&p->qhv, g, o, p->orbit.data() + b, &du, &vmask);
+      if (du != 0xff) {
+        for (uint32_t d = 0; d < n; ++d)
+          if ((vmask >> d) & 1) qadj[size_t(g) * 32 + d] |= (uint64_t(1) << 37) | (uint64_t(du) << 32);
+        qadj[size_t(g) * 32] |= uint64_t(__builtin_popcount(vmask) + 1) << 40;
+      }
+    }
   }
   hipStream_t s = p->q->stream;
   BMS2_HIP(hipMemcpyAsync(p->d_jorder, jorder.data(), jorder.size(), hipMemcpyHostToDevice, s));
-  BMS2_HIP(hipMemcpyAsync(p->d_qadj, qadj.data(), 4ull * qadj.size(), hipMemcpyHostToDevice, s));
+  BMS2_HIP(hipMemcpyAsync(p->d_qadj, qadj.data(), 8ull * qadj.size(), hipMemcpyHostToDevice, s));
   BMS2_HIP(hipMemcpyAsync(p->d_qadj8, qadj8.data(), 8ull * qadj8.size(), hipMemcpyHostToDevice, s));
   BMS2_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope
   return BMS2_OK;
@@ -1557,7 +1581,8 @@ void free_query_side(bms2_pipeline_s* p) {
   p->d_copy_pairs = nullptr;
   p->copy = ClassCopy{};
   p->num_dups = 0;
-  p->d_qpacks = p->d_qhead = p->d_qpre = p->d_qrec = p->d_qpos = p->d_qadj = nullptr;
+  p->d_qpacks = p->d_qhead = p->d_qpre = p->d_qrec = p->d_qpos = nullptr;
+  p->d_qadj = nullptr;
   p->d_jorder = nullptr;
   p->d_qadj8 = nullptr;
   p->d_big_qlev = nullptr;
@@ -1621,15 +1646,14 @@ int bind_queries(bms2_pipeline_s* p, const bms2_device_csr* query) {
   hv.row_offsets = qro.data();
   hv.column_indices = qci.data();
   hv.node_labels = qlab.data();
-  // automorphism orbits of the small queries: a node automorphic to an earlier one has its
-  // domain in every data graph, so the screen tests one node per orbit
-  std::vector<uint8_t> qorb(std::max<uint64_t>(Qn, 1), 0);
+  // automorphism orbits: a node automorphic to an earlier one has its domain in every data
+  // graph, so the screen tests one node per orbit of a small query; the large-query join breaks
+  // the symmetry of one orbit (upload_order)
+  std::vector<uint8_t> orbit(std::max<uint64_t>(Qn, 1), 0), qorb(std::max<uint64_t>(Qn, 1), 0);
   for (uint32_t g = 0; g < Q; ++g) {
     const uint32_t b = qgo[g], n = qgo[g + 1] - qgo[g];
-    if (n <= uint32_t(kSmallNq))
-      query_orbits(&hv, g, qorb.data() + b);
-    else
-      for (uint32_t u = 0; u < n; ++u) qorb[b + u] = uint8_t(u);
+    query_orbits(&hv, g, orbit.data() + b);
+    for (uint32_t u = 0; u < n; ++u) qorb[b + u] = n <= uint32_t(kSmallNq) ? orbit[b + u] : uint8_t(u);
   }
   std::vector<uint32_t> qhead(size_t(Q) * 4, 0), qpos(std::max<uint64_t>(Qn, 1), 0);
   for (uint32_t g = 0; g < Q; ++g) {
@@ -1662,7 +1686,7 @@ int bind_queries(bms2_pipeline_s* p, const bms2_device_csr* query) {
       (rc = device_alloc((void**)&p->d_qrec, 4ull * kRecDwords * std::max<uint64_t>(Qn, 1))) ||
       (rc = device_alloc((void**)&p->d_qpos, 4ull * qpos.size())) ||
       (rc = device_alloc((void**)&p->d_jorder, std::max<size_t>(size_t(Q) * 32, 32))) ||
-      (rc = device_alloc((void**)&p->d_qadj, 4ull * std::max<size_t>(size_t(Q) * 32, 32))) ||
+      (rc = device_alloc((void**)&p->d_qadj, 8ull * std::max<size_t>(size_t(Q) * 32, 32))) ||
       (rc = device_alloc((void**)&p->d_qadj8, 8ull * std::max<uint32_t>(Q, 1))) ||
       (rc = device_alloc((void**)&p->d_qorb, qorb.size())))
     return rc;
@@ -1728,6 +1752,16 @@ int bind_queries(bms2_pipeline_s* p, const bms2_device_csr* query) {
   p->qgo = std::move(qgo);
   p->qro = std::move(qro);
   p->qci = std::move(qci);
+  p->qlab = std::move(qlab);
+  p->orbit = std::move(orbit);
+  p->qhv = bms2_host_csr{};
+  p->qhv.num_graphs = Q;
+  p->qhv.total_nodes = Qn;
+  p->qhv.total_edges = p->qci.size();
+  p->qhv.graph_offsets = p->qgo.data();
+  p->qhv.row_offsets = p->qro.data();
+  p->qhv.column_indices = p->qci.data();
+  p->qhv.node_labels = p->qlab.data();
   if ((rc = upload_order(p, nullptr))) return rc;  // synchronises: the host vectors above are done
   // pair records of both classes and the small-record segments: worst case or a share of the
   // budget, whichever is smaller (the chunks are sized to what a run measures); buffers large
@@ -1920,6 +1954,10 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
   // a run screens one query per isomorphism class; traces screen every query (their per-query
   // accumulators are the planner's input)
   const bool dedup = p->dedup && p->num_dups && !tr.domsum && !tr.qnodes && !tr.pair_nodes;
+  // and the large-query join breaks the symmetry of automorphic query nodes (traces count every
+  // DFS node of the plain search)
+  // (count-all mode: in first-match mode the first match can come later under the constraint)
+  jp.sym = p->dedup && !p->opt.find_first && !tr.domsum && !tr.qnodes && !tr.pair_nodes ? 1u : 0u;
   sp.hist_map = p->hist_map;
   const size_t stage_bytes = std::max<size_t>(
       (size_t(kStageSmall) * kSmallWords + size_t(kStageLarge) * p->rec_words_l) * 8,
@@ -2095,6 +2133,12 @@ extern "C" int bms2_pipeline_run(bms2_pipeline p, uint64_t* d_counts, uint64_t* 
 
 extern "C" int bms2_pipeline_set_dedup(bms2_pipeline p, int on, uint32_t* num_dups) {
   if (!p) return set_error(BMS2_ERR_INVALID, "pipeline is null");
+  if (p->dedup != (on != 0)) {
+    // the captured joins hold the symmetry-breaking switch: capture them again
+    if (int rc = use_device(p->q)) return rc;
+    BMS2_HIP(hipStreamSynchronize(p->q->stream));
+    drop_graphs(p);
+  }
   p->dedup = on != 0;
   if (num_dups) *num_dups = p->num_dups;
   return BMS2_OK;

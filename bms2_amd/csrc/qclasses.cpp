@@ -225,7 +225,64 @@ void query_orbits(const bms2_host_csr* q, uint32_t g, uint8_t* orbit) {
   }
 }
 
+// Symmetry breaking for query g under the matching order `order` (order[d] = local node matched
+// at depth d): among the automorphism orbits of two or more nodes, the one whose second-matched
+// member comes earliest; du = the depth of its first-matched member, vmask = the depths of the
+// others.  Every induced embedding f of g (domains of automorphic nodes are equal) has exactly
+// one image f o a under the automorphisms a whose value at du is the orbit's minimum (the group
+// maps u_du onto each orbit member equally often), so the embeddings with f(d) > f(du) for every
+// d in vmask number exactly 1 / (|orbit|) of all: the join counts those and multiplies by
+// popcount(vmask) + 1.  du = 0xff, vmask = 0 without such an orbit.
+void query_symmetry(const bms2_host_csr* q, uint32_t g, const uint8_t* order, const uint8_t* orbit,
+                    uint8_t* du, uint32_t* vmask) {
+  const uint32_t n = q->graph_offsets[g + 1] - q->graph_offsets[g];
+  *du = 0xff;
+  *vmask = 0;
+  if (n < 2 || n > 32) return;
+  uint32_t depth_of[32];
+  for (uint32_t d = 0; d < n; ++d) depth_of[order ? order[d] : d] = d;
+  uint32_t best_second = ~0u;
+  for (uint32_t r = 0; r < n; ++r) {
+    if (orbit[r] != r) continue;  // r is the lowest node of its orbit
+    uint32_t depths = 0;
+    for (uint32_t u = 0; u < n; ++u)
+      if (orbit[u] == r) depths |= 1u << depth_of[u];
+    if (__builtin_popcount(depths) < 2) continue;
+    const uint32_t first = __builtin_ctz(depths), rest = depths & (depths - 1);
+    const uint32_t second = __builtin_ctz(rest);
+    if (second < best_second) {
+      best_second = second;
+      *du = uint8_t(first);
+      *vmask = rest;
+    }
+  }
+}
+
 }  // namespace bms2
+
+extern "C" int bms2_query_symmetry(const bms2_host_csr* q, const uint8_t* order, uint8_t* du,
+                                   uint32_t* vmask) {
+  if (!q || !du || !vmask ||
+      (q->num_graphs && (!q->graph_offsets || !q->row_offsets || !q->node_labels)))
+    return bms2::set_error(BMS2_ERR_INVALID, "null query batch or output");
+  std::vector<uint8_t> orbit(std::max<uint64_t>(q->total_nodes, 1));
+  for (uint32_t g = 0; g < q->num_graphs; ++g) {
+    const uint32_t b = q->graph_offsets[g], n = q->graph_offsets[g + 1] - b;
+    if (order) {
+      uint32_t seen = 0;
+      for (uint32_t d = 0; d < n && n <= 32; ++d) {
+        const uint32_t u = order[size_t(g) * 32 + d];
+        if (u >= n || ((seen >> u) & 1))
+          return bms2::set_error(BMS2_ERR_INVALID, "order of query %u is not a permutation", g);
+        seen |= 1u << u;
+      }
+    }
+    bms2::query_orbits(q, g, orbit.data() + b);
+    bms2::query_symmetry(q, g, order ? order + size_t(g) * 32 : nullptr, orbit.data() + b, du + g,
+                         vmask + g);
+  }
+  return BMS2_OK;
+}
 
 extern "C" int bms2_query_classes(const bms2_host_csr* q, uint32_t* rep, uint32_t* num_dups) {
   if (!q || !rep || (q->num_graphs && (!q->graph_offsets || !q->row_offsets || !q->node_labels)))

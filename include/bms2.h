@@ -412,8 +412,9 @@ int bms2_pipeline_set_order(bms2_pipeline p, const uint8_t* order);
  * the lowest-indexed query of each isomorphism class (bms2_query_classes, computed when the batch
  * is bound) and, after the joins, copies its count and GMCR bit to the class's other queries in
  * every data graph (inside the timed kernel span).  d_counts, d_valid and stats[0..2] are those of
- * screening every query; stats[3] counts the DFS nodes actually expanded.  on = 0 screens every
- * query (traces always do).  *num_dups (nullable) = queries that copy their class's results.  No
+ * screening every query; stats[3] counts the DFS nodes actually expanded (also reduced by the
+ * large-query join's symmetry breaking, bms2_query_symmetry).  on = 0 screens every query and
+ * searches every embedding (traces always do).  *num_dups (nullable) = queries that copy their class's results.  No
  * counterpart in the reference, which runs every query of a batch. */
 int bms2_pipeline_set_dedup(bms2_pipeline p, int on, uint32_t* num_dups);
 
@@ -429,6 +430,17 @@ int bms2_query_classes(const bms2_host_csr* q, uint32_t* rep, uint32_t* num_dups
  * Automorphic nodes have equal filter and refinement fields, hence equal domains: the fused
  * screen tests one node per orbit of a query of up to 8 nodes and copies its domain to the rest. */
 int bms2_query_orbits(const bms2_host_csr* q, uint8_t* orbit);
+
+/* Host: the symmetry breaking the fused pipeline's count-all join of queries above 8 nodes applies
+ * (with class copies on, bms2_pipeline_set_dedup) under a matching order (order[q*32 + d] = local node
+ * at depth d; NULL = identity).  Per query, of its automorphism orbits of two or more nodes the
+ * one whose second-matched member is earliest: du[q] = the depth of its first-matched member,
+ * vmask[q] = the depths of the others (du[q] = 0xff, vmask[q] = 0: none).  The join takes at a
+ * depth of vmask only candidates above the data node matched at depth du and multiplies the
+ * count by popcount(vmask) + 1: automorphic nodes have equal domains, so the automorphisms permute
+ * a pair's embeddings freely and exactly one in every |orbit| puts the orbit's smallest data node
+ * at depth du.  No counterpart in the reference. */
+int bms2_query_symmetry(const bms2_host_csr* q, const uint8_t* order, uint8_t* du, uint32_t* vmask);
 
 /* Data packs are planned in windows of BMS2_PACK_WINDOW consecutive data graphs. */
 #define BMS2_PACK_WINDOW 1024u

@@ -59,6 +59,8 @@ def lib() -> C.CDLL:
         h.orc_gmcr.argtypes = [P, P, vp, C.c_uint64, vp, vp, C.c_uint64]
         h.orc_join.restype = C.c_uint64
         h.orc_join.argtypes = [P, P, vp, C.c_uint64, vp, vp, C.c_int, vp, vp, vp]
+        h.orc_join_sym.restype = C.c_uint64
+        h.orc_join_sym.argtypes = [P, P, vp, C.c_uint64, vp, vp, C.c_int, vp, vp, vp, vp, vp]
         h.orc_join_partial.restype = C.c_uint64
         h.orc_join_partial.argtypes = [P, P, vp, C.c_uint64, C.c_uint32, vp, C.c_uint64]
         h.orc_join_with_partial.restype = C.c_uint64
@@ -163,16 +165,24 @@ def gmcr(q, d, cand) -> tuple[np.ndarray, np.ndarray]:
     return offsets, indices[:total]
 
 
-def join(q, d, cand, offsets, indices, find_first=False, order=None):
-    """(total, per-pair counts, per-pair DFS nodes)."""
+def join(q, d, cand, offsets, indices, find_first=False, order=None, sym=None):
+    """(total, per-pair counts, per-pair DFS nodes).  sym = (du u8[Q], vmask u32[Q]): the
+    symmetry-breaking constraint per query (counts are then the constrained counts)."""
     cq, cd = csr(q), csr(d)
     T = len(indices)
     counts = np.zeros(max(T, 1), np.uint64)
     nodes = np.zeros(max(T, 1), np.uint64)
     idx = np.ascontiguousarray(indices, np.uint32)
     ordp = None if order is None else np.ascontiguousarray(order, np.uint32)
-    total = lib().orc_join(C.byref(cq), C.byref(cd), _ptr(cand), cand.shape[1], _ptr(offsets),
-                           _ptr(idx), int(find_first), _ptr(ordp), _ptr(counts), _ptr(nodes))
+    if sym is None:
+        total = lib().orc_join(C.byref(cq), C.byref(cd), _ptr(cand), cand.shape[1], _ptr(offsets),
+                               _ptr(idx), int(find_first), _ptr(ordp), _ptr(counts), _ptr(nodes))
+    else:
+        du = np.ascontiguousarray(sym[0], np.uint8)
+        vm = np.ascontiguousarray(sym[1], np.uint32)
+        total = lib().orc_join_sym(C.byref(cq), C.byref(cd), _ptr(cand), cand.shape[1],
+                                   _ptr(offsets), _ptr(idx), int(find_first), _ptr(ordp),
+                                   _ptr(du), _ptr(vm), _ptr(counts), _ptr(nodes))
     return int(total), counts[:T], nodes[:T]
 
 

@@ -324,9 +324,19 @@ uint64_t orc_gmcr(const orc_csr* q, const orc_csr* d, const uint64_t* cand, uint
 uint64_t orc_join(const orc_csr* q, const orc_csr* d, const uint64_t* cand, uint64_t W,
                   const uint32_t* offsets, const uint32_t* indices, int find_first,
                   const uint32_t* order, uint64_t* counts, uint64_t* dfs_nodes) {
+  return orc_join_sym(q, d, cand, W, offsets, indices, find_first, order, nullptr, nullptr, counts,
+                      dfs_nodes);
+}
+
+uint64_t orc_join_sym(const orc_csr* q, const orc_csr* d, const uint64_t* cand, uint64_t W,
+                      const uint32_t* offsets, const uint32_t* indices, int find_first,
+                      const uint32_t* order, const uint8_t* sym_du, const uint32_t* sym_vmask,
+                      uint64_t* counts, uint64_t* dfs_nodes) {
   // JoinCandidates3Optimized (isomorphism.hpp:309-468) with standard backtracking: candidates of
   // each depth in ascending data node id (getCandidateAt, candidates.hpp:128-158), injectivity by
-  // a visited set, induced adjacency test of isValidMapping_fast_bitwise (:268-288).
+  // a visited set, induced adjacency test of isValidMapping_fast_bitwise (:268-288).  With
+  // sym_du / sym_vmask (per query; not a reference behaviour) a candidate at a depth of
+  // sym_vmask must exceed the data node matched at depth sym_du, and the count is NOT multiplied.
   uint64_t total = 0;
   for (uint32_t g = 0; g < d->num_graphs; ++g) {
     GraphRange r = graph_range(d, g);
@@ -364,6 +374,8 @@ uint64_t orc_join(const orc_csr* q, const orc_csr* d, const uint64_t* cand, uint
         if (visited[c - r.begin]) continue;
         bool ok = true;
         for (int i = 0; i < depth && ok; ++i) ok = (qadj[i][depth] != 0) == is_neighbor(d, mapping[i], c);
+        if (ok && sym_du && sym_du[qi] != 0xff && ((sym_vmask[qi] >> depth) & 1))
+          ok = c > mapping[sym_du[qi]];
         if (!ok) continue;
         if (depth == int(nq) - 1) {
           ++count;

@@ -70,6 +70,13 @@ uint64_t orc_gmcr(const orc_csr* q, const orc_csr* d, const uint64_t* cand, uint
 uint64_t orc_join(const orc_csr* q, const orc_csr* d, const uint64_t* cand, uint64_t W,
                   const uint32_t* offsets, const uint32_t* indices, int find_first,
                   const uint32_t* order, uint64_t* counts, uint64_t* dfs_nodes);
+/* orc_join with a symmetry-breaking constraint per query (not a reference behaviour: the checker of
+ * the fused pipeline's large-query join): at a depth in sym_vmask[qi] a candidate must exceed the
+ * data node matched at depth sym_du[qi] (0xff: none).  counts are the constrained counts. */
+uint64_t orc_join_sym(const orc_csr* q, const orc_csr* d, const uint64_t* cand, uint64_t W,
+                      const uint32_t* offsets, const uint32_t* indices, int find_first,
+                      const uint32_t* order, const uint8_t* sym_du, const uint32_t* sym_vmask,
+                      uint64_t* counts, uint64_t* dfs_nodes);
 /* joinPartialCandidates2 (isomorphism.hpp:1021-1178), standard backtracking: every embedding of
  * query 0's first `end` nodes in every data graph, as records of end + 1 ints {mapped global data
  * node ids, data graph id} in data graph order then DFS order; writes at most cap records (out

@@ -29,9 +29,11 @@ def molecules_1m_shard():
 
 def expected_stats(queries, data, K, find_first, rs, dedup=True):
     """The statistics bms2_pipeline_run reports next to the oracle's `rs` over every query: GMCR
-    pairs, matched pairs and embeddings are those of every query; DFS nodes are those of the
-    queries the run screens, one per isomorphism class when dedup is on (the others copy their
-    class's results), i.e. the oracle's DFS nodes over the classes' first queries."""
+    pairs, matched pairs and embeddings are those of every query; with dedup on, DFS nodes are
+    those of the queries the run screens, one per isomorphism class (the others copy their
+    class's results), with the symmetry breaking of bms2_query_symmetry in the count-all searches
+    of queries above 8 nodes on data graphs of up to 64 nodes: the oracle's staged join over the
+    classes' first queries under those constraints."""
     import numpy as np
 
     import oracle
@@ -40,9 +42,22 @@ def expected_stats(queries, data, K, find_first, rs, dedup=True):
         return rs
     rep = api.query_classes(queries)
     keep = np.flatnonzero(rep == np.arange(len(rep)))
-    if len(keep) == queries.num_graphs:
+    sub = queries if len(keep) == queries.num_graphs else graphs.GraphBatch.from_graphs(
+        [queries.graph(int(i)) for i in keep])
+    du, vm = api.query_symmetry(sub)
+    small = np.diff(np.asarray(sub.graph_offsets, np.int64)) <= 8
+    du[small], vm[small] = 0xff, 0
+    if find_first:  # the symmetry breaking serves count-all mode only
+        du[:], vm[:] = 0xff, 0
+    if len(keep) == queries.num_graphs and not vm.any():
         return rs
-    sub = graphs.GraphBatch.from_graphs([queries.graph(int(i)) for i in keep])
+    r = oracle.staged(sub, data, K, find_first, keep_domains=False)
+    _, _, nodes = oracle.join(sub, data, r.final_domains, r.gmcr_offsets, r.gmcr_indices,
+                              find_first, None, sym=(du, vm))
+    # graphs above 64 nodes are joined by the big-graph path, without the symmetry breaking
+    sizes = np.diff(np.asarray(data.graph_offsets, np.int64))
+    pair_g = np.repeat(np.arange(data.num_graphs), np.diff(r.gmcr_offsets.astype(np.int64)))
+    big = sizes[pair_g] > 64
     out = np.array(rs, copy=True)
-    out[3] = oracle.pipeline(sub, data, K, find_first)[2][3]
+    out[3] = int(nodes[~big].sum()) + int(r.dfs_nodes[big].sum())
     return out

@@ -74,3 +74,25 @@ def test_dedup_renumbered_copies_uneven(queue, K):
         assert ndup >= (queries.num_graphs + 4) // 5
         assert np.array_equal(c, rc) and np.array_equal(v, rv)
         assert np.array_equal(s[:4], expected_stats(batch, data, K, find_first, rs))
+
+
+def test_dedup_switch_on_one_pipeline(queue, molecules_1m_shard):
+    """Turning the class copies and the symmetry breaking off and on again on one pipeline (whose
+    runs replay captured graphs) gives the oracle's results every time and the matching DFS-node
+    statistics: every query searched in full when off, the constrained searches of the classes'
+    first queries when on."""
+    data, queries = molecules_1m_shard
+    rc, rv, rs = oracle.pipeline(queries, data, 6, False)
+    on = expected_stats(queries, data, 6, False, rs)
+    assert on[3] < rs[3]
+    pipe = api.Pipeline(queue, api.DeviceBatchedCSRGraph(queue, queries),
+                        api.DeviceBatchedCSRGraph(queue, data), refinement_steps=6)
+    counts, valid, stats = pipe.alloc_outputs()
+    for dedup in (True, False, True, False):
+        pipe.set_dedup(dedup)
+        for _ in range(2):  # the second run replays the captured chunk
+            pipe.run(counts, valid, stats)
+            queue.wait()
+            c, v = pipe.results(counts, valid)
+            assert np.array_equal(c, rc) and np.array_equal(v, rv)
+            assert np.array_equal(stats.to_numpy(np.uint64, 8)[:4], on if dedup else rs), dedup

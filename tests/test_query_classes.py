@@ -128,3 +128,73 @@ def test_orbits_of_symmetric_shapes():
     chain = ([O, C, C, O], [(0, 1), (1, 2), (2, 3)])
     batch = GraphBatch.from_graphs([star, ring, chain])
     assert api.query_orbits(batch).tolist() == [0, 1, 1, 1, 4] + [0] * 6 + [0, 1, 1, 0]
+
+
+def _orders(batch: GraphBatch, rng):
+    """A random connected matching order per query (u8 [Q, 32]): random scores through the
+    planner's connected ordering (an order that matches a node before any of its neighbours
+    leaves that depth unconstrained and can grow the search combinatorially)."""
+    from bms2_amd import learned
+    o = np.zeros((batch.num_graphs, 32), np.uint8)
+    for g, qi in enumerate(learned.query_infos(batch)):
+        o[g, :qi.n] = learned.score_order(qi, rng.normal(size=qi.n))
+    return o
+
+
+@pytest.mark.parametrize("workload", ["molecules-1m", "uneven-1m"])
+@pytest.mark.parametrize("ordered", [False, True])
+def test_symmetry_constraint_is_an_orbit(workload, ordered):
+    """du and vmask are the depths of one automorphism orbit of at least two nodes (du the first
+    matched), the one whose second-matched member comes earliest; queries without a non-trivial
+    orbit get none."""
+    _, queries = graphs.generate(workload, seed=4, num_data_graphs=100)
+    order = _orders(queries, np.random.default_rng(3)) if ordered else None
+    du, vm = api.query_symmetry(queries, order)
+    orb = api.query_orbits(queries)
+    go = queries.graph_offsets.astype(np.int64)
+    have = 0
+    for g in range(queries.num_graphs):
+        n = int(go[g + 1] - go[g])
+        o = order[g, :n] if ordered else np.arange(n)
+        depth = np.empty(n, np.int64)
+        depth[o] = np.arange(n)
+        local = orb[go[g]:go[g + 1]]
+        orbits = [np.flatnonzero(local == r) for r in np.unique(local)]
+        big = [sorted(depth[m]) for m in orbits if len(m) > 1]
+        if not big:
+            assert du[g] == 0xff and vm[g] == 0
+            continue
+        have += 1
+        best = min(big, key=lambda ds: ds[1])
+        assert du[g] == best[0]
+        assert int(vm[g]) == sum(1 << int(d) for d in best[1:])
+    assert have > 100
+
+
+@pytest.mark.parametrize("workload", ["molecules-1m", "uneven-1m"])
+@pytest.mark.parametrize("find_first", [False, True])
+def test_symmetry_breaking_counts(workload, find_first):
+    """Under the constraint the oracle's join finds exactly count / (popcount(vmask) + 1) of every
+    pair's embeddings and never more DFS nodes (existence unchanged in first-match mode); in
+    identity order and in random connected orders."""
+    import oracle
+    data, queries = graphs.generate(workload, seed=2, num_data_graphs=150)
+    r = oracle.staged(queries, data, 6, find_first, keep_domains=False)
+    go = queries.graph_offsets.astype(np.int64)
+    for order in (None, _orders(queries, np.random.default_rng(9))):
+        flat = None
+        if order is not None:
+            flat = np.concatenate([order[g, :int(go[g + 1] - go[g])]
+                                   for g in range(queries.num_graphs)]).astype(np.uint32)
+        du, vm = api.query_symmetry(queries, order)
+        _, c0, n0 = oracle.join(queries, data, r.final_domains, r.gmcr_offsets, r.gmcr_indices,
+                                find_first, flat)
+        _, c1, n1 = oracle.join(queries, data, r.final_domains, r.gmcr_offsets, r.gmcr_indices,
+                                find_first, flat, sym=(du, vm))
+        idx = r.gmcr_indices.astype(np.int64)
+        mult = np.array([bin(int(v)).count("1") + 1 for v in vm], np.uint64)[idx]
+        if find_first:  # existence is unchanged (the first match may come later or earlier)
+            assert np.array_equal(c1, c0)
+        else:  # every DFS node of the constrained search is one of the plain search's
+            assert np.array_equal(c1 * mult, c0)
+            assert np.all(n1 <= n0) and n1.sum() < n0.sum()
