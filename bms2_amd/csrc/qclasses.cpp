@@ -23,7 +23,13 @@
 
 namespace {
 
-constexpr uint64_t kSearchBudget = 1u << 20;  // backtracking steps per isomorphism test
+constexpr uint64_t kSearchBudget = 1u << 16;  // backtracking steps per isomorphism test
+// a query is compared with at most kBucketCompares classes of its colour-refinement bucket, and
+// the orbit searches of one query share kOrbitBudget steps: beyond them (inputs colour refinement
+// cannot tell apart, such as many non-isomorphic regular graphs of one label) queries and nodes
+// stay alone, which keeps every result exact
+constexpr uint32_t kBucketCompares = 32;
+constexpr uint64_t kOrbitBudget = 8 * kSearchBudget;
 
 uint64_t mix64(uint64_t x) {  // splitmix64 finaliser
   x += 0x9e3779b97f4a7c15ull;
@@ -66,8 +72,22 @@ QueryGraph load(const bms2_host_csr* q, uint32_t g) {
   // colour refinement: start from (label, degree), then n rounds of (own colour, multiset of
   // neighbour colours) hashed with two independent commutative sums
   for (uint32_t u = 0; u < G.n; ++u) G.label[u] = q->node_labels[b + u];
-  for (uint32_t u = 0; u < G.n; ++u)
-    G.colour[u] = mix64((uint64_t(q->node_labels[b + u]) << 8) | __builtin_popcount(G.adj[u]));
+  // initial colour: label, degree and the label counts at each BFS distance (an isomorphism
+  // invariant that splits most ties colour refinement leaves, e.g. between regular graphs)
+  for (uint32_t u = 0; u < G.n; ++u) {
+    uint64_t c = mix64((uint64_t(q->node_labels[b + u]) << 8) | __builtin_popcount(G.adj[u]));
+    uint32_t seen = 1u << u, front = seen;
+    for (uint32_t dist = 1; front; ++dist) {
+      uint32_t next = 0;
+      for (uint32_t f = front; f; f &= f - 1) next |= G.adj[__builtin_ctz(f)];
+      front = next & ~seen;
+      seen |= front;
+      uint64_t shell = 0;  // commutative over the shell's labels
+      for (uint32_t f = front; f; f &= f - 1) shell += mix64(0x100u + G.label[__builtin_ctz(f)]);
+      c = mix64(c ^ mix64(shell + dist));
+    }
+    G.colour[u] = c;
+  }
   uint64_t next[32];
   for (uint32_t round = 0; round < G.n; ++round) {
     for (uint32_t u = 0; u < G.n; ++u) {
@@ -94,7 +114,7 @@ QueryGraph load(const bms2_host_csr* q, uint32_t g) {
 // node fix_a of A to node fix_b of B.  On success map[u] is the image of A's node u.  False when
 // none exists or the budget runs out.
 bool find_isomorphism(const QueryGraph& A, const QueryGraph& B, int fix_a, int fix_b,
-                      uint32_t* map) {
+                      uint32_t* map, uint64_t* spent = nullptr) {
   const uint32_t n = A.n;
   if (n != B.n || A.m != B.m || A.key != B.key) return false;
   if (n == 0) return true;
@@ -130,6 +150,13 @@ bool find_isomorphism(const QueryGraph& A, const QueryGraph& B, int fix_a, int f
   uint32_t next_try[32];
   uint32_t used = 0;
   uint64_t steps = 0;
+  struct Tally {  // adds the steps taken to *spent on every exit
+    uint64_t* spent;
+    const uint64_t& steps;
+    ~Tally() {
+      if (spent) *spent += steps;
+    }
+  } tally{spent, steps};
   int i = 0;
   next_try[0] = fix_a >= 0 ? uint32_t(fix_b) : 0;
   while (i >= 0) {
@@ -183,9 +210,9 @@ uint32_t query_classes(const bms2_host_csr* q, uint32_t* rep) {
     rep[g] = g;
     if (!G[g].simple || G[g].n < 2) continue;
     std::vector<uint32_t>& reps = buckets[G[g].key];
-    for (uint32_t r : reps)
-      if (isomorphic(G[r], G[g])) {
-        rep[g] = r;
+    for (size_t i = 0; i < reps.size() && i < kBucketCompares; ++i)
+      if (isomorphic(G[reps[i]], G[g])) {
+        rep[g] = reps[i];
         ++dups;
         break;
       }
@@ -210,10 +237,12 @@ void query_orbits(const bms2_host_csr* q, uint32_t g, uint8_t* orbit) {
     return u;
   };
   uint32_t map[32];
+  uint64_t spent = 0;
   for (uint32_t u = 0; u < n; ++u)
     for (uint32_t v = u + 1; v < n; ++v) {
       if (G.colour[u] != G.colour[v] || G.label[u] != G.label[v] || root(u) == root(v)) continue;
-      if (!find_isomorphism(G, G, int(u), int(v), map)) continue;
+      if (spent >= kOrbitBudget) break;
+      if (!find_isomorphism(G, G, int(u), int(v), map, &spent)) continue;
       for (uint32_t w = 0; w < n; ++w) {  // every pair an automorphism relates shares an orbit
         const uint32_t a = root(w), b = root(map[w]);
         if (a != b) parent[std::max(a, b)] = std::min(a, b);

@@ -198,3 +198,29 @@ def test_symmetry_breaking_counts(workload, find_first):
         else:  # every DFS node of the constrained search is one of the plain search's
             assert np.array_equal(c1 * mult, c0)
             assert np.all(n1 <= n0) and n1.sum() < n0.sum()
+
+
+def test_regular_graphs_are_told_apart_quickly():
+    """Colour refinement cannot tell regular graphs of one label apart; the BFS label profile in
+    the initial colours and the search budgets keep such batches fast, and the classes still
+    equal networkx's (renumbered copies of some graphs join their originals)."""
+    import time
+    rng = np.random.default_rng(11)
+    gs = []
+    for i in range(60):
+        G = nx.random_regular_graph(3, 12, seed=i)
+        gs.append(([8] * 12, [tuple(e) for e in G.edges()]))
+    for i in range(0, 60, 6):
+        labels, edges = gs[i]
+        perm = rng.permutation(12)
+        gs.append(([8] * 12, [(int(perm[u]), int(perm[v])) for u, v in edges]))
+    for i in range(100):
+        G = nx.random_regular_graph(4, 32, seed=1000 + i)
+        gs.append(([8] * 32, [tuple(e) for e in G.edges()]))
+    batch = GraphBatch.from_graphs(gs)
+    t0 = time.perf_counter()
+    rep = api.query_classes(batch)
+    api.query_orbits(batch)
+    assert time.perf_counter() - t0 < 2.0
+    assert np.array_equal(rep[:70], _networkx_classes(GraphBatch.from_graphs(gs[:70])))
+    assert all(rep[60 + k] <= 6 * k for k in range(10))
