@@ -1215,8 +1215,8 @@ struct bms2_pipeline_s {
   uint32_t* d_qrep = nullptr;       // [Q] first query of each query's class
   uint8_t* d_qorb = nullptr;        // [Qn] lowest local node of each query node's automorphism
                                     // orbit (queries of up to kSmallNq nodes; itself otherwise)
-  uint32_t* d_copy_u32 = nullptr;   // ClassCopy's u32 tables: reps, chunk, words, wbegin
-  uint2* d_copy_pairs = nullptr;    // ClassCopy's dups, then wpairs
+  uint32_t* d_copy_u32 = nullptr;   // ClassCopy's u32 tables: reps, chunk, words, dpos
+  uint2* d_copy_pairs = nullptr;    // ClassCopy's dups
   ClassCopy copy{};
   uint64_t* d_dup_stats = nullptr;  // [4] statistics of the copied pairs
   // data graphs above 64 nodes: their own compact batch through the staged kernels
