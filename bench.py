@@ -432,7 +432,7 @@ def main() -> int:
     tune = None
     if not args.no_autotune:
         t0 = time.perf_counter()
-        tune = pipe.autotune(counts, valid, runs=1)
+        tune = pipe.autotune(counts, valid, runs=2)
         tune["seconds"] = round(time.perf_counter() - t0, 2)
         print(f"[bench] autotuned launch shape: {tune}", file=sys.stderr, flush=True)
     order = "identity"
@@ -472,6 +472,14 @@ def main() -> int:
                          queries_reordered=int(pl.learned.sum()))
         print(f"[bench] learned plan: {plan_info}", file=sys.stderr, flush=True)
         order = "learned"
+        if not args.no_autotune:
+            # the launch shape of the learned leg, tuned under its own order (each leg its best)
+            t0 = time.perf_counter()
+            tune_l = pipe.autotune(counts, valid, runs=2)
+            tune_l["seconds"] = round(time.perf_counter() - t0, 2)
+            tune = {"plain": tune, "learned": tune_l}
+            print(f"[bench] autotuned launch shape (learned order): {tune_l}", file=sys.stderr,
+                  flush=True)
     t_total, kernel_ms, st = leg(f"{order} order")
     if plan_info is not None and plain_sample is not None:
         lc, lv = pipe.results(counts, valid, rows=plain_sample[0].shape[0])
