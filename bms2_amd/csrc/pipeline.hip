@@ -691,6 +691,7 @@ struct JoinSmallParams {
   uint32_t find_first;
   uint64_t* qnodes;         // trace (nullable)
   uint32_t* pair_nodes;     // trace (nullable)
+  uint32_t sym;             // apply the symmetry breaking the records' headers carry
 };
 
 // Depth-first join of the small-query records (up to kSmallNq nodes), one (query, graph) pair
@@ -767,8 +768,8 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
     }
   };
   bool exhausted = false, busy = false, pf = false, has_res = false;
-  uint32_t slot = 0, pslot = 0, q = 0, g = 0, nadj1 = 0;
-  int depth = 0, last = 0;
+  uint32_t slot = 0, pslot = 0, q = 0, g = 0, nadj1 = 0, sym_v = 0, mult = 1;
+  int depth = 0, last = 0, sym_du = 0, sdepth = -1;
   uint64_t used = 0, cnt = 0, nodes = 0, qa = 0, cur = 0, xcur = 0, fpack = 0;
   uint32_t saved = 0;
   uint64_t pw[kSmallWords];  // the lane's reserved record, loaded one handout ahead
@@ -788,8 +789,21 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
         slot = pslot;
         q = uint32_t(pw[0]);
         g = uint32_t(pw[0] >> 32);
-        last = int(pw[1] & 0xffu) - 2;  // the depth whose candidates end a match
+        last = int(pw[1] & 0xfu) - 2;  // the depth whose candidates end a match
         qa = pw[1];  // byte d >= 1: earlier depths adjacent to depth d
+        // symmetry breaking (bits 4-7 of byte 0: on | first orbit depth du; bit 7 of byte d:
+        // depth d's candidates must exceed the node at du); sdepth = du when depth du + 1 is
+        // constrained (checked per step at du), else -1; deeper depths fold it into X
+        sym_v = 0;
+        sdepth = -1;
+        mult = 1;
+        if (p.sym && ((qa >> 7) & 1)) {
+#pragma unroll
+          for (int d = 1; d < NQ; ++d) sym_v |= uint32_t((qa >> (8 * d + 7)) & 1) << d;
+          sym_du = int((qa >> 4) & 7);
+          if ((sym_v >> (sym_du + 1)) & 1) sdepth = sym_du;
+          mult = uint32_t(__builtin_popcount(sym_v)) + 1;
+        }
         // bit d: depths d and d + 1 not adjacent (bit d of byte d + 1 clear)
         nadj1 = ~0u;
 #pragma unroll
@@ -888,7 +902,9 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
         const int nd = depth + 1;
         // adjacent ? N(v) : ~N(v) as N(v) ^ f with f all ones or zero
         const uint32_t f1 = uint32_t(__builtin_amdgcn_sbfe(int(nadj1), depth, 1));
-        const uint64_t Rn = xcur & (nv ^ ((uint64_t(f1) << 32) | f1)) & ~vbit;
+        // at the orbit's first depth, when the next depth is constrained: candidates above v only
+        const uint64_t drop = depth == sdepth ? (vbit << 1) - 1 : vbit;
+        const uint64_t Rn = xcur & (nv ^ ((uint64_t(f1) << 32) | f1)) & ~drop;
         if (depth == last) {
           cnt += uint64_t(popc64(Rn));
           if (p.find_first && cnt) finished = true;
@@ -910,13 +926,18 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
           }
           const uint64_t c = clo | (uint64_t(chi) << 32);
           used |= vbit;
-          const uint64_t xn = dn & c & ~used;
+          uint64_t xn = dn & c & ~used;
           if (cur) {
             RS[depth][lane] = cur;
             saved |= 1u << depth;
           }
           if (depth < NQ - 3) NF[depth][lane] = nv;
           fpack = (fpack & ~(uint64_t(63) << (6 * depth))) | (uint64_t(v) << (6 * depth));
+          if ((sym_v >> (nd + 1)) & 1 & uint32_t(nd + 1 >= sym_du + 2)) {
+            // depth nd + 1 constrained by the node at du (matched: du <= depth)
+            const uint32_t fu = uint32_t(fpack >> (6 * sym_du)) & 63u;
+            xn &= ~((uint64_t(2) << fu) - 1);
+          }
           if (nd < NQ - 2) X[nd][lane] = xn;  // backtracking returns to depths <= NQ - 3 only
           depth = nd;
           cur = Rn;
@@ -925,7 +946,7 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
       }
     }
     if (finished) {
-      const uint64_t c = p.find_first ? (cnt ? 1 : 0) : cnt;
+      const uint64_t c = p.find_first ? (cnt ? 1 : 0) : cnt * mult;
       res_at = uint64_t(g) * p.Q + q;  // a finished lane stays idle until the next handout
       res = c;
       has_res = true;
@@ -1534,16 +1555,19 @@ int upload_order(bms2_pipeline_s* p, const uint8_t* order) {
       qadj[size_t(g) * 32 + d] = w;
       if (n <= uint32_t(kSmallNq)) qadj8[g] |= uint64_t(w) << (8 * d);
     }
-    if (n > uint32_t(kSmallNq)) {
-      // symmetry breaking of the large-query join under this order
-      uint8_t du = 0xff;
-      uint32_t vmask = 0;
-      query_symmetry(&p->qhv, g, o, p->orbit.data() + b, &du, &vmask);
-      if (du != 0xff) {
-        for (uint32_t d = 0; d < n; ++d)
-          if ((vmask >> d) & 1) qadj[size_t(g) * 32 + d] |= (uint64_t(1) << 37) | (uint64_t(du) << 32);
-        qadj[size_t(g) * 32] |= uint64_t(__builtin_popcount(vmask) + 1) << 40;
-      }
+    // symmetry breaking of the joins under this order
+    uint8_t du = 0xff;
+    uint32_t vmask = 0;
+    query_symmetry(&p->qhv, g, o, p->orbit.data() + b, &du, &vmask);
+    if (du != 0xff && n > uint32_t(kSmallNq)) {
+      for (uint32_t d = 0; d < n; ++d)
+        if ((vmask >> d) & 1) qadj[size_t(g) * 32 + d] |= (uint64_t(1) << 37) | (uint64_t(du) << 32);
+      qadj[size_t(g) * 32] |= uint64_t(__builtin_popcount(vmask) + 1) << 40;
+    } else if (du != 0xff) {
+      // the small records' header: bits 4-7 of byte 0 (on | du), bit 7 of byte d (d in vmask)
+      qadj8[g] |= 0x80u | (uint64_t(du) << 4);
+      for (uint32_t d = 1; d < n; ++d)
+        if ((vmask >> d) & 1) qadj8[g] |= uint64_t(1) << (8 * d + 7);
     }
   }
   hipStream_t s = p->q->stream;
@@ -1958,6 +1982,7 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
   // DFS node of the plain search)
   // (count-all mode: in first-match mode the first match can come later under the constraint)
   jp.sym = p->dedup && !p->opt.find_first && !tr.domsum && !tr.qnodes && !tr.pair_nodes ? 1u : 0u;
+  js.sym = jp.sym;
   sp.hist_map = p->hist_map;
   const size_t stage_bytes = std::max<size_t>(
       (size_t(kStageSmall) * kSmallWords + size_t(kStageLarge) * p->rec_words_l) * 8,

@@ -431,8 +431,8 @@ int bms2_query_classes(const bms2_host_csr* q, uint32_t* rep, uint32_t* num_dups
  * screen tests one node per orbit of a query of up to 8 nodes and copies its domain to the rest. */
 int bms2_query_orbits(const bms2_host_csr* q, uint8_t* orbit);
 
-/* Host: the symmetry breaking the fused pipeline's count-all join of queries above 8 nodes applies
- * (with class copies on, bms2_pipeline_set_dedup) under a matching order (order[q*32 + d] = local node
+/* Host: the symmetry breaking the fused pipeline's count-all joins apply on data graphs of up to
+ * 64 nodes (with class copies on, bms2_pipeline_set_dedup) under a matching order (order[q*32 + d] = local node
  * at depth d; NULL = identity).  Per query, of its automorphism orbits of two or more nodes the
  * one whose second-matched member is earliest: du[q] = the depth of its first-matched member,
  * vmask[q] = the depths of the others (du[q] = 0xff, vmask[q] = 0: none).  The join takes at a

@@ -32,8 +32,8 @@ def expected_stats(queries, data, K, find_first, rs, dedup=True):
     pairs, matched pairs and embeddings are those of every query; with dedup on, DFS nodes are
     those of the queries the run screens, one per isomorphism class (the others copy their
     class's results), with the symmetry breaking of bms2_query_symmetry in the count-all searches
-    of queries above 8 nodes on data graphs of up to 64 nodes: the oracle's staged join over the
-    classes' first queries under those constraints."""
+    on data graphs of up to 64 nodes: the oracle's staged join over the classes' first queries
+    under those constraints."""
     import numpy as np
 
     import oracle
@@ -45,8 +45,6 @@ def expected_stats(queries, data, K, find_first, rs, dedup=True):
     sub = queries if len(keep) == queries.num_graphs else graphs.GraphBatch.from_graphs(
         [queries.graph(int(i)) for i in keep])
     du, vm = api.query_symmetry(sub)
-    small = np.diff(np.asarray(sub.graph_offsets, np.int64)) <= 8
-    du[small], vm[small] = 0xff, 0
     if find_first:  # the symmetry breaking serves count-all mode only
         du[:], vm[:] = 0xff, 0
     if len(keep) == queries.num_graphs and not vm.any():
