@@ -33,17 +33,21 @@ constexpr int kSegRecs = 32;     // small records per segment (a flush writes se
 constexpr int kRecDwords = 48;   // screening record of one query node
 constexpr int kPreDwords = 20;   // prefilter record of one query
 constexpr int kSumDwords = 16;   // prefilter summary of one data graph
+// the screen's tuple scratch at the start of its dynamic LDS: Bloom words (u64 units), adjacency
+// rows, labels
+constexpr int kScreenBloomWords = 64 * 8 * 4 / 8;
+constexpr int kScreenScratchBytes = kScreenBloomWords * 8 + 64 * 8 + 64;
 constexpr int kSmallNq = 8;      // queries of up to 8 nodes use the small records and join
 constexpr int kSmallWords = 2 + kSmallNq;
 
 // Screening record layout (u32 words): [0] lower end of the node's key range (label << 8 |
 // degree, or the degree alone for the wildcard), [1..8] path Bloom, [9..10] ring counts, [11]
-// degree | label << 8, [12 + HW k .. 11 + HW (k + 1)] shell level k < K (HW = 2 compact or 4
-// full words), then at A = 12 + HW K: [A] 255 - degree (the range's span), [A + 1] the data key
-// bits compared (0xffff, 0xff for the wildcard), [A + 2] the node's depth in the matching order.  Shell words hold the even / odd nibbles of the histogram, one count
-// per byte; the screen keeps the data node's counts as 15 - d per byte, so one 64-bit add over
-// eight labels gives q + 15 - d per byte (at most 30: no carry between bytes), with bit 4 set
-// exactly when q > d.
+// degree | label << 8, [12 .. 11 + S] the K shell levels' label counts (S = shell_dwords: 5-bit
+// fields, six to a word, over 8 compact or 16 labels per level), then at A = 12 + S: [A] 255 -
+// degree (the range's span), [A + 1] the data key bits compared (0xffff, 0xff for the wildcard),
+// [A + 2] the node's depth in the matching order.  The screen keeps the data node's counts as
+// 15 - d per field, so one 64-bit add over twelve fields gives q + 15 - d per field (at most 30:
+// no carry between fields), with the field's bit 4 set exactly when q > d.
 struct ScreenParams {
   const uint32_t* graph_offsets;
   const uint32_t* row_offsets;
@@ -115,7 +119,7 @@ struct QueryPrepParams {
   const uint32_t* rec_pos;  // [Qn] record slot of each query node
   uint32_t* qrec;
   uint32_t hist_map;        // 8 labels (4 bits each) of the compact shell histograms
-  uint32_t compact;         // 1: shell levels as 2 words over the hist_map labels, 0: 4 words
+  uint32_t compact;         // 1: shell levels over the 8 hist_map labels, 0: over all 16
   const uint8_t* jorder;    // [Q][32] local query node matched at each DFS depth
 };
 
@@ -130,6 +134,31 @@ __device__ inline uint32_t compact_hist(uint64_t h, uint32_t map) {
     out |= uint32_t((h >> (4 * lab)) & 15u) << (4 * i);
   }
   return out;
+}
+
+// Shell counts of K levels x L labels (4 bits each) as 5-bit fields, six to a u32 (bits 0-29):
+// field f = level f / L, label slot f % L.  The node test adds the query's field to the data
+// node's 15 - count; the sum is at most 30, so no carry leaves a field, and its bit 4 is set iff
+// the query's count is the larger.  The fields run to an even number of u32 words (the test adds
+// u64s); padding fields are 0 on both sides and pass.
+template <int K, int L>
+constexpr int shell_dwords() {
+  return 2 * ((K * L + 11) / 12);
+}
+constexpr uint32_t kShellFieldBits = 0x21084210u;  // bit 4 of each of the six 5-bit fields
+
+// nib[k]: level k's L nibbles (label slot i at bits 4i); invert: store 15 - count
+template <int K, int L>
+__device__ inline void pack_shells(const uint64_t (&nib)[K > 0 ? K : 1], bool invert,
+                                   uint32_t (&out)[shell_dwords<K, L>() > 0 ? shell_dwords<K, L>() : 1]) {
+#pragma unroll
+  for (int w = 0; w < shell_dwords<K, L>(); ++w) out[w] = 0;
+#pragma unroll
+  for (int f = 0; f < K * L; ++f) {
+    uint32_t v = uint32_t(nib[f / L] >> (4 * (f % L))) & 15u;
+    if (invert) v = 15u - v;
+    out[f / 6] |= v << (5 * (f % 6));
+  }
 }
 
 __device__ inline uint32_t cload(const uint32_t* p, uint32_t i) {
@@ -165,23 +194,24 @@ __global__ __launch_bounds__(64) void query_prep_kernel(QueryPrepParams p) {
   r[9] = uint32_t(t.cycle);
   r[10] = uint32_t(t.cycle >> 32);
   r[11] = qd | (lab << 8);
-  const uint32_t hw = p.compact ? 2u : 4u, aux = 12 + hw * K;
+  // shell levels: 8 labels (compact) or 16 per level in 5-bit fields
+  uint64_t nib[K > 0 ? K : 1];
+  nib[0] = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) nib[k] = p.compact ? compact_hist(t.hist[k], p.hist_map) : t.hist[k];
+  uint32_t aux;
   if (p.compact) {
+    uint32_t sw[shell_dwords<K, 8>() > 0 ? shell_dwords<K, 8>() : 1];
+    pack_shells<K, 8>(nib, false, sw);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint32_t c = compact_hist(t.hist[k], p.hist_map);
-      r[12 + 2 * k + 0] = c & 0x0f0f0f0fu;
-      r[12 + 2 * k + 1] = (c >> 4) & 0x0f0f0f0fu;
-    }
+    for (int w = 0; w < shell_dwords<K, 8>(); ++w) r[12 + w] = sw[w];
+    aux = 12 + shell_dwords<K, 8>();
   } else {
+    uint32_t sw[shell_dwords<K, 16>() > 0 ? shell_dwords<K, 16>() : 1];
+    pack_shells<K, 16>(nib, false, sw);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint32_t lo = uint32_t(t.hist[k]), hi = uint32_t(t.hist[k] >> 32);
-      r[12 + 4 * k + 0] = lo & 0x0f0f0f0fu;
-      r[12 + 4 * k + 1] = (lo >> 4) & 0x0f0f0f0fu;
-      r[12 + 4 * k + 2] = hi & 0x0f0f0f0fu;
-      r[12 + 4 * k + 3] = (hi >> 4) & 0x0f0f0f0fu;
-    }
+    for (int w = 0; w < shell_dwords<K, 16>(); ++w) r[12 + w] = sw[w];
+    aux = 12 + shell_dwords<K, 16>();
   }
   r[aux + 0] = 255u - qd;                 // span of the key range
   r[aux + 1] = wild ? 0xffu : 0xffffu;    // key bits compared: degree only for the wildcard
@@ -239,12 +269,18 @@ __global__ void query_pre_kernel(const uint32_t* __restrict__ qhead, const uint3
 
 // ------------------------------------------------------------------ screening
 
-template <int K, int HW>  // HW: u32 words per shell level (2 compact, 4 full)
-__global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
-  __shared__ uint64_t adj[64];
-  __shared__ uint8_t lbl[64];
+// HW: 2 for the compact shell histograms (8 labels per level), 4 for all 16 labels.  Five waves
+// per SIMD: the register cap (96 VGPRs) spills one 8-byte value outside the node loop; six (80)
+// spill inside the per-query path and ran 3.5 ms slower than five on 1M molecules.
+template <int K, int HW>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void screen_kernel(
+    ScreenParams p) {
   __shared__ __align__(16) uint32_t gsum_lds[kPackGraphs * kSumDwords];
-  extern __shared__ __align__(16) uint64_t stage[];  // staged records; Bloom scratch first
+  // staged pair records; before them the tuples' scratch: Bloom words, adjacency rows, labels
+  // (the static LDS stays at the 512-byte summaries, so five waves per SIMD fit the 160 KB)
+  extern __shared__ __align__(16) uint64_t stage[];
+  uint64_t* const adj = stage + kScreenBloomWords;
+  uint8_t* const lbl = reinterpret_cast<uint8_t*>(adj + 64);
   const uint32_t lane = lane_id();
   const uint32_t pack = p.pack_begin + blockIdx.x;
   const uint32_t gb = p.pack_gbegin[pack], ge = p.pack_gbegin[pack + 1];
@@ -286,25 +322,25 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
   // ring counts with the top bit of each u16 set: (d | 0x8000) - q has bit 15 set iff d >= q
   const uint32_t d_cyc0 = uint32_t(t.cycle) | 0x80008000u;
   const uint32_t d_cyc1 = uint32_t(t.cycle >> 32) | 0x80008000u;
-  // shell levels as 15 - d per byte, in the byte order of the query records' shell words
-  constexpr int HL = HW / 2;  // u64 words per shell level
-  constexpr uint64_t kFifteens = 0x0f0f0f0f0f0f0f0full;
-  uint64_t d_hist[K > 0 ? K : 1][HL];
+  // shell levels as 15 - count in the 5-bit fields of the query records' shell words
+  constexpr int SL = HW * 4;                  // labels per level
+  constexpr int SD = shell_dwords<K, SL>();   // shell u32 words of a record
+  constexpr int SQ = SD / 2;                  // u64 words the test adds
+  uint64_t d_sh[SQ > 0 ? SQ : 1];
+  {
+    uint64_t nib[K > 0 ? K : 1];
+    nib[0] = 0;
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    if (HW == 2) {
-      const uint32_t c = compact_hist(t.hist[k], p.hist_map);
-      d_hist[k][0] = kFifteens - ((c & 0x0f0f0f0fu) | (uint64_t((c >> 4) & 0x0f0f0f0fu) << 32));
-    } else {
-      const uint32_t lo = uint32_t(t.hist[k]), hi = uint32_t(t.hist[k] >> 32);
-      d_hist[k][0] = kFifteens - ((lo & 0x0f0f0f0fu) | (uint64_t((lo >> 4) & 0x0f0f0f0fu) << 32));
-      d_hist[k][HL - 1] =
-          kFifteens - ((hi & 0x0f0f0f0fu) | (uint64_t((hi >> 4) & 0x0f0f0f0fu) << 32));
+    for (int k = 0; k < K; ++k) nib[k] = HW == 2 ? compact_hist(t.hist[k], p.hist_map) : t.hist[k];
+    uint32_t sw[SD > 0 ? SD : 1];
+    pack_shells<K, SL>(nib, true, sw);
+#pragma unroll
+    for (int j = 0; j < SQ; ++j) {
+      d_sh[j] = (uint64_t(sw[2 * j + 1]) << 32) | sw[2 * j];
+      // opaque from here on: the test adds the query word to this value as it stands (left
+      // visible, the compiler rebuilds q + 15 - d from d with a borrow chain, twice the work)
+      asm volatile("" : "+v"(d_sh[j]));
     }
-    // opaque from here on: the test adds the query word to this value as it stands (left
-    // visible, the compiler rebuilds q + 15 - d from d with a borrow chain, twice the work)
-#pragma unroll
-    for (int j = 0; j < HL; ++j) asm volatile("" : "+v"(d_hist[k][j]));
   }
 
   const uint64_t T = ballot(lane < nn && lane == mytop);  // top lane of every non-empty graph
@@ -386,7 +422,7 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
   __syncthreads();
   const uint32_t myslotbit = lane < nn ? 1u << myslot : 0u;  // this lane's graph in a gpass mask
   // the node test's masks held in VGPRs (as scalar constants they are rebuilt per test)
-  uint32_t k_sh_bits = 0x10101010u, k_ring_bits = 0x80008000u;
+  uint32_t k_sh_bits = kShellFieldBits, k_ring_bits = 0x80008000u;
   asm volatile("" : "+v"(k_sh_bits), "+v"(k_ring_bits));
 
   for (uint32_t qb = 0; qb < p.Q; qb += 64) {
@@ -440,7 +476,7 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
       const uint32_t* rend = r + size_t(qmap >> 24) * kRecDwords;
       do {
         // the whole record in one batch of scalar loads, then a branch-free test
-        constexpr int A = 12 + HW * K;  // key span, key mask, domain offset
+        constexpr int A = 12 + SD;  // key span, key mask, domain offset
         uint32_t w[A + 3];
 #pragma unroll
         for (int x = 0; x < A + 3; ++x) w[x] = cload(r, x);
@@ -461,11 +497,8 @@ __global__ __launch_bounds__(64) void screen_kernel(ScreenParams p) {
           // every shell level at once (an empty query level adds 0 and passes every data node)
           uint64_t sh = 0;
 #pragma unroll
-          for (int k = 0; k < K; ++k)
-#pragma unroll
-            for (int j = 0; j < HL; ++j)
-              sh |= (uint64_t(w[12 + HW * k + 2 * j]) | (uint64_t(w[13 + HW * k + 2 * j]) << 32)) +
-                    d_hist[k][j];
+          for (int j = 0; j < SQ; ++j)
+            sh |= (uint64_t(w[12 + 2 * j]) | (uint64_t(w[13 + 2 * j]) << 32)) + d_sh[j];
           fail |= uint32_t(sh | (sh >> 32)) & k_sh_bits;
           m &= ballot((acc | fail) == 0);
         }
@@ -1986,7 +2019,7 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
   sp.hist_map = p->hist_map;
   const size_t stage_bytes = std::max<size_t>(
       (size_t(kStageSmall) * kSmallWords + size_t(kStageLarge) * p->rec_words_l) * 8,
-      64 * 8 * sizeof(uint32_t));
+      size_t(kScreenScratchBytes));
   return dispatch_levels(p->opt.refinement_steps, [&](auto kc) -> int {
     constexpr int K = decltype(kc)::value;
     hipLaunchKernelGGL(query_prep_kernel<K>, dim3(p->num_qpacks), dim3(64), 0, s, qp);
