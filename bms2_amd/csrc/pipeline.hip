@@ -27,9 +27,9 @@
 namespace bms2 {
 
 constexpr int kPackGraphs = 8;   // graphs per pack
-constexpr int kStageSmall = 64;  // LDS-staged small pair records per wave
+constexpr int kStageSmall = 48;  // LDS-staged small pair records per wave
 constexpr int kStageLarge = 16;  // LDS-staged large pair records per wave
-constexpr int kSegRecs = 32;     // small records per segment (a flush writes several)
+constexpr int kSegRecs = 64;     // most small records per segment (a flush writes one)
 constexpr int kRecDwords = 48;   // screening record of one query node
 constexpr int kPreDwords = 20;   // prefilter record of one query
 constexpr int kSumDwords = 16;   // prefilter summary of one data graph
@@ -269,15 +269,16 @@ __global__ void query_pre_kernel(const uint32_t* __restrict__ qhead, const uint3
 
 // ------------------------------------------------------------------ screening
 
-// HW: 2 for the compact shell histograms (8 labels per level), 4 for all 16 labels.  Five waves
-// per SIMD: the register cap (96 VGPRs) spills one 8-byte value outside the node loop; six (80)
-// spill inside the per-query path and ran 3.5 ms slower than five on 1M molecules.
+// HW: 2 for the compact shell histograms (8 labels per level), 4 for all 16 labels.  Six waves
+// per SIMD: the register cap (80 VGPRs) spills two 8-byte addresses used once per 64-query block
+// and once per flush, and 48 staged small records keep the LDS at 6.5 KB per wave when the large
+// records have at most 16 nodes (six waves fit the 160 KB).
 template <int K, int HW>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void screen_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void screen_kernel(
     ScreenParams p) {
   __shared__ __align__(16) uint32_t gsum_lds[kPackGraphs * kSumDwords];
   // staged pair records; before them the tuples' scratch: Bloom words, adjacency rows, labels
-  // (the static LDS stays at the 512-byte summaries, so five waves per SIMD fit the 160 KB)
+  // (the static LDS stays at the 512-byte summaries)
   extern __shared__ __align__(16) uint64_t stage[];
   uint64_t* const adj = stage + kScreenBloomWords;
   uint8_t* const lbl = reinterpret_cast<uint8_t*>(adj + 64);
@@ -438,6 +439,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
       q_nq = e.z;
       q_map = e.w;
       if (e.x) {
+        // one graph at a time: unrolled, the two graphs' summaries set the kernel's register peak
+#pragma unroll 1
         for (uint32_t j = 0; j < ng; ++j) {
           const uint4* gs = reinterpret_cast<const uint4*>(gsum + j * kSumDwords);
           const uint4 ga = gs[0], gb4 = gs[1], gc = gs[2], gr = gs[3];
@@ -750,18 +753,18 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
   // backtracking jumps straight to the deepest saved depth; the node matched at depth d is
   // 6-bit field d of `fpack`, its neighbour row NF[d] (read by deeper descents only).
   constexpr int NQ = kSmallNq;
-  constexpr uint32_t kRing = 160;
+  constexpr uint32_t kRing = 128 + kSegRecs;
   constexpr uint32_t kBatch = 24;  // idle lanes that trigger a handout (16: +0.6%, 32: +4%)
   constexpr uint32_t kSegClaim = 8;  // segments claimed per atomic on the global cursor
   // a refill pushes a segment only while fewer than 128 entries are queued
   static_assert(128 + kSegRecs <= int(kRing), "ring must hold what a refill may queue");
-  static_assert(S <= 16 && kSegRecs <= 32, "slot and offset must fit the ring entry");
+  static_assert(S <= 16 && kSegRecs <= 64, "slot and offset must fit the ring entry");
   __shared__ uint64_t RS[NQ - 2][64];  // saved remaining candidates per depth
   __shared__ uint64_t X[NQ - 2][64];   // candidate base per depth (above)
   __shared__ uint64_t D[NQ - 2][64];   // domain of depth d + 2 (matching order)
   __shared__ uint64_t NF[NQ - 3][64];  // neighbour row of the node matched per depth
   __shared__ uint64_t ADJ[S][64];      // adjacency rows of each slot's pack
-  __shared__ uint16_t RING[kRing];     // queued record: offset in its slot's segment | slot << 5
+  __shared__ uint16_t RING[kRing];     // queued record: offset in its slot's segment | slot << 6
   __shared__ uint32_t FIRST[S];        // first record of each slot's segment
   __shared__ uint32_t USE[S];
   const uint32_t lane = lane_id();
@@ -867,7 +870,7 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
             } else {
               ADJ[s][lane] = st_adj;
               for (uint32_t e = lane; e < st_count; e += 64)
-                RING[(qt + e) % kRing] = uint16_t(e | (uint32_t(s) << 5));
+                RING[(qt + e) % kRing] = uint16_t(e | (uint32_t(s) << 6));
               if (lane == 0) {
                 USE[s] = st_count;
                 FIRST[s] = st_first;
@@ -891,8 +894,8 @@ __global__ __launch_bounds__(64) void join_small_kernel(JoinSmallParams p) {
         // issue the record's loads; nothing waits for them before the next handout
         pf = true;
         const uint32_t ent = RING[(qh + rank) % kRing];
-        pslot = ent >> 5;
-        const uint64_t* r = p.recs + uint64_t(FIRST[pslot] + (ent & 31u)) * kSmallWords;
+        pslot = ent >> 6;
+        const uint64_t* r = p.recs + uint64_t(FIRST[pslot] + (ent & 63u)) * kSmallWords;
 #pragma unroll
         for (int x = 0; x < kSmallWords; ++x) pw[x] = r[x];
       }
