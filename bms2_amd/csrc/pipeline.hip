@@ -27,8 +27,10 @@
 namespace bms2 {
 
 constexpr int kPackGraphs = 8;   // graphs per pack
-constexpr int kStageSmall = 48;  // LDS-staged small pair records per wave
-constexpr int kStageLarge = 16;  // LDS-staged large pair records per wave
+constexpr int kStageSmall = 62;  // LDS-staged small pair records per wave
+constexpr int kStageLarge = 8;   // LDS-staged large pair records per wave (>= kPackGraphs)
+static_assert(kStageSmall >= kPackGraphs && kStageLarge >= kPackGraphs,
+              "a stage must hold one query's records over the pack's graphs");
 constexpr int kSegRecs = 64;     // most small records per segment (a flush writes one)
 constexpr int kRecDwords = 48;   // screening record of one query node
 constexpr int kPreDwords = 20;   // prefilter record of one query
@@ -271,8 +273,9 @@ __global__ void query_pre_kernel(const uint32_t* __restrict__ qhead, const uint3
 
 // HW: 2 for the compact shell histograms (8 labels per level), 4 for all 16 labels.  Six waves
 // per SIMD: the register cap (80 VGPRs) spills two 8-byte addresses used once per 64-query block
-// and once per flush, and 48 staged small records keep the LDS at 6.5 KB per wave when the large
-// records have at most 16 nodes (six waves fit the 160 KB).
+// and once per flush, and 62 staged small and 8 large records keep the LDS at 6.5 KB per wave
+// when the large records have at most 16 nodes (six waves fit the 160 KB).  Seven waves (72
+// VGPRs, 48 small records) spilled on the per-query path and ran 1 ms slower on 1M molecules.
 template <int K, int HW>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void screen_kernel(
     ScreenParams p) {
@@ -573,7 +576,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void
         ns = nrec;
       else
         nl = nrec;
-      if (ns > uint32_t(kStageSmall - kPackGraphs) || nl > uint32_t(kStageLarge - kPackGraphs)) flush();
+      // room for the next query's records (one per graph of the pack)
+      if (ns + ng > uint32_t(kStageSmall) || nl + ng > uint32_t(kStageLarge)) flush();
     }
     // GMCR bit rows of the block, and count 0 for every pair of the block outside the GMCR
     // (one coalesced 512-byte store per graph)
