@@ -27,7 +27,7 @@
 namespace bms2 {
 
 constexpr int kPackGraphs = 8;   // graphs per pack
-constexpr int kStageSmall = 62;  // LDS-staged small pair records per wave
+constexpr int kStageSmall = 59;  // LDS-staged small pair records per wave
 constexpr int kStageLarge = 8;   // LDS-staged large pair records per wave (>= kPackGraphs)
 static_assert(kStageSmall >= kPackGraphs && kStageLarge >= kPackGraphs,
               "a stage must hold one query's records over the pack's graphs");
@@ -273,9 +273,11 @@ __global__ void query_pre_kernel(const uint32_t* __restrict__ qhead, const uint3
 
 // HW: 2 for the compact shell histograms (8 labels per level), 4 for all 16 labels.  Six waves
 // per SIMD: the register cap (80 VGPRs) spills two 8-byte addresses used once per 64-query block
-// and once per flush, and 62 staged small and 8 large records keep the LDS at 6.5 KB per wave
-// when the large records have at most 16 nodes (six waves fit the 160 KB).  Seven waves (72
-// VGPRs, 48 small records) spilled on the per-query path and ran 1 ms slower on 1M molecules.
+// and once per flush, and 59 staged small and 8 large records keep the LDS at 6,384 bytes per
+// wave when the large records have at most 16 nodes: 24 waves per CU fit the 160 KB after the
+// allocation rounds up (62 records, 6,624 bytes, held the screen at 21 waves per CU, 28.9 against
+// 27.0 ms on 1M molecules).  Seven waves (72 VGPRs, 48 small records) spilled on the per-query
+// path and ran 1 ms slower.
 template <int K, int HW>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void screen_kernel(
     ScreenParams p) {
