@@ -108,6 +108,8 @@ struct JoinParams {
   uint64_t* ctl;
   uint32_t find_first;
   uint32_t sym;             // apply the symmetry breaking of qadj
+  uint32_t split;           // hand a long search's remaining root candidates to idle lanes
+                            // (count-all runs without traces: partial counts add up)
   uint64_t* qnodes;         // trace (nullable): [Q] DFS nodes
   uint32_t* pair_nodes;     // trace (nullable): [D][Q] DFS nodes per pair, saturating
 };
@@ -581,13 +583,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void
       // room for the next query's records (one per graph of the pack)
       if (ns + ng > uint32_t(kStageSmall) || nl + ng > uint32_t(kStageLarge)) flush();
     }
-    // GMCR bit rows of the block, and count 0 for every pair of the block outside the GMCR
-    // (one coalesced 512-byte store per graph)
+    // GMCR bit rows of the block, and count 0 for every pair of the block (one coalesced
+    // 512-byte store per graph; the joins overwrite or add to the GMCR pairs' counts)
     if (lane < ng) p.valid[uint64_t(slot_gid) * p.VW + (qb >> 6)] = vbits;
     for (uint32_t j = 0; j < ng; ++j) {
-      const uint64_t vb = readlane64(vbits, j);
       const uint32_t gj = __builtin_amdgcn_readlane(slot_gid, j);
-      if (qq < p.Q && !((vb >> lane) & 1)) p.counts[uint64_t(gj) * p.Q + qq] = 0;
+      if (qq < p.Q) p.counts[uint64_t(gj) * p.Q + qq] = 0;
     }
     vbits = 0;
   }
@@ -600,7 +601,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void
 // Depth-first join of the large-query records (more than kSmallNq nodes), one (query, graph)
 // pair per lane (isomorphism.hpp:291-472 semantics with standard backtracking); LDS stacks per
 // depth: remaining candidates R, matched node F and its neighbour row NF; domain words, query
-// adjacency and data adjacency rows read from global memory per step.
+// adjacency and data adjacency rows read from global memory per step.  Once a wave has no
+// records left, a lane whose search has run kSplitAfter steps hands its remaining root
+// candidates to idle lanes, one each (split runs): the pieces add their counts atomically to
+// the pair's count, which the screen zeroed, so one heavy pair no longer holds the kernel's
+// tail alone.
 template <int NQ>
 __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
   __shared__ uint64_t R[NQ][64];   // remaining candidates per depth
@@ -616,8 +621,10 @@ __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
   uint64_t rec = 0, used = 0, cnt = 0, nodes = 0;
   uint32_t q = 0, g = 0, n0 = 0, nq = 0, mult = 1;
   int depth = 0;
+  bool split = false;  // the pair's search is shared with other lanes
   uint64_t st_matched = 0, st_emb = 0, st_nodes = 0;
   const uint64_t* recs = p.recs;
+  constexpr uint64_t kSplitAfter = 32;
   // records claimed by this wave from the global cursor, handed to idle lanes without atomics
   uint64_t local_next = 0, local_end = 0;
   constexpr uint64_t kClaim = 256;
@@ -650,8 +657,49 @@ __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
         nodes = 0;
         R[0][lane] = recs[rec * RW + 2];
         mult = p.sym ? max(uint32_t(p.qadj[size_t(q) * 32] >> 40) & 63u, 1u) : 1u;
+        split = false;
       }
       local_next += take;
+      if (p.split && exhausted && local_next >= local_end) {
+        // no records left: long searches give their remaining root candidates to idle lanes
+        uint64_t free_l = ballot(!busy);
+        const uint64_t r0 = busy && depth > 0 ? R[0][lane] : 0;
+        uint64_t donors = ballot(r0 != 0 && nodes >= kSplitAfter);
+        while (free_l && donors) {
+          const int dl = int(ctz64(donors));
+          donors &= donors - 1;
+          uint64_t give = readlane64(r0, dl);
+          const uint64_t d_rec = readlane64(rec, dl);
+          const uint32_t d_q = __builtin_amdgcn_readlane(q, dl), d_g = __builtin_amdgcn_readlane(g, dl);
+          const uint32_t d_n0 = __builtin_amdgcn_readlane(n0, dl), d_nq = __builtin_amdgcn_readlane(nq, dl);
+          const uint32_t d_mult = __builtin_amdgcn_readlane(mult, dl);
+          while (give && free_l) {
+            const int rl = int(ctz64(free_l));
+            free_l &= free_l - 1;
+            const uint64_t bit = give & (~give + 1);
+            give &= give - 1;
+            if (lane == uint32_t(rl)) {
+              busy = true;
+              split = true;
+              rec = d_rec;
+              q = d_q;
+              g = d_g;
+              n0 = d_n0;
+              nq = d_nq;
+              mult = d_mult;
+              depth = 0;
+              used = 0;
+              cnt = 0;
+              nodes = 0;
+              R[0][lane] = bit;
+            }
+          }
+          if (lane == uint32_t(dl)) {
+            R[0][lane] = give;  // the candidates not handed out
+            split = true;
+          }
+        }
+      }
     }
     if (!ballot(busy)) {
       if (exhausted && local_next >= local_end) break;
@@ -701,10 +749,17 @@ __global__ __launch_bounds__(64) void join_kernel(JoinParams p) {
     }
     if (finished) {
       const uint64_t c = p.find_first ? (cnt ? 1 : 0) : cnt * mult;
-      p.counts[uint64_t(g) * p.Q + q] = c;
+      if (split) {
+        // the pair counts as matched once: when the first non-zero piece lands
+        const uint64_t old = atomicAdd(
+            reinterpret_cast<unsigned long long*>(p.counts + uint64_t(g) * p.Q + q), (unsigned long long)c);
+        st_matched += old == 0 && c != 0 ? 1 : 0;
+      } else {
+        p.counts[uint64_t(g) * p.Q + q] = c;
+        st_matched += c ? 1 : 0;
+      }
       if (p.qnodes) atomicAdd(reinterpret_cast<unsigned long long*>(p.qnodes + q), (unsigned long long)nodes);
       if (p.pair_nodes) p.pair_nodes[uint64_t(g) * p.Q + q] = uint32_t(min(nodes, uint64_t(0xffffffffu)));
-      st_matched += c ? 1 : 0;
       st_emb += c;
       st_nodes += nodes;
       busy = false;
@@ -2025,6 +2080,7 @@ int run_packs(bms2_pipeline p, uint32_t pack_end, uint32_t g_limit, uint64_t* d_
   // (count-all mode: in first-match mode the first match can come later under the constraint)
   jp.sym = p->dedup && !p->opt.find_first && !tr.domsum && !tr.qnodes && !tr.pair_nodes ? 1u : 0u;
   js.sym = jp.sym;
+  jp.split = !p->opt.find_first && !tr.domsum && !tr.qnodes && !tr.pair_nodes ? 1u : 0u;
   sp.hist_map = p->hist_map;
   const size_t stage_bytes = std::max<size_t>(
       (size_t(kStageSmall) * kSmallWords + size_t(kStageLarge) * p->rec_words_l) * 8,
